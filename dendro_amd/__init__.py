@@ -1,0 +1,10 @@
+"""dendro_amd: MI355X-native matrix-free octree FEM solve path of Dendro.
+
+The compute path is libdendrohip.so (HIP kernels for gfx950 behind the C ABI of
+include/dendro_hip.h); this package holds its build recipe and Python bindings.
+"""
+from .core import (  # noqa: F401
+    DAMG, F32, F64, Mesh, NORM_PRECONDITIONED, NORM_UNPRECONDITIONED, Octree, PTS_CLUSTERS,
+    PTS_GAUSS, PTS_LOGNORMAL, PTS_MIXED, PTS_UNIFORM, Solver, DendroError, Transfer,
+    default_params, etype, init, lib, points, stencils,
+)

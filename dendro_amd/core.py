@@ -1,0 +1,348 @@
+"""ctypes bindings of libdendrohip.so (include/dendro_hip.h).
+
+Host-side objects (octrees, the DA hierarchy) are plain numpy; device vectors are any
+objects exposing a device pointer (torch tensors on cuda, via .data_ptr()).  Every call
+checks the C status and raises DendroError with the library's message.  There is no
+fallback: if the shared library is missing, importing the bindings fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdendrohip.so")
+
+F64, F32 = 0, 1
+NORM_UNPRECONDITIONED, NORM_PRECONDITIONED = 0, 1
+SMOOTHER_JACOBI, SMOOTHER_LEARNED = 0, 1
+PTS_GAUSS, PTS_LOGNORMAL, PTS_MIXED, PTS_CLUSTERS, PTS_UNIFORM = range(5)
+
+# every symbol include/dendro_hip.h declares
+EXPORTS = [
+    "dgh_last_error", "dgh_etype", "dgh_stencils_generate", "dgh_points_generate",
+    "dgh_octree_from_points", "dgh_octree_from_leaves", "dgh_octree_balance", "dgh_octree_coarsen",
+    "dgh_octree_size", "dgh_octree_get", "dgh_octree_free",
+    "dgh_damg_build", "dgh_damg_num_levels", "dgh_damg_level_info", "dgh_damg_level_arrays",
+    "dgh_damg_level_octants", "dgh_damg_transfer_arrays", "dgh_damg_coarsen_coefficients",
+    "dgh_damg_free",
+    "dgh_init", "dgh_finalize",
+    "dgh_mesh_create", "dgh_mesh_set_coefficients", "dgh_mesh_info", "dgh_mesh_destroy",
+    "dgh_matvec", "dgh_diagonal", "dgh_matvec_multi",
+    "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add",
+    "dgh_solver_default_params", "dgh_solver_create", "dgh_solver_destroy", "dgh_solver_solve",
+    "dgh_solver_vcycle",
+]
+
+
+class DendroError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class LevelInfo(C.Structure):
+    _fields_ = [("n_elem", C.c_uint32), ("n_node", C.c_uint32), ("n_oct", C.c_uint32),
+                ("max_depth", C.c_uint32), ("n_hanging", C.c_uint32)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("omega", C.c_double), ("max_it", C.c_int),
+                ("nu_pre", C.c_int), ("nu_post", C.c_int), ("norm_type", C.c_int),
+                ("null_space", C.c_int), ("vcycle_prec", C.c_int), ("smoother", C.c_int),
+                ("use_graph", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("rnorm0", C.c_double),
+                ("rnorm", C.c_double), ("rel_true", C.c_double), ("seconds", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libdendrohip.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} not built: run python -m dendro_amd.build")
+        _lib = C.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            getattr(_lib, name).restype = C.c_int
+    return _lib
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    lib().dgh_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc):
+    if rc < 0:
+        raise DendroError(rc, last_error())
+    return rc
+
+
+def ptr(a):
+    """Host numpy array or device tensor -> void*."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags.c_contiguous
+        return C.c_void_p(a.ctypes.data)
+    if hasattr(a, "data_ptr"):
+        assert a.is_contiguous()
+        return C.c_void_p(a.data_ptr())
+    if isinstance(a, int):
+        return C.c_void_p(a)
+    raise TypeError(type(a))
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+# ------------------------------------------------------------------ host: tables, points
+def etype(child, mask):
+    return check(lib().dgh_etype(int(child), int(mask)))
+
+
+def stencils():
+    L = np.zeros(9216)
+    M = np.zeros(9216)
+    R2 = np.zeros(9216)
+    R1 = np.zeros(73728)
+    check(lib().dgh_stencils_generate(ptr(L), ptr(M), ptr(R2), ptr(R1)))
+    return {"lap2": L.reshape(8, 18, 8, 8), "mass2": M.reshape(8, 18, 8, 8),
+            "rmat2": R2.reshape(8, 18, 8, 8), "rmat1": R1.reshape(8, 8, 18, 8, 8)}
+
+
+def points(kind, n, seed, cluster_seed=0):
+    out = np.zeros((n, 3))
+    check(lib().dgh_points_generate(int(kind), C.c_uint64(n), C.c_uint64(seed),
+                                    C.c_uint64(cluster_seed), ptr(out)))
+    return out
+
+
+# ------------------------------------------------------------------ host: octrees
+class Octree:
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def from_points(cls, pts, max_depth, max_pts=1, inc_corner=True):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        h = C.c_void_p()
+        check(lib().dgh_octree_from_points(ptr(pts), C.c_uint64(pts.size // 3), int(max_depth),
+                                           int(max_pts), int(bool(inc_corner)), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_leaves(cls, anchors, levels, max_depth):
+        a = np.ascontiguousarray(anchors, dtype=np.uint32)
+        l = np.ascontiguousarray(levels, dtype=np.uint8)
+        h = C.c_void_p()
+        check(lib().dgh_octree_from_leaves(ptr(a), ptr(l), C.c_uint64(len(l)), int(max_depth),
+                                           C.byref(h)))
+        return cls(h)
+
+    def balance(self, inc_corner=True):
+        h = C.c_void_p()
+        check(lib().dgh_octree_balance(self.h, int(bool(inc_corner)), C.byref(h)))
+        return Octree(h)
+
+    def coarsen(self):
+        h = C.c_void_p()
+        check(lib().dgh_octree_coarsen(self.h, C.byref(h)))
+        return Octree(h)
+
+    def size(self):
+        n = C.c_uint64()
+        d = C.c_uint32()
+        check(lib().dgh_octree_size(self.h, C.byref(n), C.byref(d)))
+        return n.value, d.value
+
+    def leaves(self):
+        n, d = self.size()
+        a = np.zeros((n, 3), np.uint32)
+        l = np.zeros(n, np.uint8)
+        check(lib().dgh_octree_get(self.h, ptr(a), ptr(l)))
+        return a, l, d
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dgh_octree_free(self.h)
+            self.h = None
+
+
+class Level:
+    """Arrays of one DA level (host numpy)."""
+
+    def __init__(self, info, nlist, child, hnmask, lev, anchor):
+        self.n_elem = info.n_elem
+        self.n_node = info.n_node
+        self.n_oct = info.n_oct
+        self.max_depth = info.max_depth
+        self.n_hanging = info.n_hanging
+        self.nlist, self.child, self.hnmask, self.lev, self.anchor = nlist, child, hnmask, lev, anchor
+
+
+class DAMG:
+    """Host multigrid hierarchy of DAs (level 0 = coarsest)."""
+
+    def __init__(self, octree, nlevels, min_coarse=500):
+        h = C.c_void_p()
+        nl = C.c_int(int(nlevels))
+        check(lib().dgh_damg_build(octree.h, C.byref(nl), int(min_coarse), C.byref(h)))
+        self.h = h
+        self.nlevels = nl.value
+        self._levels = {}
+
+    def info(self, l):
+        li = LevelInfo()
+        check(lib().dgh_damg_level_info(self.h, int(l), C.byref(li)))
+        return li
+
+    def level(self, l):
+        if l not in self._levels:
+            li = self.info(l)
+            E = li.n_elem
+            nlist = np.zeros((E, 8), np.uint32)
+            child = np.zeros(E, np.uint8)
+            hn = np.zeros(E, np.uint8)
+            lev = np.zeros(E, np.uint8)
+            anc = np.zeros((E, 3), np.uint32)
+            check(lib().dgh_damg_level_arrays(self.h, int(l), ptr(nlist), ptr(child), ptr(hn),
+                                              ptr(lev), ptr(anc)))
+            self._levels[l] = Level(li, nlist, child, hn, lev, anc)
+        return self._levels[l]
+
+    def octants(self, l):
+        li = self.info(l)
+        anc = np.zeros((li.n_oct, 3), np.uint32)
+        lev = np.zeros(li.n_oct, np.uint8)
+        fl = np.zeros(li.n_oct, np.uint8)
+        check(lib().dgh_damg_level_octants(self.h, int(l), ptr(anc), ptr(lev), ptr(fl)))
+        return anc, lev, fl
+
+    def transfer(self, fine_level):
+        li = self.info(fine_level)
+        ce = np.zeros(li.n_node, np.uint32)
+        pos = np.zeros(li.n_node, np.uint8)
+        check(lib().dgh_damg_transfer_arrays(self.h, int(fine_level), ptr(ce), ptr(pos)))
+        return ce, pos
+
+    def coarsen_coefficients(self, kappa, mass, level):
+        li = self.info(level)
+        k = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+        m = None if mass is None else np.ascontiguousarray(mass, np.float64)
+        ok = np.zeros(li.n_elem) if k is not None else None
+        om = np.zeros(li.n_elem) if m is not None else None
+        check(lib().dgh_damg_coarsen_coefficients(self.h, ptr(k), ptr(m), int(level), ptr(ok), ptr(om)))
+        return ok, om
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dgh_damg_free(self.h)
+            self.h = None
+
+
+# ------------------------------------------------------------------ device
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    check(lib().dgh_init(int(device)))
+    _initialised = True
+
+
+class Mesh:
+    """One DA level uploaded to the device."""
+
+    def __init__(self, level):
+        self.level = level
+        h = C.c_void_p()
+        check(lib().dgh_mesh_create(level.n_elem, level.n_node, level.max_depth, ptr(level.nlist),
+                                    ptr(level.child), ptr(level.hnmask), ptr(level.lev), C.byref(h)))
+        self.h = h
+        self.n_elem, self.n_node = level.n_elem, level.n_node
+
+    def set_coefficients(self, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0):
+        k = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+        m = None if mass is None else np.ascontiguousarray(mass, np.float64)
+        check(lib().dgh_mesh_set_coefficients(self.h, ptr(k), ptr(m), C.c_double(lap_fac),
+                                              C.c_double(mass_fac)))
+
+    def matvec(self, x, y, prec=F64, stream=None):
+        check(lib().dgh_matvec(self.h, int(prec), ptr(x), ptr(y), _stream(stream)))
+
+    def diagonal(self, d, prec=F64, stream=None):
+        check(lib().dgh_diagonal(self.h, int(prec), ptr(d), _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dgh_mesh_destroy(self.h)
+            self.h = None
+
+
+class Transfer:
+    def __init__(self, coarse, fine, celem, pos):
+        self.coarse, self.fine = coarse, fine   # keep the meshes alive
+        h = C.c_void_p()
+        check(lib().dgh_transfer_create(coarse.h, fine.h, ptr(np.ascontiguousarray(celem, np.uint32)),
+                                        ptr(np.ascontiguousarray(pos, np.uint8)), C.byref(h)))
+        self.h = h
+
+    def restrict(self, f, c, prec=F64, stream=None):
+        check(lib().dgh_restrict(self.h, int(prec), ptr(f), ptr(c), _stream(stream)))
+
+    def prolong_add(self, c, f_in, f_out, prec=F64, stream=None):
+        check(lib().dgh_prolong_add(self.h, int(prec), ptr(c), ptr(f_in), ptr(f_out), _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dgh_transfer_destroy(self.h)
+            self.h = None
+
+
+def default_params(**kw):
+    p = SolverParams()
+    check(lib().dgh_solver_default_params(C.byref(p)))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Solver:
+    def __init__(self, meshes, transfers, params):
+        self.meshes, self.transfers = list(meshes), list(transfers)
+        self.params = params
+        mh = (C.c_void_p * len(meshes))(*[m.h.value for m in meshes])
+        th = (C.c_void_p * max(1, len(transfers)))(*([t.h.value for t in transfers] or [0]))
+        h = C.c_void_p()
+        check(lib().dgh_solver_create(len(meshes), mh, th if transfers else None, C.byref(params),
+                                      C.byref(h)))
+        self.h = h
+
+    def solve(self, b, x, history_len=0, stream=None):
+        st = SolveStats()
+        hist = np.zeros(max(1, history_len))
+        check(lib().dgh_solver_solve(self.h, ptr(b), ptr(x), C.byref(st),
+                                     ptr(hist) if history_len else None, int(history_len),
+                                     _stream(stream)))
+        return st, hist[: min(history_len, st.iterations + 1)]
+
+    def vcycle(self, r, z, stream=None):
+        check(lib().dgh_solver_vcycle(self.h, ptr(r), ptr(z), _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dgh_solver_destroy(self.h)
+            self.h = None

@@ -1,0 +1,749 @@
+// Device side of libdendrohip: mesh upload, operator application, intergrid transfer and
+// the GMG-preconditioned CG solver, behind the C ABI of include/dendro_hip.h.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../host/octree.hpp"
+#include "kernels.hpp"
+#include "vector_kernels.hpp"
+
+namespace dgh {
+
+#define HIP_CHECK(expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) fail(DGH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------- device buffers
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    release();
+    p = o.p;
+    n = o.n;
+    o.p = nullptr;
+    o.n = 0;
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count) {
+      if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+        p = nullptr;
+        fail(DGH_ERR_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+      }
+    }
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T* h, size_t count) {
+    alloc(count);
+    if (count) HIP_CHECK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+};
+
+static inline unsigned blocks_for(size_t n, int bs = VBLOCK) { return unsigned((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------- library state
+struct State {
+  bool ready = false;
+  int device = 0;
+  DevBuf<double> L64, M64, W64;   // [144][64], [144][64], [144][27][8]
+  DevBuf<float> L32, M32, W32;
+  rocblas_handle blas = nullptr;
+};
+static State g;
+
+static void require_init() {
+  if (!g.ready) fail(DGH_ERR_STATE, "dgh_init has not been called");
+}
+
+template <class T> const T* Ltab();
+template <class T> const T* Mtab();
+template <class T> const T* Wtab();
+template <> const double* Ltab<double>() { return g.L64.p; }
+template <> const double* Mtab<double>() { return g.M64.p; }
+template <> const double* Wtab<double>() { return g.W64.p; }
+template <> const float* Ltab<float>() { return g.L32.p; }
+template <> const float* Mtab<float>() { return g.M32.p; }
+template <> const float* Wtab<float>() { return g.W32.p; }
+
+}  // namespace dgh
+
+using namespace dgh;
+
+// ---------------------------------------------------------------- mesh
+struct dgh_mesh_s {
+  uint32_t E = 0, N = 0, maxD = 0;
+  double hFac = 1.0;
+  std::vector<uint32_t> perm;   // perm[original element] = device position
+  std::vector<TileInfo> h_tiles;
+  DevBuf<TileInfo> tiles;
+  DevBuf<uint32_t> nl;          // [E][8] device order
+  DevBuf<uint8_t> lev;          // [E]
+  DevBuf<uint8_t> cs;           // [E] stencil index 18c+t per element
+  DevBuf<double> kappa, mass;   // [E] device order (Jacobian2) or empty
+  double lap_fac = 1.0, mass_fac = 1.0;
+  // host copies of the small per-element arrays (device order) for assembly
+  std::vector<uint32_t> h_nl;
+  std::vector<uint8_t> h_lev, h_cs;
+  std::vector<double> h_kappa, h_mass;
+};
+
+struct dgh_transfer_s {
+  dgh_mesh coarse = nullptr, fine = nullptr;
+  DevBuf<uint32_t> celem;   // [N_f] coarse element (device order)
+  DevBuf<uint8_t> pos;      // [N_f]
+};
+
+namespace dgh {
+
+template <class T>
+static void launch_matvec(dgh_mesh m, const T* x, T* y, hipStream_t st) {
+  HIP_CHECK(hipMemsetAsync(y, 0, sizeof(T) * m->N, st));
+  const int nt = int(m->h_tiles.size());
+  if (!nt) return;
+  const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
+  hipLaunchKernelGGL(matvec_kernel<T>, dim3(grid), dim3(TILE), 0, st, m->tiles.p, nt, m->nl.p,
+                     m->lev.p, m->kappa.p, m->mass.p, m->lap_fac, m->mass_fac, m->maxD, m->hFac,
+                     Ltab<T>(), Mtab<T>(), x, y);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <class T>
+static void launch_diag(dgh_mesh m, T* d, hipStream_t st) {
+  HIP_CHECK(hipMemsetAsync(d, 0, sizeof(T) * m->N, st));
+  const int nt = int(m->h_tiles.size());
+  if (!nt) return;
+  const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
+  hipLaunchKernelGGL(diag_kernel<T>, dim3(grid), dim3(TILE), 0, st, m->tiles.p, nt, m->nl.p,
+                     m->lev.p, m->kappa.p, m->mass.p, m->lap_fac, m->mass_fac, m->maxD, m->hFac,
+                     Ltab<T>(), Mtab<T>(), d);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <class T>
+static void launch_restrict(dgh_transfer t, const T* f, T* c, hipStream_t st) {
+  HIP_CHECK(hipMemsetAsync(c, 0, sizeof(T) * t->coarse->N, st));
+  const uint32_t nf = t->fine->N;
+  if (!nf) return;
+  hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(nf)), dim3(VBLOCK), 0, st, nf, t->celem.p,
+                     t->pos.p, t->coarse->nl.p, t->coarse->cs.p, Wtab<T>(), f, c);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <class T>
+static void launch_prolong(dgh_transfer t, const T* c, const T* f_in, T* f_out, hipStream_t st) {
+  const uint32_t nf = t->fine->N;
+  if (!nf) return;
+  hipLaunchKernelGGL(prolong_kernel<T>, dim3(blocks_for(nf)), dim3(VBLOCK), 0, st, nf, t->celem.p,
+                     t->pos.p, t->coarse->nl.p, t->coarse->cs.p, Wtab<T>(), c, f_in, f_out);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace dgh
+
+// ---------------------------------------------------------------- solver
+namespace dgh {
+
+template <class T>
+struct LevelVecs {
+  DevBuf<T> invd, b, x, r, y;
+};
+
+}  // namespace dgh
+
+struct dgh_solver_s {
+  int nlevels = 0;
+  std::vector<dgh_mesh> meshes;
+  std::vector<dgh_transfer> transfers;
+  dgh_solver_params prm{};
+  std::vector<LevelVecs<double>> v64;
+  std::vector<LevelVecs<float>> v32;
+  int nc = 0;                     // coarsest size
+  DevBuf<double> Ainv64;          // dense coarsest inverse
+  DevBuf<float> Ainv32;
+  DevBuf<double> scal;            // device scalars (dot results)
+  // outer CG vectors (finest, FP64)
+  DevBuf<double> r, z, p, w;
+};
+
+namespace dgh {
+
+static void project_mean_dev(uint32_t n, double* v, double* scratch, hipStream_t st) {
+  HIP_CHECK(hipMemsetAsync(scratch, 0, sizeof(double) * 3, st));
+  hipLaunchKernelGGL(dot3_kernel<double>, dim3(std::min(1024u, blocks_for(n))), dim3(VBLOCK), 0, st, n,
+                     v, (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
+                     (const double*)nullptr, (const double*)nullptr, scratch);
+  hipLaunchKernelGGL(subtract_mean_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n,
+                     (const double*)scratch, v);
+}
+static void project_mean_dev(uint32_t n, float* v, double* scratch, hipStream_t st) {
+  HIP_CHECK(hipMemsetAsync(scratch, 0, sizeof(double) * 3, st));
+  hipLaunchKernelGGL(dot3_kernel<float>, dim3(std::min(1024u, blocks_for(n))), dim3(VBLOCK), 0, st, n,
+                     v, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr,
+                     (const float*)nullptr, (const float*)nullptr, scratch);
+  hipLaunchKernelGGL(subtract_mean_kernel<float>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n,
+                     (const double*)scratch, v);
+}
+
+template <class T> std::vector<LevelVecs<T>>& vecs(dgh_solver s);
+template <> std::vector<LevelVecs<double>>& vecs<double>(dgh_solver s) { return s->v64; }
+template <> std::vector<LevelVecs<float>>& vecs<float>(dgh_solver s) { return s->v32; }
+template <class T> const T* ainv(dgh_solver s);
+template <> const double* ainv<double>(dgh_solver s) { return s->Ainv64.p; }
+template <> const float* ainv<float>(dgh_solver s) { return s->Ainv32.p; }
+
+// Richardson/Jacobi sweeps on level l (see KSPSolve_Richardson restatement in the oracle).
+template <class T>
+static void smooth(dgh_solver s, int l, const T* b, T* x, int nu, bool guess_zero, hipStream_t st) {
+  auto& V = vecs<T>(s)[l];
+  dgh_mesh m = s->meshes[l];
+  const uint32_t n = m->N;
+  const T om = T(s->prm.omega);
+  for (int it = 0; it < nu; ++it) {
+    if (it == 0 && guess_zero) {
+      hipLaunchKernelGGL(smooth_first_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, om, b,
+                         (const T*)V.invd.p, x);
+    } else {
+      launch_matvec<T>(m, x, V.y.p, st);
+      hipLaunchKernelGGL(smooth_update_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, om, b,
+                         (const T*)V.y.p, (const T*)V.invd.p, x);
+    }
+  }
+}
+
+// PCMG multiplicative V-cycle on level l: x = B_l b (x overwritten).
+template <class T>
+static void vcycle(dgh_solver s, int l, const T* b, T* x, hipStream_t st) {
+  auto& VV = vecs<T>(s);
+  if (l == 0) {
+    const int n = s->nc;
+    const T* bb = b;
+    if (s->prm.null_space) {
+      HIP_CHECK(hipMemcpyAsync(VV[0].r.p, b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+      project_mean_dev(uint32_t(n), VV[0].r.p, s->scal.p + 4, st);
+      bb = VV[0].r.p;
+    }
+    hipLaunchKernelGGL(dense_apply_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, ainv<T>(s), bb, x);
+    return;
+  }
+  auto& V = VV[l];
+  auto& Vc = VV[l - 1];
+  dgh_mesh m = s->meshes[l];
+  const uint32_t n = m->N;
+  smooth<T>(s, l, b, x, s->prm.nu_pre, true, st);
+  launch_matvec<T>(m, x, V.y.p, st);
+  hipLaunchKernelGGL(residual_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, b,
+                     (const T*)V.y.p, V.r.p);
+  launch_restrict<T>(s->transfers[l - 1], V.r.p, Vc.b.p, st);
+  vcycle<T>(s, l - 1, Vc.b.p, Vc.x.p, st);
+  launch_prolong<T>(s->transfers[l - 1], Vc.x.p, x, x, st);
+  smooth<T>(s, l, b, x, s->prm.nu_post, false, st);
+}
+
+// z = B r at the finest level, FP64 in and out.
+static void precondition(dgh_solver s, const double* r, double* z, hipStream_t st) {
+  const int top = s->nlevels - 1;
+  const uint32_t n = s->meshes[top]->N;
+  if (s->prm.vcycle_prec == DGH_F32) {
+    auto& V = s->v32[top];
+    // the finest level's b and x buffers of the FP32 cycle carry r and z
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, r, V.b.p);
+    vcycle<float>(s, top, V.b.p, V.x.p, st);
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n,
+                       (const float*)V.x.p, z);
+  } else {
+    vcycle<double>(s, top, r, z, st);
+  }
+  if (s->prm.null_space) project_mean_dev(n, z, s->scal.p + 4, st);
+}
+
+static void dots(uint32_t n, const double* a0, const double* b0, const double* a1, const double* b1,
+                 double* dev_out, double* host_out, int cnt, hipStream_t st) {
+  HIP_CHECK(hipMemsetAsync(dev_out, 0, sizeof(double) * 3, st));
+  hipLaunchKernelGGL(dot3_kernel<double>, dim3(std::min(1024u, blocks_for(n))), dim3(VBLOCK), 0, st, n,
+                     a0, b0, a1, b1, (const double*)nullptr, (const double*)nullptr, dev_out);
+  HIP_CHECK(hipMemcpyAsync(host_out, dev_out, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// Assembles the coarsest operator from its element blocks and inverts it on the device
+// (Cholesky, rocSOLVER).  With a constant null space, A + a 11^T (a = trace / n^2).
+static void setup_coarse(dgh_solver s) {
+  dgh_mesh m = s->meshes[0];
+  const int n = int(m->N);
+  s->nc = n;
+  if (n > 40000) fail(DGH_ERR_ARG, "coarsest level has %d nodes; use more levels (dense solve limit 40000)", n);
+  std::vector<double> L(9216), M(9216);
+  generate_stencils(L.data(), M.data(), nullptr, nullptr);
+  std::vector<double> A(size_t(n) * n, 0.0);
+  const double hFac = m->hFac;
+  for (uint32_t e = 0; e < m->E; ++e) {
+    const double h = hFac * double(1u << (m->maxD - m->h_lev[e]));
+    const double k1 = m->h_kappa.empty() ? m->lap_fac : m->h_kappa[e];
+    const double k2 = m->h_kappa.empty() ? m->mass_fac : (m->h_mass.empty() ? 0.0 : m->h_mass[e]);
+    const double fac1 = k1 * h / 2.0, fac2 = k2 * h * h * h / 8.0;
+    const double* Ls = L.data() + size_t(m->h_cs[e]) * 64;
+    const double* Ms = M.data() + size_t(m->h_cs[e]) * 64;
+    const uint32_t* idx = m->h_nl.data() + size_t(e) * 8;
+    for (int k = 0; k < 8; ++k)
+      for (int j = 0; j < 8; ++j) A[size_t(idx[k]) * n + idx[j]] += fac1 * Ls[k * 8 + j] + fac2 * Ms[k * 8 + j];
+  }
+  if (s->prm.null_space) {
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += A[size_t(i) * n + i];
+    const double a = tr / (double(n) * n);
+    for (double& v : A) v += a;
+  }
+  s->Ainv64.upload(A.data(), A.size());
+  if (!g.blas) {
+    if (rocblas_create_handle(&g.blas) != rocblas_status_success) fail(DGH_ERR_HIP, "rocblas_create_handle failed");
+  }
+  DevBuf<rocblas_int> info(1);
+  if (rocsolver_dpotrf(g.blas, rocblas_fill_lower, n, s->Ainv64.p, n, info.p) != rocblas_status_success)
+    fail(DGH_ERR_HIP, "rocsolver_dpotrf failed");
+  rocblas_int hinfo = 0;
+  HIP_CHECK(hipMemcpy(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost));
+  if (hinfo != 0) fail(DGH_ERR_DIVERGED, "coarsest operator is not positive definite (pivot %d)", int(hinfo));
+  if (rocsolver_dpotri(g.blas, rocblas_fill_lower, n, s->Ainv64.p, n, info.p) != rocblas_status_success)
+    fail(DGH_ERR_HIP, "rocsolver_dpotri failed");
+  HIP_CHECK(hipMemcpy(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost));
+  if (hinfo != 0) fail(DGH_ERR_DIVERGED, "coarsest inverse failed (%d)", int(hinfo));
+  hipLaunchKernelGGL(symmetrize_kernel<double>, dim3((n + 255) / 256, n), dim3(256), 0, 0, n, s->Ainv64.p);
+  HIP_CHECK(hipGetLastError());
+  if (s->prm.vcycle_prec == DGH_F32) {
+    s->Ainv32.alloc(size_t(n) * n);
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(size_t(n) * n)), dim3(VBLOCK), 0, 0,
+                       uint32_t(size_t(n) * n), (const double*)s->Ainv64.p, s->Ainv32.p);
+  }
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
+}  // namespace dgh
+
+// ================================================================ extern "C"
+extern "C" {
+
+int dgh_init(int device) {
+  return guarded([&] {
+    HIP_CHECK(hipSetDevice(device));
+    g.device = device;
+    std::vector<double> L(9216), M(9216), W(144 * 27 * 8);
+    generate_stencils(L.data(), M.data(), nullptr, nullptr);
+    generate_transfer_weights(W.data());
+    g.L64.upload(L.data(), L.size());
+    g.M64.upload(M.data(), M.size());
+    g.W64.upload(W.data(), W.size());
+    std::vector<float> Lf(L.begin(), L.end()), Mf(M.begin(), M.end()), Wf(W.begin(), W.end());
+    g.L32.upload(Lf.data(), Lf.size());
+    g.M32.upload(Mf.data(), Mf.size());
+    g.W32.upload(Wf.data(), Wf.size());
+    g.ready = true;
+  });
+}
+
+int dgh_finalize(void) {
+  return guarded([&] {
+    g.L64.release();
+    g.M64.release();
+    g.W64.release();
+    g.L32.release();
+    g.M32.release();
+    g.W32.release();
+    if (g.blas) rocblas_destroy_handle(g.blas);
+    g.blas = nullptr;
+    g.ready = false;
+  });
+}
+
+int dgh_mesh_create(uint32_t n_elem, uint32_t n_node, uint32_t max_depth, const uint32_t* nlist,
+                    const uint8_t* child, const uint8_t* hn_mask, const uint8_t* lev, dgh_mesh* out) {
+  return guarded([&] {
+    require_init();
+    if (!out || (n_elem && (!nlist || !child || !hn_mask || !lev))) fail(DGH_ERR_ARG, "NULL argument");
+    if (max_depth < 2 || max_depth > 21) fail(DGH_ERR_ARG, "max_depth %u out of range", max_depth);
+    auto m = std::make_unique<dgh_mesh_s>();
+    m->E = n_elem;
+    m->N = n_node;
+    m->maxD = max_depth;
+    m->hFac = 1.0 / double(1u << (max_depth - 1));
+    // stencil index and class of every element
+    std::vector<uint8_t> cs(n_elem);
+    std::vector<uint32_t> count(144, 0);
+    for (uint32_t e = 0; e < n_elem; ++e) {
+      const int t = etype_of(child[e], hn_mask[e]);
+      if (t < 0) fail(DGH_ERR_MESH, "element %u: hanging mask %u invalid for child %u", e, hn_mask[e], child[e]);
+      if (lev[e] < 1 || lev[e] >= max_depth + 1) fail(DGH_ERR_MESH, "element %u: level %u out of range", e, lev[e]);
+      cs[e] = uint8_t(18 * child[e] + t);
+      count[t == 0 ? 0 : cs[e]]++;
+      for (int k = 0; k < 8; ++k)
+        if (nlist[size_t(e) * 8 + k] >= n_node) fail(DGH_ERR_MESH, "element %u: node index out of range", e);
+    }
+    // class offsets and tiles
+    std::vector<uint32_t> off(145, 0);
+    for (int c = 0; c < 144; ++c) off[c + 1] = off[c] + count[c];
+    for (int c = 0; c < 144; ++c) {
+      for (uint32_t s = off[c]; s < off[c + 1]; s += TILE)
+        m->h_tiles.push_back(TileInfo{s, std::min<uint32_t>(TILE, off[c + 1] - s), uint32_t(c), 0});
+    }
+    m->perm.resize(n_elem);
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+    for (uint32_t e = 0; e < n_elem; ++e) {
+      const int cls = (hn_mask[e] == 0) ? 0 : cs[e];
+      m->perm[e] = fill[cls]++;
+    }
+    m->h_nl.resize(size_t(n_elem) * 8);
+    m->h_lev.resize(n_elem);
+    m->h_cs.resize(n_elem);
+    for (uint32_t e = 0; e < n_elem; ++e) {
+      const uint32_t d = m->perm[e];
+      std::memcpy(&m->h_nl[size_t(d) * 8], nlist + size_t(e) * 8, 32);
+      m->h_lev[d] = lev[e];
+      m->h_cs[d] = cs[e];
+    }
+    m->nl.upload(m->h_nl.data(), m->h_nl.size());
+    m->lev.upload(m->h_lev.data(), n_elem);
+    m->cs.upload(m->h_cs.data(), n_elem);
+    m->tiles.upload(m->h_tiles.data(), m->h_tiles.size());
+    *out = m.release();
+  });
+}
+
+int dgh_mesh_set_coefficients(dgh_mesh m, const double* kappa, const double* mass, double lap_fac,
+                              double mass_fac) {
+  return guarded([&] {
+    require_init();
+    if (!m) fail(DGH_ERR_ARG, "NULL mesh");
+    m->lap_fac = lap_fac;
+    m->mass_fac = mass_fac;
+    m->kappa.release();
+    m->mass.release();
+    m->h_kappa.clear();
+    m->h_mass.clear();
+    if (kappa) {
+      m->h_kappa.resize(m->E);
+      for (uint32_t e = 0; e < m->E; ++e) m->h_kappa[m->perm[e]] = kappa[e];
+      m->kappa.upload(m->h_kappa.data(), m->E);
+      if (mass) {
+        m->h_mass.resize(m->E);
+        for (uint32_t e = 0; e < m->E; ++e) m->h_mass[m->perm[e]] = mass[e];
+        m->mass.upload(m->h_mass.data(), m->E);
+      }
+    }
+  });
+}
+
+int dgh_mesh_info(dgh_mesh m, uint32_t* n_elem, uint32_t* n_node) {
+  return guarded([&] {
+    if (!m) fail(DGH_ERR_ARG, "NULL mesh");
+    if (n_elem) *n_elem = m->E;
+    if (n_node) *n_node = m->N;
+  });
+}
+
+int dgh_mesh_destroy(dgh_mesh m) {
+  delete m;
+  return DGH_OK;
+}
+
+int dgh_matvec(dgh_mesh m, int prec, const void* x, void* y, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!m || !x || !y) fail(DGH_ERR_ARG, "NULL argument");
+    if (x == y) fail(DGH_ERR_ARG, "x and y must not alias");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (prec == DGH_F64)
+      launch_matvec<double>(m, static_cast<const double*>(x), static_cast<double*>(y), st);
+    else if (prec == DGH_F32)
+      launch_matvec<float>(m, static_cast<const float*>(x), static_cast<float*>(y), st);
+    else
+      fail(DGH_ERR_ARG, "unknown precision %d", prec);
+  });
+}
+
+int dgh_diagonal(dgh_mesh m, int prec, void* d, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!m || !d) fail(DGH_ERR_ARG, "NULL argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (prec == DGH_F64)
+      launch_diag<double>(m, static_cast<double*>(d), st);
+    else if (prec == DGH_F32)
+      launch_diag<float>(m, static_cast<float*>(d), st);
+    else
+      fail(DGH_ERR_ARG, "unknown precision %d", prec);
+  });
+}
+
+int dgh_matvec_multi(dgh_mesh m, int prec, int k, const void* x, void* y, void* stream) {
+  return guarded([&] {
+    (void)m; (void)prec; (void)k; (void)x; (void)y; (void)stream;
+    fail(DGH_ERR_STATE, "dgh_matvec_multi is not available in this build");
+  });
+}
+
+int dgh_transfer_create(dgh_mesh coarse, dgh_mesh fine, const uint32_t* coarse_elem, const uint8_t* pos,
+                        dgh_transfer* out) {
+  return guarded([&] {
+    require_init();
+    if (!coarse || !fine || !out || (fine->N && (!coarse_elem || !pos))) fail(DGH_ERR_ARG, "NULL argument");
+    auto t = std::make_unique<dgh_transfer_s>();
+    t->coarse = coarse;
+    t->fine = fine;
+    std::vector<uint32_t> ce(fine->N);
+    for (uint32_t j = 0; j < fine->N; ++j) {
+      if (coarse_elem[j] >= coarse->E) fail(DGH_ERR_MESH, "fine node %u: coarse element out of range", j);
+      if (pos[j] >= 27) fail(DGH_ERR_MESH, "fine node %u: position code out of range", j);
+      ce[j] = coarse->perm[coarse_elem[j]];
+    }
+    t->celem.upload(ce.data(), ce.size());
+    t->pos.upload(pos, fine->N);
+    *out = t.release();
+  });
+}
+
+int dgh_transfer_destroy(dgh_transfer t) {
+  delete t;
+  return DGH_OK;
+}
+
+int dgh_restrict(dgh_transfer t, int prec, const void* f, void* c, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!t || !f || !c) fail(DGH_ERR_ARG, "NULL argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (prec == DGH_F64)
+      launch_restrict<double>(t, static_cast<const double*>(f), static_cast<double*>(c), st);
+    else if (prec == DGH_F32)
+      launch_restrict<float>(t, static_cast<const float*>(f), static_cast<float*>(c), st);
+    else
+      fail(DGH_ERR_ARG, "unknown precision %d", prec);
+  });
+}
+
+int dgh_prolong_add(dgh_transfer t, int prec, const void* c, const void* f_in, void* f_out, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!t || !c || !f_out) fail(DGH_ERR_ARG, "NULL argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (prec == DGH_F64)
+      launch_prolong<double>(t, static_cast<const double*>(c), static_cast<const double*>(f_in),
+                             static_cast<double*>(f_out), st);
+    else if (prec == DGH_F32)
+      launch_prolong<float>(t, static_cast<const float*>(c), static_cast<const float*>(f_in),
+                            static_cast<float*>(f_out), st);
+    else
+      fail(DGH_ERR_ARG, "unknown precision %d", prec);
+  });
+}
+
+int dgh_solver_default_params(dgh_solver_params* p) {
+  if (!p) return DGH_ERR_ARG;
+  std::memset(p, 0, sizeof(*p));
+  p->rtol = 1e-8;         // -ksp_rtol
+  p->omega = 0.667;       // -mg_levels_ksp_richardson_scale
+  p->max_it = 10;         // -ksp_max_it
+  p->nu_pre = 4;          // -mg_levels_ksp_max_it
+  p->nu_post = 4;
+  p->norm_type = DGH_NORM_PRECONDITIONED;
+  p->null_space = 0;
+  p->vcycle_prec = DGH_F64;
+  p->smoother = DGH_SMOOTHER_JACOBI;
+  p->use_graph = 0;
+  return DGH_OK;
+}
+
+int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* transfers,
+                      const dgh_solver_params* params, dgh_solver* out) {
+  return guarded([&] {
+    require_init();
+    if (nlevels < 1 || !meshes || !params || !out || (nlevels > 1 && !transfers))
+      fail(DGH_ERR_ARG, "invalid arguments");
+    if (params->nu_pre < 1 || params->nu_post < 0 || params->max_it < 0 || !(params->omega > 0))
+      fail(DGH_ERR_ARG, "invalid smoother parameters");
+    if (params->vcycle_prec != DGH_F64 && params->vcycle_prec != DGH_F32) fail(DGH_ERR_ARG, "bad vcycle_prec");
+    if (params->smoother != DGH_SMOOTHER_JACOBI) fail(DGH_ERR_STATE, "learned smoother not available in this build");
+    auto s = std::make_unique<dgh_solver_s>();
+    s->nlevels = nlevels;
+    s->prm = *params;
+    s->meshes.assign(meshes, meshes + nlevels);
+    if (nlevels > 1) s->transfers.assign(transfers, transfers + nlevels - 1);
+    for (int l = 0; l < nlevels; ++l) {
+      if (!meshes[l]) fail(DGH_ERR_ARG, "NULL mesh at level %d", l);
+      if (l > 0 && (s->transfers[l - 1]->coarse != meshes[l - 1] || s->transfers[l - 1]->fine != meshes[l]))
+        fail(DGH_ERR_ARG, "transfer %d does not connect levels %d and %d", l - 1, l - 1, l);
+    }
+    s->scal.alloc(8);
+    // per-level vectors and Jacobi diagonals
+    s->v64.resize(nlevels);
+    if (params->vcycle_prec == DGH_F32) s->v32.resize(nlevels);
+    for (int l = 0; l < nlevels; ++l) {
+      const uint32_t n = meshes[l]->N;
+      auto& V = s->v64[l];
+      V.invd.alloc(n);
+      V.b.alloc(n);
+      V.x.alloc(n);
+      V.r.alloc(n);
+      V.y.alloc(n);
+      launch_diag<double>(meshes[l], V.y.p, 0);
+      hipLaunchKernelGGL(reciprocal_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, 0, n,
+                         (const double*)V.y.p, V.invd.p);
+      if (params->vcycle_prec == DGH_F32) {
+        auto& F = s->v32[l];
+        F.invd.alloc(n);
+        F.b.alloc(n);
+        F.x.alloc(n);
+        F.r.alloc(n);
+        F.y.alloc(n);
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n)), dim3(VBLOCK), 0, 0, n,
+                           (const double*)V.invd.p, F.invd.p);
+      }
+    }
+    HIP_CHECK(hipDeviceSynchronize());
+    setup_coarse(s.get());
+    const uint32_t nf = meshes[nlevels - 1]->N;
+    s->r.alloc(nf);
+    s->z.alloc(nf);
+    s->p.alloc(nf);
+    s->w.alloc(nf);
+    *out = s.release();
+  });
+}
+
+int dgh_solver_destroy(dgh_solver s) {
+  delete s;
+  return DGH_OK;
+}
+
+int dgh_solver_vcycle(dgh_solver s, const double* r, double* z, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!s || !r || !z) fail(DGH_ERR_ARG, "NULL argument");
+    precondition(s, r, z, static_cast<hipStream_t>(stream));
+  });
+}
+
+// PETSc KSPSolve_CG restated (zero initial guess, left preconditioning); in the FP32-V-cycle
+// mode the preconditioner is inexact and beta takes the flexible (Polak-Ribiere) form
+// z_k.(r_k - r_{k-1}) / z_{k-1}.r_{k-1}.
+int dgh_solver_solve(dgh_solver s, const double* b, double* x, dgh_solve_stats* stats, double* history,
+                     int history_len, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!s || !b || !x) fail(DGH_ERR_ARG, "NULL argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int top = s->nlevels - 1;
+    dgh_mesh m = s->meshes[top];
+    const uint32_t n = m->N;
+    const bool flexible = (s->prm.vcycle_prec == DGH_F32);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, st));
+    double* sc = s->scal.p;
+    double hs[3];
+    // r = b (projected), x = 0
+    HIP_CHECK(hipMemcpyAsync(s->r.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    if (s->prm.null_space) project_mean_dev(n, s->r.p, sc + 4, st);
+    HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+    dots(n, s->r.p, s->r.p, nullptr, nullptr, sc, hs, 1, st);
+    const double bnorm = std::sqrt(hs[0]);
+    precondition(s, s->r.p, s->z.p, st);
+    dots(n, s->z.p, s->z.p, s->z.p, s->r.p, sc, hs, 2, st);
+    double dp = std::sqrt(s->prm.norm_type == DGH_NORM_PRECONDITIONED ? hs[0] : bnorm * bnorm);
+    double rz = hs[1], ratio = 0.0;
+    const double dp0 = dp;
+    if (history && history_len > 0) history[0] = dp;
+    int it = 0;
+    bool conv = (dp0 == 0.0), bad = false;
+    while (!conv && it < s->prm.max_it) {
+      if (it == 0) {
+        HIP_CHECK(hipMemcpyAsync(s->p.p, s->z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      } else {
+        hipLaunchKernelGGL(aypx_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, ratio,
+                           (const double*)s->z.p, s->p.p);
+      }
+      launch_matvec<double>(m, s->p.p, s->w.p, st);
+      dots(n, s->p.p, s->w.p, nullptr, nullptr, sc, hs, 1, st);
+      const double a = rz / hs[0];
+      hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)nullptr, a,
+                         1.0, (const double*)s->p.p, x);
+      hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)nullptr, -a,
+                         1.0, (const double*)s->w.p, s->r.p);
+      precondition(s, s->r.p, s->z.p, st);
+      // z.z, z.r and (flexible) z.w in one pass
+      HIP_CHECK(hipMemsetAsync(sc, 0, sizeof(double) * 3, st));
+      hipLaunchKernelGGL(dot3_kernel<double>, dim3(std::min(1024u, blocks_for(n))), dim3(VBLOCK), 0, st, n,
+                         (const double*)s->z.p, (const double*)s->z.p, (const double*)s->z.p,
+                         (const double*)s->r.p, flexible ? (const double*)s->z.p : (const double*)nullptr,
+                         (const double*)s->w.p, sc);
+      HIP_CHECK(hipMemcpyAsync(hs, sc, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      double rr = 0.0;
+      if (s->prm.norm_type == DGH_NORM_UNPRECONDITIONED) {
+        double t[1];
+        dots(n, s->r.p, s->r.p, nullptr, nullptr, sc, t, 1, st);
+        rr = t[0];
+      }
+      // standard: beta = z_k.r_k / z_{k-1}.r_{k-1}; flexible: z_k.(r_k - r_{k-1}) / z_{k-1}.r_{k-1}
+      // with r_{k-1} = r_k + a w, i.e. numerator -a z_k.w
+      ratio = flexible ? (-a * hs[2]) / rz : hs[1] / rz;
+      rz = hs[1];
+      dp = std::sqrt(s->prm.norm_type == DGH_NORM_PRECONDITIONED ? hs[0] : rr);
+      ++it;
+      if (history && it < history_len) history[it] = dp;
+      if (!std::isfinite(dp)) {
+        bad = true;
+        break;
+      }
+      if (dp <= s->prm.rtol * dp0) conv = true;
+    }
+    HIP_CHECK(hipEventRecord(e1, st));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    if (stats) {
+      // true residual ||b - A x|| / ||b|| in FP64 (b projected when constants are the null space)
+      HIP_CHECK(hipMemcpyAsync(s->r.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      if (s->prm.null_space) project_mean_dev(n, s->r.p, sc + 4, st);
+      launch_matvec<double>(m, x, s->w.p, st);
+      hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)nullptr, -1.0,
+                         1.0, (const double*)s->w.p, s->r.p);
+      double t[1];
+      dots(n, s->r.p, s->r.p, nullptr, nullptr, sc, t, 1, st);
+      stats->iterations = it;
+      stats->converged = conv ? 1 : 0;
+      stats->rnorm0 = dp0;
+      stats->rnorm = dp;
+      stats->rel_true = bnorm > 0 ? std::sqrt(t[0]) / bnorm : 0.0;
+      stats->seconds = ms * 1e-3;
+    }
+    if (bad) fail(DGH_ERR_DIVERGED, "CG breakdown: non-finite residual at iteration %d", it);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// Host-side octree and DA construction of libdendrohip.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace dgh {
+
+// ---- octrees (octree.cpp); octants at the tree's max depth D
+std::vector<Oct> octree_from_points(const double* xyz, uint64_t n, uint32_t D, uint32_t max_pts,
+                                    bool inc_corner);
+std::vector<Oct> balance(const std::vector<Oct>& in, uint32_t D, bool inc_corner);
+std::vector<Oct> coarsen(const std::vector<Oct>& in, uint32_t D);
+bool is_complete_linear(const std::vector<Oct>& v, uint32_t D);
+
+// ---- stencils (stencils.cpp)
+void generate_stencils(double* lap2, double* mass2, double* rmat2, double* rmat1);
+void generate_transfer_weights(double* W /* [144][27][8] */);
+
+// ---- one DA level (da.cpp)
+// Octants of the DA live at depth Dda = D + 1 (the octree is embedded one level deeper so
+// that positive-boundary pseudo-octants fit, src/oct/OctFunctions.C:636-700); the anchors
+// keep their coordinates, levels grow by one.
+struct DALevel {
+  uint32_t D = 0;                    // octree max depth; DA max depth is D + 1
+  std::vector<Oct> octs;             // elements [0, n_elem) then pseudo-octants, DA levels
+  uint32_t n_elem = 0, n_node = 0;
+  std::vector<uint32_t> node_of;     // node index of each octant, or kNoNode
+  std::vector<uint32_t> nlist;       // [n_elem][8]
+  std::vector<uint8_t> child, hnmask, lev;
+  uint32_t n_hanging = 0;
+  uint32_t max_depth() const { return D + 1; }
+  // index of the octant whose region holds the finest cell (x,y,z), or -1
+  int64_t find_cell(uint32_t x, uint32_t y, uint32_t z, bool elements_only) const;
+};
+constexpr uint32_t kNoNode = 0xffffffffu;
+
+void build_da(const std::vector<Oct>& tree, uint32_t D, DALevel& da);
+
+// Fine node -> (coarse element, position code) map between two DA levels of one hierarchy.
+void build_transfer(const DALevel& coarse, const DALevel& fine, std::vector<uint32_t>& celem,
+                    std::vector<uint8_t>& pos);
+
+// Matching of coarse elements to fine elements (same element or 8 children), in order:
+// first fine element of each coarse element and whether it is the same element.
+void match_levels(const DALevel& coarse, const DALevel& fine, std::vector<uint32_t>& first_fine,
+                  std::vector<uint8_t>& same);
+
+}  // namespace dgh

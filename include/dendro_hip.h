@@ -1,0 +1,250 @@
+/*
+ * dendro_hip.h -- C ABI of libdendrohip.so, the MI355X (gfx950) implementation of Dendro's
+ * matrix-free octree FEM solve path (oda element-loop MatVec, omg intergrid transfer,
+ * damped-Jacobi V-cycle inside PCG).
+ *
+ * Conventions (all entry points):
+ *   - every function returns int: 0 = success, negative = error (DGH_ERR_*);
+ *     dgh_last_error() returns the message of the most recent failure on the calling thread;
+ *   - plain pointers and sizes only; "_dev" pointers are device memory owned by the caller,
+ *     every other pointer is host memory;
+ *   - stream arguments are hipStream_t passed as void* (NULL = the null stream); device work is
+ *     stream-ordered and the call returns before it completes unless stated otherwise;
+ *   - level 0 of a hierarchy is the coarsest and level nlevels-1 the finest, as in the
+ *     reference's DAMG array (src/omg/omg.C:1838-1863).
+ *
+ * The reference interface each entry point replaces is cited beside it (paths relative to the
+ * reference tree).
+ */
+#ifndef DENDRO_HIP_H
+#define DENDRO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define DGH_OK 0
+#define DGH_ERR_ARG (-1)       /* invalid argument or size                           */
+#define DGH_ERR_HIP (-2)       /* HIP runtime error                                  */
+#define DGH_ERR_STATE (-3)     /* call out of order (e.g. no dgh_init)               */
+#define DGH_ERR_NOMEM (-4)     /* host or device allocation failed                   */
+#define DGH_ERR_MESH (-5)      /* mesh invariant violated (invalid hanging mask ...) */
+#define DGH_ERR_DIVERGED (-6)  /* solver breakdown / NaN / max_it reached            */
+
+/* precision of a device vector */
+#define DGH_F64 0
+#define DGH_F32 1
+
+/* Copies the message of the last error on this thread into buf (NUL-terminated). */
+int dgh_last_error(char* buf, size_t len);
+
+/* ================================================================ host: stencil tables */
+
+/* Element-type map (childNum, hanging mask) -> eType in [0,18).
+ * Replaces GET_ETYPE_BLOCK / getElemType<c> (include/oda/odaUtils.h:334-374,
+ * include/oda/odaUtils.txx:13-94, include/oda/cnumEtypes.h:15-229).
+ * Returns the eType, or DGH_ERR_MESH for a mask that is not one of the 18 for that child. */
+int dgh_etype(int child_num, int hn_mask);
+
+/* Generates the Type2 Laplacian and mass element stencils [8][18][8][8], the Type2
+ * restriction stencils [8][18][8][8] and the Type1 restriction stencils [8][8][18][8][8]
+ * (coarse child, fine child, coarse eType, coarse vertex, fine vertex) in the reference's
+ * index order, from the closed-form hanging-node shape functions.  Replaces the readers
+ * createLmatType2 / createMmatType2 (examples/src/backend/handleType2Stencils.C:678-987),
+ * readRmatType2Stencil (src/omg/RmatType2StencilsAux.C:15-43) and readRmatType1Stencil
+ * (src/omg/RmatType1StencilsAux.C:13-45), which parse the Stencils/ *.inp tables.
+ * Any output pointer may be NULL. */
+int dgh_stencils_generate(double* lap2, double* mass2, double* rmat2, double* rmat1);
+
+/* ================================================================ host: synthetic points */
+#define DGH_PTS_GAUSS 0      /* per-axis N(0.5, 0.1)                                   */
+#define DGH_PTS_LOGNORMAL 1  /* per-axis lognormal(0,1) rescaled to [0,1] by min/max   */
+#define DGH_PTS_MIXED 2      /* n/2 uniform + n/2 in 8 Gaussian clusters (sd 0.02)     */
+#define DGH_PTS_CLUSTERS 3   /* n in 8 Gaussian clusters (sd 0.02)                     */
+#define DGH_PTS_UNIFORM 4    /* U(0,1)^3                                               */
+/* Writes n points (xyz interleaved) from a xoshiro256** stream seeded by splitmix64(seed);
+ * Gaussian variates by Box-Muller.  cluster_seed seeds the cluster centres (MIXED, CLUSTERS).
+ * Plays the role of scripts/genGaussSeq.C and scripts/genLogNormSeq.C. */
+int dgh_points_generate(int kind, uint64_t n, uint64_t seed, uint64_t cluster_seed, double* xyz);
+
+/* ================================================================ host: octrees */
+typedef struct dgh_octree_s* dgh_octree;
+
+/* Points -> complete linear octree (at most max_pts points per leaf, depth <= max_depth) ->
+ * 2:1 balanced octree (corner neighbours included when inc_corner != 0).  Applies the
+ * driver's point preprocessing first: quantise to max_depth, drop coordinates <= 0 or
+ * outside the unit cube, remove duplicate cells (examples/src/drivers/tstMg.C:148-192).
+ * Replaces ot::points2Octree (src/oct/Construct.C:727) + ot::balanceOctree
+ * (src/oct/Balance.C:31). */
+int dgh_octree_from_points(const double* xyz, uint64_t n, uint32_t max_depth, uint32_t max_pts,
+                           int inc_corner, dgh_octree* out);
+/* Wraps a given sorted, linear, complete octree (anchors xyz[n][3], levels lev[n]). */
+int dgh_octree_from_leaves(const uint32_t* xyz, const uint8_t* lev, uint64_t n,
+                           uint32_t max_depth, dgh_octree* out);
+/* Minimal 2:1 balanced refinement.  Replaces ot::balanceOctree (src/oct/Balance.C:31). */
+int dgh_octree_balance(dgh_octree in, int inc_corner, dgh_octree* out);
+/* One level of coarsening of complete sibling families.
+ * Replaces ot::coarsenOctree (src/oct/Coarsen.C:53-106). */
+int dgh_octree_coarsen(dgh_octree in, dgh_octree* out);
+int dgh_octree_size(dgh_octree t, uint64_t* n, uint32_t* max_depth);
+int dgh_octree_get(dgh_octree t, uint32_t* xyz, uint8_t* lev);
+int dgh_octree_free(dgh_octree t);
+
+/* ================================================================ host: DA hierarchy */
+typedef struct dgh_damg_s* dgh_damg;
+
+typedef struct {
+  uint32_t n_elem;      /* elements (ot::DA::getElementSize)                           */
+  uint32_t n_node;      /* owned nodes incl. positive-boundary nodes (getNodeSize +    */
+                        /* getBoundaryNodeSize, src/oda/oda.C:530-533)                 */
+  uint32_t n_oct;       /* elements + positive-boundary pseudo-octants                 */
+  uint32_t max_depth;   /* DA max depth = octree max depth + 1 (odaFactory.C:160)      */
+  uint32_t n_hanging;   /* elements with a nonzero hanging mask                        */
+} dgh_level_info;
+
+/* Builds the multigrid hierarchy of DAs from the finest balanced octree: coarsen + balance
+ * until *nlevels levels exist or a level has fewer than min_coarse octants (the reference
+ * uses 500), then the per-level DA (+1 embedding, positive-boundary pseudo-octants, regular
+ * node flags, 8-node lists with hanging vertices resolved to the parent's vertex node,
+ * Morton child numbers, hanging masks, levels) and the fine-node -> coarse-element transfer
+ * maps.  *nlevels returns the actual count.  Replaces ot::DAMGCreateAndSetDA
+ * (src/omg/omg.C:1306-2171), the DA constructor (src/oda/oda.C:175-196,
+ * src/oda/private/odaFactory.C:100-330, src/oda/private/odaBuildNlist.C:47-1900) and
+ * createInterpolationType1 / dummyRestrictMatVecType1 (src/omg/omg.C:2234-2290,
+ * src/omg/RestrictMatVec.C:410-484). */
+int dgh_damg_build(dgh_octree finest, int* nlevels, uint32_t min_coarse, dgh_damg* out);
+int dgh_damg_num_levels(dgh_damg h, int* nlevels);
+int dgh_damg_level_info(dgh_damg h, int level, dgh_level_info* info);
+/* Element arrays of one level (any pointer may be NULL):
+ *   nlist[n_elem][8]  node index of each vertex (the reference's uncompressed m_uiNlist,
+ *                     renumbered to the compact node vector, src/oda/oda.C:621-646);
+ *   child[n_elem]     Morton child number (include/oda/oda.txx:1087-1109);
+ *   hn_mask[n_elem]   hanging-vertex mask (src/oda/oda.C:784-789);
+ *   lev[n_elem]       DA level (include/oda/oda.txx:51-56);
+ *   anchor[n_elem][3] DA anchor coordinates. */
+int dgh_damg_level_arrays(dgh_damg h, int level, uint32_t* nlist, uint8_t* child,
+                          uint8_t* hn_mask, uint8_t* lev, uint32_t* anchor);
+/* All octants of one level (elements then pseudo-octants, Morton order):
+ * anchor[n_oct][3], lev[n_oct], flags[n_oct] (bit 7 = regular node, bit 6 = pseudo-octant). */
+int dgh_damg_level_octants(dgh_damg h, int level, uint32_t* anchor, uint8_t* lev,
+                           uint8_t* flags);
+/* Transfer map from level fine_level-1 (coarse) to fine_level: for every fine node, the
+ * index of a coarse element whose closure holds it and the node's position code
+ * pos = tx + 3 ty + 9 tz (t in {0,1,2}: offset 0, h/2, h inside that coarse element). */
+int dgh_damg_transfer_arrays(dgh_damg h, int fine_level, uint32_t* coarse_elem, uint8_t* pos);
+/* Per-element coefficients of every level from finest-level values: kappa/mass of a
+ * coarse element = same element's value, or the mean of its 8 children taken in child
+ * order.  Replaces ASSIGN_MAT_PROP_FINE_TO_COARSE_ELEM_BLOCK
+ * (examples/src/backend/matProp.C:192-232).  out_kappa/out_mass hold n_elem of the level. */
+int dgh_damg_coarsen_coefficients(dgh_damg h, const double* fine_kappa, const double* fine_mass,
+                                  int level, double* out_kappa, double* out_mass);
+int dgh_damg_free(dgh_damg h);
+
+/* ================================================================ device: runtime */
+
+/* Selects the device and creates the library state (stencil tables, streams).
+ * Plays the role of DAMG_Initialize / DA_Initialize (src/omg/omg.C:66, src/oda/odaUtils.C:1742). */
+int dgh_init(int device);
+int dgh_finalize(void);
+
+/* ================================================================ device: one mesh level */
+typedef struct dgh_mesh_s* dgh_mesh;
+
+/* Uploads one DA level once (host arrays as returned by dgh_damg_level_arrays).  Element
+ * data are stored on the device grouped by stencil (Morton order inside a group).
+ * Replaces CreateJacobian1 (examples/src/backend/odaJac.C:54-72) and the per-call element
+ * decode of Jacobian1MatMult (odaJac.C:290-374). */
+int dgh_mesh_create(uint32_t n_elem, uint32_t n_node, uint32_t max_depth, const uint32_t* nlist,
+                    const uint8_t* child, const uint8_t* hn_mask, const uint8_t* lev,
+                    dgh_mesh* out);
+/* Operator coefficients.  kappa == NULL: uniform Laplacian factor lap_fac and mass factor
+ * mass_fac (Jacobian1, "-lapMatPropFac"/"-massMatPropFac", odaJac.C:309-313); otherwise the
+ * per-element (kappa, mass) of Jacobian2 (examples/src/backend/omgJac.C:1665-1685; mass may
+ * be NULL for zero mass). */
+int dgh_mesh_set_coefficients(dgh_mesh m, const double* kappa, const double* mass,
+                              double lap_fac, double mass_fac);
+int dgh_mesh_info(dgh_mesh m, uint32_t* n_elem, uint32_t* n_node);
+int dgh_mesh_destroy(dgh_mesh m);
+
+/* y = A x.  Replaces Jacobian1MatMult (odaJac.C:290-374) and Jacobian2MatMult
+ * (omgJac.C:1687-1775).  prec selects FP64 or FP32 vectors and arithmetic. */
+int dgh_matvec(dgh_mesh m, int prec, const void* x_dev, void* y_dev, void* stream);
+/* d = diag(A).  Replaces Jacobian1MatGetDiagonal (odaJac.C:160-223) and
+ * Jacobian2MatGetDiagonal (omgJac.C:1594-1662). */
+int dgh_diagonal(dgh_mesh m, int prec, void* d_dev, void* stream);
+/* k right-hand sides at once: X, Y are [N][k] row-major (node-major, RHS fastest). */
+int dgh_matvec_multi(dgh_mesh m, int prec, int k, const void* x_dev, void* y_dev, void* stream);
+
+/* ================================================================ device: intergrid transfer */
+typedef struct dgh_transfer_s* dgh_transfer;
+
+/* Uploads the fine-node -> coarse-element map (dgh_damg_transfer_arrays).
+ * Replaces createInterpolationType1 (src/omg/omg.C:2234-2290). */
+int dgh_transfer_create(dgh_mesh coarse, dgh_mesh fine, const uint32_t* coarse_elem,
+                        const uint8_t* pos, dgh_transfer* out);
+int dgh_transfer_destroy(dgh_transfer t);
+/* c = R f.  Replaces restrictMatVecType1 (src/omg/RestrictMatVec.C:486-606). */
+int dgh_restrict(dgh_transfer t, int prec, const void* f_dev, void* c_dev, void* stream);
+/* f_out = f_in + R^T c (f_in may equal f_out; f_in == NULL means f_out = R^T c).
+ * Replaces prolongMatVecType1 / addProlongMatVec (src/omg/ProlongMatVec.C:32-284). */
+int dgh_prolong_add(dgh_transfer t, int prec, const void* c_dev, const void* f_in_dev,
+                    void* f_out_dev, void* stream);
+
+/* ================================================================ device: GMG-PCG */
+typedef struct dgh_solver_s* dgh_solver;
+
+#define DGH_NORM_UNPRECONDITIONED 0 /* stop on ||r||_2 <= rtol ||b||_2                 */
+#define DGH_NORM_PRECONDITIONED 1   /* stop on ||Br||_2 <= rtol ||Br_0||_2 (PETSc CG)  */
+
+#define DGH_SMOOTHER_JACOBI 0
+#define DGH_SMOOTHER_LEARNED 1
+
+typedef struct {
+  double rtol;          /* relative tolerance (-ksp_rtol)                              */
+  double omega;         /* Richardson scale (-mg_levels_ksp_richardson_scale)          */
+  int max_it;           /* outer CG iterations (-ksp_max_it)                           */
+  int nu_pre;           /* pre-smoothing sweeps (-mg_levels_ksp_max_it)                */
+  int nu_post;          /* post-smoothing sweeps                                       */
+  int norm_type;        /* DGH_NORM_*                                                  */
+  int null_space;       /* 1: constants are the operator's null space (pure Neumann)   */
+  int vcycle_prec;      /* DGH_F64: plain FP64; DGH_F32: FP32 V-cycle, FP64 outer CG   */
+  int smoother;         /* DGH_SMOOTHER_*                                              */
+  int use_graph;        /* capture the V-cycle in a hipGraph and replay it             */
+} dgh_solver_params;
+
+typedef struct {
+  int iterations;       /* CG iterations done                                          */
+  int converged;        /* 1 if the stopping test was met                              */
+  double rnorm0;        /* initial norm used by the stopping test                      */
+  double rnorm;         /* final norm used by the stopping test                        */
+  double rel_true;      /* ||b - A x||_2 / ||b||_2 recomputed in FP64 after the solve  */
+  double seconds;       /* device time of the solve (hipEvents)                        */
+} dgh_solve_stats;
+
+/* Fills p with the reference defaults of bin/options (CG, V(4,4), omega 0.667, rtol 1e-8). */
+int dgh_solver_default_params(dgh_solver_params* p);
+/* meshes[0..nlevels-1] coarsest first; transfers[i] maps level i to level i+1.  Computes the
+ * Jacobi diagonals, assembles the coarsest operator (CreateAndComputeFullJacobian1,
+ * odaJac.C:466-536) and factors it on the host.  Replaces DAMGSetKSP + DAMGSetUpLevel
+ * (src/omg/omg.C:779-1046, 1171-1303) with the options of bin/options:50-53,182-206. */
+int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* transfers,
+                      const dgh_solver_params* params, dgh_solver* out);
+int dgh_solver_destroy(dgh_solver s);
+/* Preconditioned CG on the finest level, x_dev = A^{-1} b_dev (x starts at 0).
+ * Replaces DAMGSolve -> KSPSolve (src/omg/omg.C:1121-1169).  Synchronous.
+ * history (may be NULL) receives up to history_len stopping-test norms, one per iteration
+ * starting with the initial one. */
+int dgh_solver_solve(dgh_solver s, const double* b_dev, double* x_dev, dgh_solve_stats* stats,
+                     double* history, int history_len, void* stream);
+/* z = B r: one application of the V-cycle preconditioner (PCApply_MG), FP64 vectors. */
+int dgh_solver_vcycle(dgh_solver s, const double* r_dev, double* z_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DENDRO_HIP_H */

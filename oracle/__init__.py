@@ -1,0 +1,172 @@
+"""CPU oracle of the reference's solve path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings of oracle/liboracle.so (built from oracle/oracle.c by oracle/Makefile).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the
+checker; the product (dendro_amd/) never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-B", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} not built: run make -C oracle")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.or_mg_create.restype = C.c_void_p
+        _lib.or_mg_create.argtypes = [C.c_int]
+        _lib.or_mg_free.argtypes = [C.c_void_p]
+        for f in ["or_mg_set_level", "or_mg_setup", "or_mg_vcycle", "or_pcg", "or_mg_apply"]:
+            getattr(_lib, f).argtypes = None
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert isinstance(a, np.ndarray) and a.flags.c_contiguous
+    return C.c_void_p(a.ctypes.data)
+
+
+def etype(child, mask):
+    return lib().or_etype(int(child), int(mask))
+
+
+def stencils():
+    L, M, R2, R1 = np.zeros(9216), np.zeros(9216), np.zeros(9216), np.zeros(73728)
+    lib().or_stencils(_p(L), _p(M), _p(R2), _p(R1))
+    return {"lap2": L.reshape(8, 18, 8, 8), "mass2": M.reshape(8, 18, 8, 8),
+            "rmat2": R2.reshape(8, 18, 8, 8), "rmat1": R1.reshape(8, 8, 18, 8, 8)}
+
+
+_ST = None
+
+
+def _st():
+    global _ST
+    if _ST is None:
+        s = stencils()
+        _ST = {k: np.ascontiguousarray(v.reshape(-1)) for k, v in s.items()}
+    return _ST
+
+
+def _coef(level, kappa, mass):
+    k = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+    m = None if mass is None else np.ascontiguousarray(mass, np.float64)
+    return k, m
+
+
+def matvec(level, x, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0):
+    st = _st()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(level.n_node)
+    k, m = _coef(level, kappa, mass)
+    lib().or_matvec(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), C.c_uint32(level.max_depth),
+                    _p(level.nlist), _p(level.child), _p(level.hnmask), _p(level.lev), _p(k), _p(m),
+                    C.c_double(lap_fac), C.c_double(mass_fac), _p(st["lap2"]), _p(st["mass2"]), _p(x), _p(y))
+    return y
+
+
+def diagonal(level, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0):
+    st = _st()
+    d = np.zeros(level.n_node)
+    k, m = _coef(level, kappa, mass)
+    lib().or_diagonal(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), C.c_uint32(level.max_depth),
+                      _p(level.nlist), _p(level.child), _p(level.hnmask), _p(level.lev), _p(k), _p(m),
+                      C.c_double(lap_fac), C.c_double(mass_fac), _p(st["lap2"]), _p(st["mass2"]), _p(d))
+    return d
+
+
+class _OrLevel(C.Structure):
+    _fields_ = [("E", C.c_uint32), ("N", C.c_uint32), ("max_depth", C.c_uint32),
+                ("nlist", C.c_void_p), ("child", C.c_void_p), ("hnmask", C.c_void_p),
+                ("lev", C.c_void_p), ("anchor", C.c_void_p)]
+
+
+def _orlevel(L):
+    return _OrLevel(L.n_elem, L.n_node, L.max_depth, L.nlist.ctypes.data, L.child.ctypes.data,
+                    L.hnmask.ctypes.data, L.lev.ctypes.data, L.anchor.ctypes.data)
+
+
+def restrict(coarse, fine, f):
+    st = _st()
+    f = np.ascontiguousarray(f, np.float64)
+    c = np.zeros(coarse.n_node)
+    oc, of = _orlevel(coarse), _orlevel(fine)
+    rc = lib().or_restrict(C.byref(oc), C.byref(of), _p(st["rmat2"]), _p(st["rmat1"]), _p(f), _p(c))
+    if rc:
+        raise RuntimeError(f"or_restrict failed ({rc})")
+    return c
+
+
+def prolong(coarse, fine, c):
+    st = _st()
+    c = np.ascontiguousarray(c, np.float64)
+    f = np.zeros(fine.n_node)
+    oc, of = _orlevel(coarse), _orlevel(fine)
+    rc = lib().or_prolong(C.byref(oc), C.byref(of), _p(st["rmat2"]), _p(st["rmat1"]), _p(c), _p(f))
+    if rc:
+        raise RuntimeError(f"or_prolong failed ({rc})")
+    return f
+
+
+class MG:
+    """Oracle multigrid hierarchy; levels: list (coarsest first) of (Level, kappa, mass,
+    lap_fac, mass_fac)."""
+
+    def __init__(self, levels, omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=False):
+        self._keep = []
+        self.h = lib().or_mg_create(len(levels))
+        for i, (L, kappa, mass, lf, mf) in enumerate(levels):
+            k, m = _coef(L, kappa, mass)
+            self._keep += [L, k, m]
+            rc = lib().or_mg_set_level(C.c_void_p(self.h), C.c_int(i), C.c_uint32(L.n_elem),
+                                       C.c_uint32(L.n_node), C.c_uint32(L.max_depth), _p(L.nlist),
+                                       _p(L.child), _p(L.hnmask), _p(L.lev), _p(L.anchor), _p(k), _p(m),
+                                       C.c_double(lf), C.c_double(mf))
+            assert rc == 0
+        rc = lib().or_mg_setup(C.c_void_p(self.h), C.c_double(omega), C.c_int(nu_pre), C.c_int(nu_post),
+                               C.c_int(int(null_space)))
+        if rc:
+            raise RuntimeError(f"or_mg_setup failed ({rc})")
+        self.n = levels[-1][0].n_node
+
+    def vcycle(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros(self.n)
+        rc = lib().or_mg_vcycle(C.c_void_p(self.h), _p(r), _p(z))
+        if rc:
+            raise RuntimeError(f"or_mg_vcycle failed ({rc})")
+        return z
+
+    def apply(self, level, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(len(x))
+        lib().or_mg_apply(C.c_void_p(self.h), C.c_int(level), _p(x), _p(y))
+        return y
+
+    def pcg(self, b, rtol=1e-8, max_it=100, norm_type=0):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n)
+        hist = np.zeros(max_it + 1)
+        conv = C.c_int()
+        its = lib().or_pcg(C.c_void_p(self.h), _p(b), _p(x), C.c_double(rtol), C.c_int(max_it),
+                           C.c_int(norm_type), _p(hist), C.c_int(max_it + 1), C.byref(conv))
+        return x, its, bool(conv.value), hist[: its + 1]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_mg_free(C.c_void_p(self.h))
+            self.h = None

@@ -1,0 +1,599 @@
+/*
+ * oracle.c -- CPU restatement of the reference's octree FEM solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker; never by the product.
+ *
+ * Paths cite the reference tree.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ element types */
+/* include/oda/cnumEtypes.h:15-229 (= Matlab/cNumBasedHangingMasks.m), eType order */
+static const unsigned char kMask[8][18] = {
+    {0, 4, 2, 6, 16, 20, 18, 22, 14, 30, 84, 86, 94, 50, 54, 62, 118, 126},
+    {0, 1, 8, 9, 32, 33, 40, 41, 13, 45, 49, 57, 61, 168, 169, 173, 185, 189},
+    {0, 8, 1, 9, 64, 72, 65, 73, 11, 75, 200, 201, 203, 81, 89, 91, 217, 219},
+    {0, 2, 4, 6, 128, 130, 132, 134, 7, 135, 162, 166, 167, 196, 198, 199, 230, 231},
+    {0, 1, 32, 33, 64, 65, 96, 97, 35, 99, 69, 101, 103, 224, 225, 227, 229, 231},
+    {0, 2, 128, 130, 16, 18, 144, 146, 138, 154, 19, 147, 155, 208, 210, 218, 211, 219},
+    {0, 4, 16, 20, 128, 132, 144, 148, 21, 149, 140, 156, 157, 176, 180, 181, 188, 189},
+    {0, 8, 64, 72, 32, 40, 96, 104, 76, 108, 42, 106, 110, 112, 120, 124, 122, 126}};
+
+/* getElemType<cNum> (include/oda/odaUtils.txx:13-94): switch over the 18 masks */
+int or_etype(int child, int mask) {
+  if (child < 0 || child > 7) return -1;
+  for (int t = 0; t < 18; ++t)
+    if (kMask[child][t] == mask) return t;
+  return -1;
+}
+
+/* ------------------------------------------------------------------ stencils */
+/* Matlab/parentOfCoarse3d.m: parent vertices of a child's master element */
+static const int kEpnuga[27][3] = {
+    {-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {-1, 1, 1},
+    {1, 1, 1},    {0, -1, -1}, {0, 1, -1},  {-1, 0, -1}, {1, 0, -1}, {0, -1, 1},  {0, 1, 1},
+    {-1, 0, 1},   {1, 0, 1},   {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0}, {1, 1, 0},   {0, 0, -1},
+    {0, 0, 1},    {-1, 0, 0},  {1, 0, 0},   {0, -1, 0},  {0, 1, 0},  {0, 0, 0}};
+static const int kMtIdx[8][8] = {{0, 8, 10, 20, 16, 24, 22, 26},  {8, 1, 20, 11, 24, 17, 26, 23},
+                                 {10, 20, 2, 9, 22, 26, 18, 25},  {20, 11, 9, 3, 26, 23, 25, 19},
+                                 {16, 24, 22, 26, 4, 12, 14, 21}, {24, 17, 26, 23, 12, 5, 21, 15},
+                                 {22, 26, 18, 25, 14, 21, 6, 13}, {26, 23, 25, 19, 21, 15, 13, 7}};
+static const int kCube[8][3] = {{-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {1, 1, -1},
+                                {-1, -1, 1},  {1, -1, 1},  {-1, 1, 1},  {1, 1, 1}};
+
+static void parent_coords(int c, double P[8][3]) {
+  for (int i = 0; i < 8; ++i)
+    for (int d = 0; d < 3; ++d) P[i][d] = 4.0 * kEpnuga[kMtIdx[c][i]][d];
+  double off[3];
+  for (int d = 0; d < 3; ++d) off[d] = P[c][d] - kCube[c][d];
+  for (int i = 0; i < 8; ++i)
+    for (int d = 0; d < 3; ++d) P[i][d] -= off[d];
+}
+
+/* Matlab/coarseShapeFunction3DCoeffs.m: phiCoeffs(i,:) = inv(M) e_i */
+static void coarse_coeffs(int c, int mask, double phi[8][8]) {
+  double P[8][3], M[8][8], A[8][16];
+  parent_coords(c, P);
+  for (int i = 0; i < 8; ++i) {
+    double X[3];
+    for (int d = 0; d < 3; ++d) X[d] = (mask & (1 << i)) ? P[i][d] : kCube[i][d];
+    M[i][0] = 1.0;
+    M[i][1] = X[0];
+    M[i][2] = X[1];
+    M[i][3] = X[2];
+    M[i][4] = X[0] * X[1];
+    M[i][5] = X[1] * X[2];
+    M[i][6] = X[2] * X[0];
+    M[i][7] = X[0] * X[1] * X[2];
+  }
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 16; ++j) A[i][j] = (j < 8) ? M[i][j] : (j - 8 == i ? 1.0 : 0.0);
+  for (int col = 0; col < 8; ++col) { /* Gauss-Jordan with partial pivoting */
+    int p = col;
+    for (int r = col + 1; r < 8; ++r)
+      if (fabs(A[r][col]) > fabs(A[p][col])) p = r;
+    for (int j = 0; j < 16; ++j) {
+      double t = A[col][j];
+      A[col][j] = A[p][j];
+      A[p][j] = t;
+    }
+    double iv = 1.0 / A[col][col];
+    for (int j = 0; j < 16; ++j) A[col][j] *= iv;
+    for (int r = 0; r < 8; ++r)
+      if (r != col) {
+        double f = A[r][col];
+        for (int j = 0; j < 16; ++j) A[r][j] -= f * A[col][j];
+      }
+  }
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j) phi[i][j] = A[j][8 + i]; /* column i of inv(M) */
+}
+
+/* Matlab/evalPhi.m, Matlab/evalGradPhi.m */
+static double eval_phi(const double* c, double x, double y, double z) {
+  return c[0] + c[1] * x + c[2] * y + c[3] * z + c[4] * x * y + c[5] * y * z + c[6] * z * x +
+         c[7] * x * y * z;
+}
+static double eval_grad(const double* c, int d, double x, double y, double z) {
+  if (d == 0) return c[1] + c[4] * y + c[6] * z + c[7] * y * z;
+  if (d == 1) return c[2] + c[4] * x + c[5] * z + c[7] * x * z;
+  return c[3] + c[5] * y + c[6] * x + c[7] * x * y;
+}
+
+/* Matlab/computeLapStencilsType2.m, computeMassStencilsType2.m (2-point Gauss, weights 1),
+ * computeRPstencilsType2.m, computeRPstencilsType1.m */
+void or_stencils(double* lap2, double* mass2, double* rmat2, double* rmat1) {
+  const double g[2] = {-1.0 / sqrt(3.0), 1.0 / sqrt(3.0)};
+  static const double xlb[8] = {-1, 0, -1, 0, -1, 0, -1, 0}, xub[8] = {0, 1, 0, 1, 0, 1, 0, 1};
+  static const double ylb[8] = {-1, -1, 0, 0, -1, -1, 0, 0}, yub[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+  static const double zlb[8] = {-1, -1, -1, -1, 0, 0, 0, 0}, zub[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  for (int c = 0; c < 8; ++c)
+    for (int t = 0; t < 18; ++t) {
+      double phi[8][8];
+      coarse_coeffs(c, kMask[c][t], phi);
+      size_t base = ((size_t)c * 18 + t) * 64;
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) {
+          double L = 0.0, M = 0.0;
+          for (int m = 0; m < 2; ++m)
+            for (int n = 0; n < 2; ++n)
+              for (int p = 0; p < 2; ++p) {
+                for (int d = 0; d < 3; ++d)
+                  L += eval_grad(phi[i], d, g[m], g[n], g[p]) * eval_grad(phi[j], d, g[m], g[n], g[p]);
+                M += eval_phi(phi[i], g[m], g[n], g[p]) * eval_phi(phi[j], g[m], g[n], g[p]);
+              }
+          if (lap2) lap2[base + i * 8 + j] = L;
+          if (mass2) mass2[base + i * 8 + j] = M;
+        }
+      if (rmat2)
+        for (int i = 0; i < 8; ++i)
+          for (int f = 0; f < 8; ++f)
+            rmat2[base + i * 8 + f] = eval_phi(phi[i], kCube[f][0], kCube[f][1], kCube[f][2]);
+      if (rmat1)
+        for (int cf = 0; cf < 8; ++cf) {
+          size_t b1 = (((size_t)c * 8 + cf) * 18 + t) * 64;
+          double lx = xlb[cf], ux = xub[cf], ly = ylb[cf], uy = yub[cf], lz = zlb[cf], uz = zub[cf];
+          double pts[8][3] = {{lx, ly, lz}, {ux, ly, lz}, {lx, uy, lz}, {ux, uy, lz},
+                              {lx, ly, uz}, {ux, ly, uz}, {lx, uy, uz}, {ux, uy, uz}};
+          for (int i = 0; i < 8; ++i)
+            for (int f = 0; f < 8; ++f)
+              rmat1[b1 + i * 8 + f] = eval_phi(phi[i], pts[f][0], pts[f][1], pts[f][2]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ operator */
+/* JACOBIAN_TYPE1_MULT_BLOCK (examples/src/backend/odaJac.C:271-288) and
+ * JAC_TYPE2_ELEM_MULT_BLOCK (examples/src/backend/omgJac.C:1665-1685):
+ * h = hFac 2^(maxD-lev), hFac = 1/2^(maxD-1); fac1 = kappa h/2, fac2 = mass h^3/8;
+ * out[idx[k]] += (fac1 L[k][j] + fac2 M[k][j]) in[idx[j]] in element order. */
+void or_matvec(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist,
+               const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+               const double* kappa, const double* mass, double lap_fac, double mass_fac,
+               const double* lap2, const double* mass2, const double* x, double* y) {
+  memset(y, 0, sizeof(double) * N);
+  const double hFac = 1.0 / ((double)(1u << (maxD - 1)));
+  for (uint32_t e = 0; e < E; ++e) {
+    const double h = hFac * (double)(1u << (maxD - lev[e]));
+    const double k1 = kappa ? kappa[e] : lap_fac;
+    const double k2 = kappa ? (mass ? mass[e] : 0.0) : mass_fac;
+    const double fac1 = k1 * h / 2.0;
+    const double fac2 = k2 * h * h * h / 8.0;
+    const int c = child[e];
+    int t = 0;
+    if (hnmask[e]) t = or_etype(c, hnmask[e]);
+    const double* L = lap2 + ((size_t)c * 18 + t) * 64;
+    const double* M = mass2 + ((size_t)c * 18 + t) * 64;
+    const uint32_t* idx = nlist + (size_t)e * 8;
+    for (int k = 0; k < 8; ++k)
+      for (int j = 0; j < 8; ++j)
+        y[idx[k]] += ((fac1 * L[k * 8 + j]) + (fac2 * M[k * 8 + j])) * x[idx[j]];
+  }
+}
+
+/* Jacobian1MatGetDiagonal (odaJac.C:160-223), JAC_TYPE2_ELEM_DIAG_BLOCK (omgJac.C:1594-1612) */
+void or_diagonal(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist,
+                 const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+                 const double* kappa, const double* mass, double lap_fac, double mass_fac,
+                 const double* lap2, const double* mass2, double* d) {
+  memset(d, 0, sizeof(double) * N);
+  const double hFac = 1.0 / ((double)(1u << (maxD - 1)));
+  for (uint32_t e = 0; e < E; ++e) {
+    const double h = hFac * (double)(1u << (maxD - lev[e]));
+    const double k1 = kappa ? kappa[e] : lap_fac;
+    const double k2 = kappa ? (mass ? mass[e] : 0.0) : mass_fac;
+    const double fac1 = k1 * h / 2.0;
+    const double fac2 = k2 * h * h * h / 8.0;
+    const int c = child[e];
+    int t = 0;
+    if (hnmask[e]) t = or_etype(c, hnmask[e]);
+    const double* L = lap2 + ((size_t)c * 18 + t) * 64;
+    const double* M = mass2 + ((size_t)c * 18 + t) * 64;
+    for (int k = 0; k < 8; ++k)
+      d[nlist[(size_t)e * 8 + k]] += ((fac1 * L[k * 9]) + (fac2 * M[k * 9]));
+  }
+}
+
+/* ------------------------------------------------------------------ intergrid transfer */
+/* The simultaneous coarse/fine loop of INTERGRID_TRANSFER_LOOP_BLOCK
+ * (src/omg/RestrictMatVec.C:82-151): advance the fine element until its anchor matches the
+ * coarse element's; same level -> Type2 stencil RmatType2[cNumC][ctype], else the 8
+ * children with RmatType1[cNumC][cNumF][ctype].  Non-hanging fine vertices only.  The
+ * FineTouchedStatus bits (dummyRestrictMatVecType1, RestrictMatVec.C:153-484) let every
+ * (fine node, coarse node) pair contribute exactly once; here the first occurrence in loop
+ * order is kept (all occurrences carry the same value, the coarse shape function at the
+ * fine node). */
+typedef struct {
+  uint32_t* cap;   /* per fine node: count of taken coarse nodes */
+  uint32_t* ids;   /* per fine node: up to kTake taken coarse nodes */
+} taken_t;
+#define kTake 64
+
+static int transfer(const or_level* C, const or_level* F, const double* rmat2,
+                    const double* rmat1, const double* in, double* out, int restrict_dir) {
+  taken_t tk;
+  tk.cap = (uint32_t*)calloc(F->N, sizeof(uint32_t));
+  tk.ids = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)F->N * kTake);
+  if (!tk.cap || !tk.ids) {
+    free(tk.cap);
+    free(tk.ids);
+    return -1;
+  }
+  memset(out, 0, sizeof(double) * (restrict_dir ? C->N : F->N));
+  uint32_t f = 0;
+  int rc = 0;
+  for (uint32_t c = 0; c < C->E && rc == 0; ++c) {
+    const uint32_t* ca = C->anchor + 3 * (size_t)c;
+    while (f < F->E && (F->anchor[3 * (size_t)f] != ca[0] || F->anchor[3 * (size_t)f + 1] != ca[1] ||
+                        F->anchor[3 * (size_t)f + 2] != ca[2]))
+      ++f;
+    if (f >= F->E) {
+      rc = -2;
+      break;
+    }
+    const int cNumC = C->child[c];
+    int ctype = 0;
+    if (C->hnmask[c]) ctype = or_etype(cNumC, C->hnmask[c]);
+    const uint32_t* cIdx = C->nlist + (size_t)c * 8;
+    const int nsub = (F->lev[f] == C->lev[c]) ? 1 : 8;
+    for (int cNumF = 0; cNumF < nsub; ++cNumF, ++f) {
+      const double* R = (nsub == 1) ? rmat2 + ((size_t)cNumC * 18 + ctype) * 64
+                                    : rmat1 + (((size_t)cNumC * 8 + cNumF) * 18 + ctype) * 64;
+      const uint8_t fh = F->hnmask[f];
+      const uint32_t* fIdx = F->nlist + (size_t)f * 8;
+      for (int fC = 0; fC < 8; ++fC) {
+        if (fh & (1 << fC)) continue;
+        const uint32_t fn = fIdx[fC];
+        for (int cC = 0; cC < 8; ++cC) {
+          const uint32_t cn = cIdx[cC];
+          uint32_t* ids = tk.ids + (size_t)fn * kTake;
+          int seen = 0;
+          for (uint32_t q = 0; q < tk.cap[fn]; ++q)
+            if (ids[q] == cn) {
+              seen = 1;
+              break;
+            }
+          if (seen) continue;
+          if (tk.cap[fn] >= kTake) {
+            rc = -3;
+            break;
+          }
+          ids[tk.cap[fn]++] = cn;
+          const double Rval = R[cC * 8 + fC];
+          if (restrict_dir)
+            out[cn] += Rval * in[fn];
+          else
+            out[fn] += Rval * in[cn];
+        }
+      }
+    }
+  }
+  free(tk.cap);
+  free(tk.ids);
+  return rc;
+}
+
+int or_restrict(const or_level* C, const or_level* F, const double* rmat2, const double* rmat1,
+                const double* f, double* c) {
+  return transfer(C, F, rmat2, rmat1, f, c, 1);
+}
+int or_prolong(const or_level* C, const or_level* F, const double* rmat2, const double* rmat1,
+               const double* c, double* f) {
+  return transfer(C, F, rmat2, rmat1, c, f, 0);
+}
+
+/* ------------------------------------------------------------------ multigrid + CG */
+typedef struct {
+  or_level lv;
+  const double *kappa, *mass;
+  double lap_fac, mass_fac;
+  double* invdiag;
+  double *b, *x, *r, *z;   /* work vectors of this level */
+} mg_level;
+
+struct or_mg_s {
+  int nlevels;
+  mg_level* L;
+  double *lap2, *mass2, *rmat2, *rmat1;
+  double omega;
+  int nu_pre, nu_post, null_space;
+  /* coarse LU (dense, partial pivoting) */
+  int nc;
+  double* lu;
+  int* piv;
+};
+
+or_mg* or_mg_create(int nlevels) {
+  if (nlevels < 1) return NULL;
+  or_mg* mg = (or_mg*)calloc(1, sizeof(or_mg));
+  mg->nlevels = nlevels;
+  mg->L = (mg_level*)calloc(nlevels, sizeof(mg_level));
+  mg->lap2 = (double*)malloc(sizeof(double) * 9216);
+  mg->mass2 = (double*)malloc(sizeof(double) * 9216);
+  mg->rmat2 = (double*)malloc(sizeof(double) * 9216);
+  mg->rmat1 = (double*)malloc(sizeof(double) * 73728);
+  or_stencils(mg->lap2, mg->mass2, mg->rmat2, mg->rmat1);
+  return mg;
+}
+
+void or_mg_free(or_mg* mg) {
+  if (!mg) return;
+  for (int l = 0; l < mg->nlevels; ++l) {
+    free(mg->L[l].invdiag);
+    free(mg->L[l].b);
+    free(mg->L[l].x);
+    free(mg->L[l].r);
+    free(mg->L[l].z);
+  }
+  free(mg->L);
+  free(mg->lap2);
+  free(mg->mass2);
+  free(mg->rmat2);
+  free(mg->rmat1);
+  free(mg->lu);
+  free(mg->piv);
+  free(mg);
+}
+
+int or_mg_set_level(or_mg* mg, int l, uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist,
+                    const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+                    const uint32_t* anchor, const double* kappa, const double* mass, double lap_fac,
+                    double mass_fac) {
+  if (!mg || l < 0 || l >= mg->nlevels) return -1;
+  mg_level* L = &mg->L[l];
+  L->lv.E = E;
+  L->lv.N = N;
+  L->lv.max_depth = maxD;
+  L->lv.nlist = nlist;
+  L->lv.child = child;
+  L->lv.hnmask = hnmask;
+  L->lv.lev = lev;
+  L->lv.anchor = anchor;
+  L->kappa = kappa;
+  L->mass = mass;
+  L->lap_fac = lap_fac;
+  L->mass_fac = mass_fac;
+  return 0;
+}
+
+static void apply_A(or_mg* mg, int l, const double* x, double* y) {
+  mg_level* L = &mg->L[l];
+  or_matvec(L->lv.E, L->lv.N, L->lv.max_depth, L->lv.nlist, L->lv.child, L->lv.hnmask, L->lv.lev,
+            L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2, mg->mass2, x, y);
+}
+
+int or_mg_apply(or_mg* mg, int level, const double* x, double* y) {
+  if (!mg || level < 0 || level >= mg->nlevels) return -1;
+  apply_A(mg, level, x, y);
+  return 0;
+}
+
+static void project_mean(double* v, uint32_t n) {
+  double s = 0.0;
+  for (uint32_t i = 0; i < n; ++i) s += v[i];
+  s /= (double)n;
+  for (uint32_t i = 0; i < n; ++i) v[i] -= s;
+}
+
+/* Coarsest operator assembled from the same element blocks (CreateAndComputeFullJacobian1,
+ * odaJac.C:466-536; BUILD_FULL_JAC_TYPE2_BLOCK, omgJac.C:1803-1858) and factored densely
+ * (the reference's -mg_coarse_pc_type lu, bin/options:203-206).  With a constant null space
+ * the factored matrix is A + a 11^T (a = trace/n^2) applied to mean-free right-hand sides,
+ * which returns the mean-free solution of A x = b. */
+static int coarse_factor(or_mg* mg) {
+  mg_level* L = &mg->L[0];
+  const int n = (int)L->lv.N;
+  mg->nc = n;
+  mg->lu = (double*)calloc((size_t)n * n, sizeof(double));
+  mg->piv = (int*)malloc(sizeof(int) * n);
+  if (!mg->lu || !mg->piv) return -1;
+  double* e = (double*)calloc(n, sizeof(double));
+  double* col = (double*)malloc(sizeof(double) * n);
+  for (int j = 0; j < n; ++j) { /* column j = A e_j */
+    e[j] = 1.0;
+    apply_A(mg, 0, e, col);
+    e[j] = 0.0;
+    for (int i = 0; i < n; ++i) mg->lu[(size_t)i * n + j] = col[i];
+  }
+  free(e);
+  free(col);
+  if (mg->null_space) {
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += mg->lu[(size_t)i * n + i];
+    const double a = tr / ((double)n * n);
+    for (size_t q = 0; q < (size_t)n * n; ++q) mg->lu[q] += a;
+  }
+  for (int k = 0; k < n; ++k) { /* LU with partial pivoting, row-major */
+    int p = k;
+    double best = fabs(mg->lu[(size_t)k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(mg->lu[(size_t)i * n + k]) > best) {
+        best = fabs(mg->lu[(size_t)i * n + k]);
+        p = i;
+      }
+    mg->piv[k] = p;
+    if (best == 0.0) return -2;
+    if (p != k)
+      for (int j = 0; j < n; ++j) {
+        double t = mg->lu[(size_t)k * n + j];
+        mg->lu[(size_t)k * n + j] = mg->lu[(size_t)p * n + j];
+        mg->lu[(size_t)p * n + j] = t;
+      }
+    const double iv = 1.0 / mg->lu[(size_t)k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      double* ri = mg->lu + (size_t)i * n;
+      const double f = ri[k] * iv;
+      ri[k] = f;
+      if (f == 0.0) continue;
+      const double* rk = mg->lu + (size_t)k * n;
+      for (int j = k + 1; j < n; ++j) ri[j] -= f * rk[j];
+    }
+  }
+  return 0;
+}
+
+static void coarse_solve(or_mg* mg, const double* b, double* x) {
+  const int n = mg->nc;
+  for (int i = 0; i < n; ++i) x[i] = b[i];
+  if (mg->null_space) project_mean(x, (uint32_t)n);
+  for (int k = 0; k < n; ++k)
+    if (mg->piv[k] != k) {
+      double t = x[k];
+      x[k] = x[mg->piv[k]];
+      x[mg->piv[k]] = t;
+    }
+  for (int i = 0; i < n; ++i) {
+    double s = x[i];
+    const double* ri = mg->lu + (size_t)i * n;
+    for (int j = 0; j < i; ++j) s -= ri[j] * x[j];
+    x[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = x[i];
+    const double* ri = mg->lu + (size_t)i * n;
+    for (int j = i + 1; j < n; ++j) s -= ri[j] * x[j];
+    x[i] = s / ri[i];
+  }
+}
+
+int or_mg_setup(or_mg* mg, double omega, int nu_pre, int nu_post, int null_space) {
+  if (!mg) return -1;
+  mg->omega = omega;
+  mg->nu_pre = nu_pre;
+  mg->nu_post = nu_post;
+  mg->null_space = null_space;
+  for (int l = 0; l < mg->nlevels; ++l) {
+    mg_level* L = &mg->L[l];
+    const uint32_t N = L->lv.N;
+    L->invdiag = (double*)malloc(sizeof(double) * N);
+    L->b = (double*)calloc(N, sizeof(double));
+    L->x = (double*)calloc(N, sizeof(double));
+    L->r = (double*)calloc(N, sizeof(double));
+    L->z = (double*)calloc(N, sizeof(double));
+    or_diagonal(L->lv.E, N, L->lv.max_depth, L->lv.nlist, L->lv.child, L->lv.hnmask, L->lv.lev,
+                L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2, mg->mass2, L->invdiag);
+    /* PCSetUp_Jacobi: the reciprocal of the diagonal */
+    for (uint32_t i = 0; i < N; ++i) L->invdiag[i] = 1.0 / L->invdiag[i];
+  }
+  return coarse_factor(mg);
+}
+
+/* KSPRICHARDSON + PCJACOBI, norm type none (bin/options:187-193):
+ *   r = b (zero guess) or b - A x;  repeat nu: x += omega * invdiag .* r; r = b - A x
+ * (the residual after the last sweep is not formed). */
+static void richardson(or_mg* mg, int l, const double* b, double* x, int nu, int guess_zero) {
+  mg_level* L = &mg->L[l];
+  const uint32_t N = L->lv.N;
+  double* r = L->r;
+  if (guess_zero) {
+    memcpy(r, b, sizeof(double) * N);
+  } else {
+    apply_A(mg, l, x, r);
+    for (uint32_t i = 0; i < N; ++i) r[i] = b[i] - r[i];
+  }
+  for (int it = 0; it < nu; ++it) {
+    for (uint32_t i = 0; i < N; ++i) x[i] = x[i] + mg->omega * (r[i] * L->invdiag[i]);
+    if (it + 1 < nu) {
+      apply_A(mg, l, x, r);
+      for (uint32_t i = 0; i < N; ++i) r[i] = b[i] - r[i];
+    }
+  }
+}
+
+/* PCMGMCycle_Private, multiplicative V-cycle: pre-smooth (zero guess), residual, restrict,
+ * zero coarse guess, recurse (coarsest: direct solve), interpolate-add, post-smooth. */
+static int vcycle(or_mg* mg, int l, const double* b, double* x) {
+  if (l == 0) {
+    coarse_solve(mg, b, x);
+    return 0;
+  }
+  mg_level* L = &mg->L[l];
+  mg_level* Lc = &mg->L[l - 1];
+  const uint32_t N = L->lv.N;
+  memset(x, 0, sizeof(double) * N);
+  richardson(mg, l, b, x, mg->nu_pre, 1);
+  apply_A(mg, l, x, L->r);
+  for (uint32_t i = 0; i < N; ++i) L->r[i] = b[i] - L->r[i];
+  int rc = or_restrict(&Lc->lv, &L->lv, mg->rmat2, mg->rmat1, L->r, Lc->b);
+  if (rc) return rc;
+  memset(Lc->x, 0, sizeof(double) * Lc->lv.N);
+  rc = vcycle(mg, l - 1, Lc->b, Lc->x);
+  if (rc) return rc;
+  rc = or_prolong(&Lc->lv, &L->lv, mg->rmat2, mg->rmat1, Lc->x, L->z);
+  if (rc) return rc;
+  for (uint32_t i = 0; i < N; ++i) x[i] = x[i] + L->z[i];
+  richardson(mg, l, b, x, mg->nu_post, 0);
+  return 0;
+}
+
+int or_mg_vcycle(or_mg* mg, const double* r, double* z) {
+  if (!mg) return -1;
+  const int top = mg->nlevels - 1;
+  int rc = vcycle(mg, top, r, z);
+  if (mg->null_space) project_mean(z, mg->L[top].lv.N);
+  return rc;
+}
+
+static double dot(const double* a, const double* b, uint32_t n) {
+  double s = 0.0;
+  for (uint32_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* KSPSolve_CG (PETSc, left preconditioning), zero initial guess:
+ *   r = b; z = B r; dp = ||z|| or ||r||; beta = z.r
+ *   loop: p = z (+ beta/betaold p); w = A p; a = beta / p.w; x += a p; r -= a w;
+ *         z = B r; dp = norm; test dp <= rtol dp0; beta = z.r */
+int or_pcg(or_mg* mg, const double* b_in, double* x, double rtol, int max_it, int norm_type,
+           double* history, int history_len, int* converged) {
+  const int top = mg->nlevels - 1;
+  const uint32_t N = mg->L[top].lv.N;
+  double* b = (double*)malloc(sizeof(double) * N);
+  double* r = (double*)malloc(sizeof(double) * N);
+  double* z = (double*)malloc(sizeof(double) * N);
+  double* p = (double*)malloc(sizeof(double) * N);
+  double* w = (double*)malloc(sizeof(double) * N);
+  memcpy(b, b_in, sizeof(double) * N);
+  if (mg->null_space) project_mean(b, N);
+  memset(x, 0, sizeof(double) * N);
+  memcpy(r, b, sizeof(double) * N);
+  or_mg_vcycle(mg, r, z);
+  double dp = sqrt(norm_type ? dot(z, z, N) : dot(r, r, N));
+  const double dp0 = dp;
+  if (history && history_len > 0) history[0] = dp;
+  double beta = dot(z, r, N), betaold = 0.0;
+  int it = 0;
+  *converged = (dp0 == 0.0);
+  while (!*converged && it < max_it) {
+    if (it == 0) {
+      memcpy(p, z, sizeof(double) * N);
+    } else {
+      const double bb = beta / betaold;
+      for (uint32_t i = 0; i < N; ++i) p[i] = z[i] + bb * p[i];
+    }
+    apply_A(mg, top, p, w);
+    const double dpi = dot(p, w, N);
+    betaold = beta;
+    const double a = beta / dpi;
+    for (uint32_t i = 0; i < N; ++i) x[i] += a * p[i];
+    for (uint32_t i = 0; i < N; ++i) r[i] -= a * w[i];
+    or_mg_vcycle(mg, r, z);
+    dp = sqrt(norm_type ? dot(z, z, N) : dot(r, r, N));
+    ++it;
+    if (history && it < history_len) history[it] = dp;
+    if (!(dp == dp)) break; /* NaN */
+    if (dp <= rtol * dp0) *converged = 1;
+    beta = dot(z, r, N);
+  }
+  free(b);
+  free(r);
+  free(z);
+  free(p);
+  free(w);
+  return it;
+}

@@ -1,0 +1,80 @@
+/*
+ * oracle.h -- CPU restatement of the reference's matrix-free octree FEM solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (dendro_amd/, include/) may include,
+ * link or call this code; it is loaded by tests/, by __graft_entry__.smoke() as the
+ * checker, and by bench.py's cpu_baseline leg.  Every function follows the reference
+ * source cited beside it, in the reference's loop order and arithmetic.
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   or_stencils         Stencils/ *.inp tables of the reference (tests/golden fixture),
+ *                       symmetry / row-sum / partition-of-unity invariants
+ *   or_etype            include/oda/cnumEtypes.h (tests/golden fixture), exhaustive 8x256
+ *   or_matvec/diagonal  regular-grid closed form (textbook trilinear Q1 element), brute
+ *                       force conforming assembly on tiny meshes, A*1 = mass*1
+ *   or_restrict/prolong R^T = P exactly, R*1 = 1 on fine nodes, coarse-node injection
+ *   or_mg_* / or_pcg    textbook CG / V-cycle checks: symmetric preconditioner,
+ *                       monotone A-norm error, dense-solve agreement on tiny meshes.
+ *                       PETSc (the reference's KSPCG/PCMG/Richardson/Jacobi/LU) is not in
+ *                       the reference tree, so its iterates are "parity unpinned" beyond
+ *                       these mathematical checks.
+ */
+#ifndef DENDRO_ORACLE_H
+#define DENDRO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int or_etype(int child, int mask);
+void or_stencils(double* lap2, double* mass2, double* rmat2, double* rmat1);
+
+/* y = A x over one DA level; kappa == NULL selects Jacobian1 with lap_fac/mass_fac. */
+void or_matvec(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist,
+               const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+               const double* kappa, const double* mass, double lap_fac, double mass_fac,
+               const double* lap2, const double* mass2, const double* x, double* y);
+void or_diagonal(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist,
+                 const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+                 const double* kappa, const double* mass, double lap_fac, double mass_fac,
+                 const double* lap2, const double* mass2, double* d);
+
+/* One DA level as the transfer loops see it. */
+typedef struct {
+  uint32_t E, N, max_depth;
+  const uint32_t* nlist;   /* [E][8] */
+  const uint8_t* child;
+  const uint8_t* hnmask;
+  const uint8_t* lev;
+  const uint32_t* anchor;  /* [E][3] */
+} or_level;
+
+/* c = R f (restrictMatVecType1) and f = R^T c (prolongMatVecType1). */
+int or_restrict(const or_level* coarse, const or_level* fine, const double* rmat2,
+                const double* rmat1, const double* f, double* c);
+int or_prolong(const or_level* coarse, const or_level* fine, const double* rmat2,
+               const double* rmat1, const double* c, double* f);
+
+/* Multigrid hierarchy (coarsest level 0) for the V-cycle and PCG. */
+typedef struct or_mg_s or_mg;
+or_mg* or_mg_create(int nlevels);
+void or_mg_free(or_mg* mg);
+/* the arrays must stay alive while the hierarchy is used */
+int or_mg_set_level(or_mg* mg, int level, uint32_t E, uint32_t N, uint32_t max_depth,
+                    const uint32_t* nlist, const uint8_t* child, const uint8_t* hnmask,
+                    const uint8_t* lev, const uint32_t* anchor, const double* kappa,
+                    const double* mass, double lap_fac, double mass_fac);
+int or_mg_setup(or_mg* mg, double omega, int nu_pre, int nu_post, int null_space);
+int or_mg_vcycle(or_mg* mg, const double* r, double* z);
+/* PETSc-KSPCG restatement; returns iterations, *converged 0/1, history[0..its] norms */
+int or_pcg(or_mg* mg, const double* b, double* x, double rtol, int max_it, int norm_type,
+           double* history, int history_len, int* converged);
+/* y = A x on level `level` of the hierarchy (the level's own operator). */
+int or_mg_apply(or_mg* mg, int level, const double* x, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,168 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Bar (FP64): relative 2-norm error <= 1e-12 for the MatVec, diagonal, restriction,
+prolongation, one V-cycle and the first PCG iterates; node and element indexing are shared
+bit-exactly (both sides consume the same DA arrays).  FP32 paths state their error
+against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import dendro_amd as da
+import oracle
+from tests.meshes import hierarchy, rel_err, var_coefficients
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+TOL64 = 1e-12
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    da.init(0)
+    return torch.device("cuda:0")
+
+
+def _t(a, dev, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _meshes(h, coefs=None, lap_fac=1.0, mass_fac=1.0):
+    ms = []
+    for l in range(h.n):
+        m = da.Mesh(h.levels[l])
+        if coefs is None:
+            m.set_coefficients(None, None, lap_fac, mass_fac)
+        else:
+            m.set_coefficients(coefs[l][0], coefs[l][1])
+        ms.append(m)
+    ts = [da.Transfer(ms[l - 1], ms[l], *h.transfers[l - 1]) for l in range(1, h.n)]
+    return ms, ts
+
+
+@pytest.mark.parametrize("mode", ["jac1", "jac2", "jac2_nomass"])
+def test_matvec_fp64(dev, hier_small, mode):
+    h = hier_small
+    F = h.fine
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-1, 1, F.n_node)
+    if mode == "jac1":
+        coef, kw = None, dict(lap_fac=1.0, mass_fac=1.0)
+    else:
+        c = var_coefficients(h, mass_base=None if mode == "jac2_nomass" else 1.0)
+        coef, kw = c[-1], dict(kappa=c[-1][0], mass=c[-1][1])
+    y_ref = oracle.matvec(F, x, **kw)
+    m = da.Mesh(F)
+    if coef is None:
+        m.set_coefficients(None, None, 1.0, 1.0)
+    else:
+        m.set_coefficients(coef[0], coef[1])
+    xt = _t(x, dev)
+    yt = torch.empty_like(xt)
+    m.matvec(xt, yt)
+    assert rel_err(_np(yt), y_ref) <= TOL64
+
+
+def test_diagonal_fp64(dev, hier_small):
+    h = hier_small
+    c = var_coefficients(h, mass_base=1.0)
+    m = da.Mesh(h.fine)
+    m.set_coefficients(c[-1][0], c[-1][1])
+    d = torch.empty(h.fine.n_node, dtype=torch.float64, device=dev)
+    m.diagonal(d)
+    assert rel_err(_np(d), oracle.diagonal(h.fine, c[-1][0], c[-1][1])) <= TOL64
+
+
+def test_transfer_fp64(dev, hier_small):
+    h = hier_small
+    ms, ts = _meshes(h)
+    rng = np.random.default_rng(3)
+    for l in range(1, h.n):
+        C, F = h.levels[l - 1], h.levels[l]
+        f = rng.uniform(-1, 1, F.n_node)
+        c = rng.uniform(-1, 1, C.n_node)
+        ct = torch.empty(C.n_node, dtype=torch.float64, device=dev)
+        ts[l - 1].restrict(_t(f, dev), ct)
+        assert rel_err(_np(ct), oracle.restrict(C, F, f)) <= TOL64
+        fin = _t(f, dev)
+        fout = torch.empty_like(fin)
+        ts[l - 1].prolong_add(_t(c, dev), fin, fout)
+        assert rel_err(_np(fout), f + oracle.prolong(C, F, c)) <= TOL64
+
+
+def test_fp32_operators_error_stated(dev, hier_small):
+    h = hier_small
+    F = h.fine
+    x = np.random.default_rng(4).uniform(-1, 1, F.n_node)
+    m = da.Mesh(F)
+    m.set_coefficients(None, None, 1.0, 1.0)
+    xt = _t(x, dev, torch.float32)
+    yt = torch.empty_like(xt)
+    m.matvec(xt, yt, prec=da.F32)
+    err = rel_err(_np(yt), oracle.matvec(F, x))
+    print(f"FP32 MatVec relative error vs FP64 oracle: {err:.3e}")
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("null_space", [False, True])
+def test_vcycle_fp64(dev, hier_tiny, null_space):
+    h = hier_tiny
+    coefs = var_coefficients(h, mass_base=None if null_space else 1.0)
+    ms, ts = _meshes(h, coefs)
+    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=int(null_space))
+    s = da.Solver(ms, ts, p)
+    mg = oracle.MG([(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)],
+                   omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=null_space)
+    r = np.random.default_rng(7).uniform(-1, 1, h.fine.n_node)
+    if null_space:
+        r -= r.mean()
+    z = torch.empty(h.fine.n_node, dtype=torch.float64, device=dev)
+    s.vcycle(_t(r, dev), z)
+    assert rel_err(_np(z), mg.vcycle(r)) <= 1e-11
+
+
+@pytest.mark.parametrize("norm_type", [da.NORM_PRECONDITIONED, da.NORM_UNPRECONDITIONED])
+def test_pcg_iterates_fp64(dev, hier_tiny, norm_type):
+    h = hier_tiny
+    ms, ts = _meshes(h, None, 1.0, 1.0)
+    b = np.random.default_rng(12).uniform(-1, 1, h.fine.n_node)
+    mg = oracle.MG([(h.levels[l], None, None, 1.0, 1.0) for l in range(h.n)],
+                   omega=2.0 / 3.0, nu_pre=2, nu_post=2)
+    for k in (1, 3, 5):
+        p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=k, rtol=1e-30,
+                              norm_type=norm_type)
+        s = da.Solver(ms, ts, p)
+        x = torch.zeros(h.fine.n_node, dtype=torch.float64, device=dev)
+        st, hist = s.solve(_t(b, dev), x, history_len=k + 1)
+        xo, its, _, ho = mg.pcg(b, rtol=1e-30, max_it=k, norm_type=norm_type)
+        assert st.iterations == its == k
+        assert rel_err(_np(x), xo) <= TOL64, k
+        assert np.allclose(hist, ho, rtol=1e-10, atol=0)
+
+
+def test_pcg_converges_fp64_and_mixed(dev, hier_small):
+    h = hier_small
+    coefs = var_coefficients(h, mass_base=None)
+    ms, ts = _meshes(h, coefs)
+    b = np.random.default_rng(22).uniform(-1, 1, h.fine.n_node)
+    out = {}
+    for prec in (da.F64, da.F32):
+        p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=200, rtol=1e-8,
+                              norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=prec)
+        s = da.Solver(ms, ts, p)
+        x = torch.zeros(h.fine.n_node, dtype=torch.float64, device=dev)
+        st, _ = s.solve(_t(b, dev), x)
+        assert st.converged and st.rel_true <= 1.5e-8, (prec, st.iterations, st.rel_true)
+        out[prec] = _np(x)
+        print(f"prec={prec} its={st.iterations} rel_true={st.rel_true:.2e} t={st.seconds*1e3:.2f} ms")
+    err = rel_err(out[da.F32], out[da.F64])
+    print(f"mixed-precision solution error vs FP64 solve: {err:.2e}")
+    assert err <= 1e-6
