@@ -71,6 +71,10 @@ def main():
     ap.add_argument("--nlevels", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
+    ap.add_argument("--tile", type=int, default=0, help="operator tile size (0: library default)")
+    ap.add_argument("--sweep-tiles", action="store_true", help="time the MatVec for several tile sizes")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 256 MiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solve path is "
@@ -92,6 +96,9 @@ def main():
         meshes.append(m)
     trans = [da.Transfer(meshes[l - 1], meshes[l], *trans_arr[l - 1]) for l in range(1, len(levels))]
     mf = meshes[-1]
+    if args.tile:
+        mf.set_tile(args.tile)
+    log(f"tiling: {mf.tile_info()}")
 
     # ---- MatVec: K steps, inputs resident in HBM, events on the launch stream
     stream = torch.cuda.current_stream()
@@ -115,6 +122,32 @@ def main():
     achieved = alg_bytes / (ms_per * 1e-3) / 1e9
     log(f"MatVec FP64: {ms_per*1e3:.1f} us/launch, {gdofs:.2f} GDoF/s, {achieved:.0f} GB/s algorithmic "
         f"({100*achieved/HBM_PEAK_GBS:.1f}% of HBM peak); wall {wall:.3f}s")
+
+    if args.sweep_tiles:
+        for te, blk in [(t_, b_) for t_ in (512, 1024, 2048, 4096) for b_ in (256, 512, 1024)]:
+            try:
+                mf.set_tile(te, blk)
+            except da.DendroError as e:
+                log(f"tile {te}: {e}")
+                continue
+            for _ in range(5):
+                mf.matvec(x, y, stream=stream)
+            ev0.record(stream)
+            for _ in range(50):
+                mf.matvec(x, y, stream=stream)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            ti = mf.tile_info()
+            log(f"tile {te} block {blk}: {ev0.elapsed_time(ev1) / 50 * 1e3:.1f} us  {ti}")
+        mf.set_tile(args.tile or 2048, 512)
+
+    if args.calib_copy:
+        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        c = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            c.copy_(a)
+        torch.cuda.synchronize()
 
     # FP32 MatVec rate (V-cycle precision of the mixed solver)
     x32, y32 = x.float(), torch.empty(N, dtype=torch.float32, device=dev)

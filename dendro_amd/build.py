@@ -16,7 +16,8 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libdendrohip.so")
 
-HOST_SOURCES = ["host/stencils.cpp", "host/octree.cpp", "host/da.cpp", "host/host_api.cpp"]
+HOST_SOURCES = ["host/stencils.cpp", "host/octree.cpp", "host/da.cpp", "host/tiles.cpp",
+                "host/host_api.cpp"]
 HIP_SOURCES = ["device/device_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

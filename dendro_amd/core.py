@@ -28,6 +28,7 @@ EXPORTS = [
     "dgh_damg_free",
     "dgh_init", "dgh_finalize",
     "dgh_mesh_create", "dgh_mesh_set_coefficients", "dgh_mesh_info", "dgh_mesh_destroy",
+    "dgh_mesh_set_tile", "dgh_mesh_tile_info",
     "dgh_matvec", "dgh_diagonal", "dgh_matvec_multi",
     "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add",
     "dgh_solver_default_params", "dgh_solver_create", "dgh_solver_destroy", "dgh_solver_solve",
@@ -279,6 +280,17 @@ class Mesh:
         m = None if mass is None else np.ascontiguousarray(mass, np.float64)
         check(lib().dgh_mesh_set_coefficients(self.h, ptr(k), ptr(m), C.c_double(lap_fac),
                                               C.c_double(mass_fac)))
+
+    def set_tile(self, tile_elems, block=0):
+        check(lib().dgh_mesh_set_tile(self.h, int(tile_elems), int(block)))
+
+    def tile_info(self):
+        te, nt, lds = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        nr, nn = C.c_uint64(), C.c_uint64()
+        check(lib().dgh_mesh_tile_info(self.h, C.byref(te), C.byref(nt), C.byref(nr), C.byref(nn),
+                                       C.byref(lds)))
+        return {"tile_elems": te.value, "n_tiles": nt.value, "n_records": nr.value,
+                "n_node_entries": nn.value, "lds_bytes_f64": lds.value}
 
     def matvec(self, x, y, prec=F64, stream=None):
         check(lib().dgh_matvec(self.h, int(prec), ptr(x), ptr(y), _stream(stream)))

@@ -69,8 +69,9 @@ static inline unsigned blocks_for(size_t n, int bs = VBLOCK) { return unsigned((
 struct State {
   bool ready = false;
   int device = 0;
-  DevBuf<double> L64, M64, W64;   // [144][64], [144][64], [144][27][8]
-  DevBuf<float> L32, M32, W32;
+  int max_lds = 65536;            // dynamic LDS bytes a workgroup may use
+  DevBuf<double> K64, W64;        // child-0 stencil triangles [18][72]; transfer weights [144][27][8]
+  DevBuf<float> K32, W32;
   rocblas_handle blas = nullptr;
 };
 static State g;
@@ -79,14 +80,11 @@ static void require_init() {
   if (!g.ready) fail(DGH_ERR_STATE, "dgh_init has not been called");
 }
 
-template <class T> const T* Ltab();
-template <class T> const T* Mtab();
+template <class T> const T* Ktab();
 template <class T> const T* Wtab();
-template <> const double* Ltab<double>() { return g.L64.p; }
-template <> const double* Mtab<double>() { return g.M64.p; }
+template <> const double* Ktab<double>() { return g.K64.p; }
 template <> const double* Wtab<double>() { return g.W64.p; }
-template <> const float* Ltab<float>() { return g.L32.p; }
-template <> const float* Mtab<float>() { return g.M32.p; }
+template <> const float* Ktab<float>() { return g.K32.p; }
 template <> const float* Wtab<float>() { return g.W32.p; }
 
 }  // namespace dgh
@@ -97,59 +95,90 @@ using namespace dgh;
 struct dgh_mesh_s {
   uint32_t E = 0, N = 0, maxD = 0;
   double hFac = 1.0;
-  std::vector<uint32_t> perm;   // perm[original element] = device position
-  std::vector<TileInfo> h_tiles;
-  DevBuf<TileInfo> tiles;
-  DevBuf<uint32_t> nl;          // [E][8] device order
-  DevBuf<uint8_t> lev;          // [E]
-  DevBuf<uint8_t> cs;           // [E] stencil index 18c+t per element
-  DevBuf<double> kappa, mass;   // [E] device order (Jacobian2) or empty
-  double lap_fac = 1.0, mass_fac = 1.0;
-  // host copies of the small per-element arrays (device order) for assembly
+  // element arrays in the caller's (Morton) order: transfers and coarse assembly
   std::vector<uint32_t> h_nl;
-  std::vector<uint8_t> h_lev, h_cs;
+  std::vector<uint8_t> h_child, h_hn, h_lev, h_cs;
   std::vector<double> h_kappa, h_mass;
+  DevBuf<uint32_t> nl;          // [E][8]
+  DevBuf<uint8_t> cs;           // [E] stencil index 18c+t
+  double lap_fac = 1.0, mass_fac = 1.0;
+  // owner-computes tiling (tiles.cpp)
+  TilePlan plan;
+  uint32_t ys_len = 0;
+  int block = 512;              // threads per operator workgroup (256, 512 or 1024)
+  DevBuf<TileRec> tiles;
+  DevBuf<uint16_t> lid, meta;
+  DevBuf<uint32_t> node_tab;
+  DevBuf<double> rk64, rm64;    // per-record coefficients
+  DevBuf<float> rk32, rm32;
 };
 
 struct dgh_transfer_s {
   dgh_mesh coarse = nullptr, fine = nullptr;
-  DevBuf<uint32_t> celem;   // [N_f] coarse element (device order)
-  DevBuf<uint8_t> pos;      // [N_f]
+  DevBuf<uint32_t> celem;       // [N_f] coarse element holding each fine node
+  DevBuf<uint8_t> pos;          // [N_f] position code inside it
+  DevBuf<uint32_t> rowptr, col; // R = P^T in CSR (rows: coarse nodes)
+  DevBuf<float> w;              // exact dyadic weights
 };
 
 namespace dgh {
 
 template <class T>
-static void launch_matvec(dgh_mesh m, const T* x, T* y, hipStream_t st) {
-  HIP_CHECK(hipMemsetAsync(y, 0, sizeof(T) * m->N, st));
-  const int nt = int(m->h_tiles.size());
-  if (!nt) return;
+static size_t op_lds_bytes(const dgh_mesh m) {
+  return sizeof(T) * (size_t(NSTENCIL) * KSTRIDE + m->ys_len + m->plan.max_touch);
+}
+
+template <class T> const T* rk(dgh_mesh m);
+template <class T> const T* rm(dgh_mesh m);
+template <> const double* rk<double>(dgh_mesh m) { return m->rk64.p; }
+template <> const double* rm<double>(dgh_mesh m) { return m->rm64.p; }
+template <> const float* rk<float>(dgh_mesh m) { return m->rk32.p; }
+template <> const float* rm<float>(dgh_mesh m) { return m->rm32.p; }
+
+template <class T, int MODE, int BLOCK>
+static void launch_op_b(dgh_mesh m, const T* x, T* y, hipStream_t st) {
+  const int nt = int(m->plan.tiles.size());
   const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
-  hipLaunchKernelGGL(matvec_kernel<T>, dim3(grid), dim3(TILE), 0, st, m->tiles.p, nt, m->nl.p,
-                     m->lev.p, m->kappa.p, m->mass.p, m->lap_fac, m->mass_fac, m->maxD, m->hFac,
-                     Ltab<T>(), Mtab<T>(), x, y);
+  const size_t lds = op_lds_bytes<T>(m);
+  const T* kap = rk<T>(m);
+  const T* mas = rm<T>(m);
+  if (kap && !mas)
+    hipLaunchKernelGGL((tile_op_kernel<T, T, false, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->tiles.p,
+                       nt, m->lid.p, m->meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
+                       m->node_tab.p, m->ys_len, x, y);
+  else
+    hipLaunchKernelGGL((tile_op_kernel<T, T, true, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->tiles.p,
+                       nt, m->lid.p, m->meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
+                       m->node_tab.p, m->ys_len, x, y);
   HIP_CHECK(hipGetLastError());
+}
+
+template <class T, int MODE>
+static void launch_op(dgh_mesh m, const T* x, T* y, hipStream_t st) {
+  if (m->plan.tiles.empty()) return;
+  switch (m->block) {
+    case 256: launch_op_b<T, MODE, 256>(m, x, y, st); break;
+    case 512: launch_op_b<T, MODE, 512>(m, x, y, st); break;
+    default: launch_op_b<T, MODE, 1024>(m, x, y, st); break;
+  }
+}
+
+template <class T>
+static void launch_matvec(dgh_mesh m, const T* x, T* y, hipStream_t st) {
+  launch_op<T, OP_MATVEC>(m, x, y, st);
 }
 
 template <class T>
 static void launch_diag(dgh_mesh m, T* d, hipStream_t st) {
-  HIP_CHECK(hipMemsetAsync(d, 0, sizeof(T) * m->N, st));
-  const int nt = int(m->h_tiles.size());
-  if (!nt) return;
-  const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
-  hipLaunchKernelGGL(diag_kernel<T>, dim3(grid), dim3(TILE), 0, st, m->tiles.p, nt, m->nl.p,
-                     m->lev.p, m->kappa.p, m->mass.p, m->lap_fac, m->mass_fac, m->maxD, m->hFac,
-                     Ltab<T>(), Mtab<T>(), d);
-  HIP_CHECK(hipGetLastError());
+  launch_op<T, OP_DIAG>(m, static_cast<const T*>(nullptr), d, st);
 }
 
 template <class T>
 static void launch_restrict(dgh_transfer t, const T* f, T* c, hipStream_t st) {
-  HIP_CHECK(hipMemsetAsync(c, 0, sizeof(T) * t->coarse->N, st));
-  const uint32_t nf = t->fine->N;
-  if (!nf) return;
-  hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(nf)), dim3(VBLOCK), 0, st, nf, t->celem.p,
-                     t->pos.p, t->coarse->nl.p, t->coarse->cs.p, Wtab<T>(), f, c);
+  const uint32_t nc = t->coarse->N;
+  if (!nc) return;
+  hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(nc)), dim3(VBLOCK), 0, st, nc, t->rowptr.p, t->col.p,
+                     t->w.p, f, c);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -160,6 +189,40 @@ static void launch_prolong(dgh_transfer t, const T* c, const T* f_in, T* f_out, 
   hipLaunchKernelGGL(prolong_kernel<T>, dim3(blocks_for(nf)), dim3(VBLOCK), 0, st, nf, t->celem.p,
                      t->pos.p, t->coarse->nl.p, t->coarse->cs.p, Wtab<T>(), c, f_in, f_out);
   HIP_CHECK(hipGetLastError());
+}
+
+// (Re)builds the tiling of a mesh with the largest tile size <= T whose LDS fits.
+static void plan_mesh(dgh_mesh m, uint32_t T) {
+  for (;;) {
+    build_tiles(m->E, m->N, m->h_nl.data(), m->h_child.data(), m->h_hn.data(), m->h_lev.data(), T, m->plan);
+    m->ys_len = (m->plan.max_own + 1) & ~1u;
+    if (op_lds_bytes<double>(m) <= size_t(g.max_lds) || T <= 64) break;
+    T /= 2;
+  }
+  if (op_lds_bytes<double>(m) > size_t(g.max_lds))
+    fail(DGH_ERR_ARG, "operator tiles need %zu bytes of LDS (limit %d)", op_lds_bytes<double>(m), g.max_lds);
+  m->tiles.upload(m->plan.tiles.data(), m->plan.tiles.size());
+  m->lid.upload(m->plan.lid.data(), m->plan.lid.size());
+  m->meta.upload(m->plan.meta.data(), m->plan.meta.size());
+  m->node_tab.upload(m->plan.node_tab.data(), m->plan.node_tab.size());
+  // per-record coefficient copies follow the new record order
+  m->rk64.release();
+  m->rm64.release();
+  m->rk32.release();
+  m->rm32.release();
+  const size_t R = m->plan.rec_elem.size();
+  if (!m->h_kappa.empty()) {
+    std::vector<double> k(R);
+    std::vector<float> kf(R);
+    for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = m->h_kappa[m->plan.rec_elem[r]]);
+    m->rk64.upload(k.data(), R);
+    m->rk32.upload(kf.data(), R);
+    if (!m->h_mass.empty()) {
+      for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = m->h_mass[m->plan.rec_elem[r]]);
+      m->rm64.upload(k.data(), R);
+      m->rm32.upload(kf.data(), R);
+    }
+  }
 }
 
 }  // namespace dgh
@@ -350,15 +413,22 @@ int dgh_init(int device) {
   return guarded([&] {
     HIP_CHECK(hipSetDevice(device));
     g.device = device;
-    std::vector<double> L(9216), M(9216), W(144 * 27 * 8);
+    int lds = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    g.max_lds = lds > 0 ? lds : 65536;
+    std::vector<double> L(9216), M(9216), W(144 * 27 * 8), K(NSTENCIL * KSTRIDE);
     generate_stencils(L.data(), M.data(), nullptr, nullptr);
     generate_transfer_weights(W.data());
-    g.L64.upload(L.data(), L.size());
-    g.M64.upload(M.data(), M.size());
+    for (int t = 0; t < NSTENCIL; ++t)   // child-0 stencils: [t][L triangle, M triangle]
+      for (int k = 0; k < 8; ++k)
+        for (int j = k; j < 8; ++j) {
+          K[t * KSTRIDE + tri(k, j)] = L[size_t(t) * 64 + k * 8 + j];
+          K[t * KSTRIDE + 36 + tri(k, j)] = M[size_t(t) * 64 + k * 8 + j];
+        }
+    g.K64.upload(K.data(), K.size());
     g.W64.upload(W.data(), W.size());
-    std::vector<float> Lf(L.begin(), L.end()), Mf(M.begin(), M.end()), Wf(W.begin(), W.end());
-    g.L32.upload(Lf.data(), Lf.size());
-    g.M32.upload(Mf.data(), Mf.size());
+    std::vector<float> Kf(K.begin(), K.end()), Wf(W.begin(), W.end());
+    g.K32.upload(Kf.data(), Kf.size());
     g.W32.upload(Wf.data(), Wf.size());
     g.ready = true;
   });
@@ -366,11 +436,9 @@ int dgh_init(int device) {
 
 int dgh_finalize(void) {
   return guarded([&] {
-    g.L64.release();
-    g.M64.release();
+    g.K64.release();
     g.W64.release();
-    g.L32.release();
-    g.M32.release();
+    g.K32.release();
     g.W32.release();
     if (g.blas) rocblas_destroy_handle(g.blas);
     g.blas = nullptr;
@@ -384,50 +452,51 @@ int dgh_mesh_create(uint32_t n_elem, uint32_t n_node, uint32_t max_depth, const 
     require_init();
     if (!out || (n_elem && (!nlist || !child || !hn_mask || !lev))) fail(DGH_ERR_ARG, "NULL argument");
     if (max_depth < 2 || max_depth > 21) fail(DGH_ERR_ARG, "max_depth %u out of range", max_depth);
+    if (n_elem == 0 || n_node == 0) fail(DGH_ERR_ARG, "empty mesh");
     auto m = std::make_unique<dgh_mesh_s>();
     m->E = n_elem;
     m->N = n_node;
     m->maxD = max_depth;
     m->hFac = 1.0 / double(1u << (max_depth - 1));
-    // stencil index and class of every element
-    std::vector<uint8_t> cs(n_elem);
-    std::vector<uint32_t> count(144, 0);
+    m->h_cs.resize(n_elem);
     for (uint32_t e = 0; e < n_elem; ++e) {
       const int t = etype_of(child[e], hn_mask[e]);
       if (t < 0) fail(DGH_ERR_MESH, "element %u: hanging mask %u invalid for child %u", e, hn_mask[e], child[e]);
       if (lev[e] < 1 || lev[e] >= max_depth + 1) fail(DGH_ERR_MESH, "element %u: level %u out of range", e, lev[e]);
-      cs[e] = uint8_t(18 * child[e] + t);
-      count[t == 0 ? 0 : cs[e]]++;
+      m->h_cs[e] = uint8_t(18 * child[e] + t);
       for (int k = 0; k < 8; ++k)
         if (nlist[size_t(e) * 8 + k] >= n_node) fail(DGH_ERR_MESH, "element %u: node index out of range", e);
     }
-    // class offsets and tiles
-    std::vector<uint32_t> off(145, 0);
-    for (int c = 0; c < 144; ++c) off[c + 1] = off[c] + count[c];
-    for (int c = 0; c < 144; ++c) {
-      for (uint32_t s = off[c]; s < off[c + 1]; s += TILE)
-        m->h_tiles.push_back(TileInfo{s, std::min<uint32_t>(TILE, off[c + 1] - s), uint32_t(c), 0});
-    }
-    m->perm.resize(n_elem);
-    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
-    for (uint32_t e = 0; e < n_elem; ++e) {
-      const int cls = (hn_mask[e] == 0) ? 0 : cs[e];
-      m->perm[e] = fill[cls]++;
-    }
-    m->h_nl.resize(size_t(n_elem) * 8);
-    m->h_lev.resize(n_elem);
-    m->h_cs.resize(n_elem);
-    for (uint32_t e = 0; e < n_elem; ++e) {
-      const uint32_t d = m->perm[e];
-      std::memcpy(&m->h_nl[size_t(d) * 8], nlist + size_t(e) * 8, 32);
-      m->h_lev[d] = lev[e];
-      m->h_cs[d] = cs[e];
-    }
+    m->h_nl.assign(nlist, nlist + size_t(n_elem) * 8);
+    m->h_child.assign(child, child + n_elem);
+    m->h_hn.assign(hn_mask, hn_mask + n_elem);
+    m->h_lev.assign(lev, lev + n_elem);
     m->nl.upload(m->h_nl.data(), m->h_nl.size());
-    m->lev.upload(m->h_lev.data(), n_elem);
     m->cs.upload(m->h_cs.data(), n_elem);
-    m->tiles.upload(m->h_tiles.data(), m->h_tiles.size());
+    plan_mesh(m.get(), 2048);
     *out = m.release();
+  });
+}
+
+int dgh_mesh_set_tile(dgh_mesh m, uint32_t tile_elems, uint32_t block) {
+  return guarded([&] {
+    require_init();
+    if (!m) fail(DGH_ERR_ARG, "NULL mesh");
+    if (block != 0 && block != 256 && block != 512 && block != 1024) fail(DGH_ERR_ARG, "block must be 256, 512 or 1024");
+    if (block) m->block = int(block);
+    plan_mesh(m, tile_elems);
+  });
+}
+
+int dgh_mesh_tile_info(dgh_mesh m, uint32_t* tile_elems, uint32_t* n_tiles, uint64_t* n_records,
+                       uint64_t* n_node_entries, uint32_t* lds_bytes_f64) {
+  return guarded([&] {
+    if (!m) fail(DGH_ERR_ARG, "NULL mesh");
+    if (tile_elems) *tile_elems = m->plan.tile_elems;
+    if (n_tiles) *n_tiles = uint32_t(m->plan.tiles.size());
+    if (n_records) *n_records = m->plan.rec_elem.size();
+    if (n_node_entries) *n_node_entries = m->plan.node_tab.size();
+    if (lds_bytes_f64) *lds_bytes_f64 = uint32_t(op_lds_bytes<double>(m));
   });
 }
 
@@ -436,20 +505,28 @@ int dgh_mesh_set_coefficients(dgh_mesh m, const double* kappa, const double* mas
   return guarded([&] {
     require_init();
     if (!m) fail(DGH_ERR_ARG, "NULL mesh");
+    if (!kappa && mass) fail(DGH_ERR_ARG, "mass without kappa");
     m->lap_fac = lap_fac;
     m->mass_fac = mass_fac;
-    m->kappa.release();
-    m->mass.release();
     m->h_kappa.clear();
     m->h_mass.clear();
+    if (kappa) m->h_kappa.assign(kappa, kappa + m->E);
+    if (mass) m->h_mass.assign(mass, mass + m->E);
+    const size_t R = m->plan.rec_elem.size();
+    m->rk64.release();
+    m->rm64.release();
+    m->rk32.release();
+    m->rm32.release();
     if (kappa) {
-      m->h_kappa.resize(m->E);
-      for (uint32_t e = 0; e < m->E; ++e) m->h_kappa[m->perm[e]] = kappa[e];
-      m->kappa.upload(m->h_kappa.data(), m->E);
+      std::vector<double> k(R);
+      std::vector<float> kf(R);
+      for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = kappa[m->plan.rec_elem[r]]);
+      m->rk64.upload(k.data(), R);
+      m->rk32.upload(kf.data(), R);
       if (mass) {
-        m->h_mass.resize(m->E);
-        for (uint32_t e = 0; e < m->E; ++e) m->h_mass[m->perm[e]] = mass[e];
-        m->mass.upload(m->h_mass.data(), m->E);
+        for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = mass[m->plan.rec_elem[r]]);
+        m->rm64.upload(k.data(), R);
+        m->rm32.upload(kf.data(), R);
       }
     }
   });
@@ -509,17 +586,44 @@ int dgh_transfer_create(dgh_mesh coarse, dgh_mesh fine, const uint32_t* coarse_e
   return guarded([&] {
     require_init();
     if (!coarse || !fine || !out || (fine->N && (!coarse_elem || !pos))) fail(DGH_ERR_ARG, "NULL argument");
+    if (coarse->maxD != fine->maxD) fail(DGH_ERR_ARG, "levels of different max depth");
     auto t = std::make_unique<dgh_transfer_s>();
     t->coarse = coarse;
     t->fine = fine;
-    std::vector<uint32_t> ce(fine->N);
-    for (uint32_t j = 0; j < fine->N; ++j) {
+    const uint32_t nf = fine->N, nc = coarse->N;
+    for (uint32_t j = 0; j < nf; ++j) {
       if (coarse_elem[j] >= coarse->E) fail(DGH_ERR_MESH, "fine node %u: coarse element out of range", j);
       if (pos[j] >= 27) fail(DGH_ERR_MESH, "fine node %u: position code out of range", j);
-      ce[j] = coarse->perm[coarse_elem[j]];
     }
-    t->celem.upload(ce.data(), ce.size());
-    t->pos.upload(pos, fine->N);
+    t->celem.upload(coarse_elem, nf);
+    t->pos.upload(pos, nf);
+    // R = P^T with P[j][cnl[k]] = W[s][pos_j][k]: rows are coarse nodes, columns fine nodes
+    std::vector<double> W(144 * 27 * 8);
+    generate_transfer_weights(W.data());
+    std::vector<uint32_t> rowptr(size_t(nc) + 1, 0);
+    for (uint32_t j = 0; j < nf; ++j) {
+      const uint32_t ce = coarse_elem[j];
+      const double* w = W.data() + (size_t(coarse->h_cs[ce]) * 27 + pos[j]) * 8;
+      for (int k = 0; k < 8; ++k)
+        if (w[k] != 0.0) rowptr[coarse->h_nl[size_t(ce) * 8 + k] + 1]++;
+    }
+    for (uint32_t i = 0; i < nc; ++i) rowptr[i + 1] += rowptr[i];
+    std::vector<uint32_t> col(rowptr[nc]), fillp(rowptr.begin(), rowptr.end() - 1);
+    std::vector<float> wv(rowptr[nc]);
+    for (uint32_t j = 0; j < nf; ++j) {
+      const uint32_t ce = coarse_elem[j];
+      const double* w = W.data() + (size_t(coarse->h_cs[ce]) * 27 + pos[j]) * 8;
+      for (int k = 0; k < 8; ++k) {
+        if (w[k] == 0.0) continue;
+        const uint32_t q = fillp[coarse->h_nl[size_t(ce) * 8 + k]]++;
+        col[q] = j;
+        wv[q] = float(w[k]);
+        if (double(wv[q]) != w[k]) fail(DGH_ERR_MESH, "transfer weight %.17g is not exact in FP32", w[k]);
+      }
+    }
+    t->rowptr.upload(rowptr.data(), rowptr.size());
+    t->col.upload(col.data(), col.size());
+    t->w.upload(wv.data(), wv.size());
     *out = t.release();
   });
 }

@@ -1,31 +1,25 @@
-// CDNA4 (gfx950) kernels of libdendrohip.
+// CDNA4 (gfx950) operator and transfer kernels of libdendrohip.
 //
-// Element data live on the device grouped by stencil class: class 0 holds every element
-// without hanging vertices (its Laplacian and mass stencils are the same for all 8 child
-// numbers), classes 18c+t (t > 0) hold the hanging elements of child c and type t; each
-// class is in Morton order.  A workgroup processes one tile of TILE elements of a single
-// class, so the stencil index is uniform across the workgroup and its 8x8 blocks are read
-// through the scalar cache.
+// The operator kernel is owner-computes over tiles of Morton-ordered elements (the host
+// plan of tiles.cpp): one workgroup per tile stages the tile's x values and the 18 child-0
+// stencils in LDS, evaluates every record (own and halo elements), accumulates the rows of
+// the nodes the tile owns with LDS atomics, and stores each owned node exactly once.  There
+// are no global atomics and y needs no zeroing; the result is deterministic up to the order
+// of the LDS additions.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "host/octree.hpp"
+
 namespace dgh {
 
-constexpr int TILE = 256;        // elements per workgroup (4 wavefronts)
 constexpr int NXCD = 8;          // XCDs of an MI355X; workgroups are dealt round-robin
-
-struct TileInfo {
-  uint32_t start;   // first element (device order)
-  uint32_t count;   // elements in the tile (<= TILE)
-  uint32_t s;       // stencil index 18c+t of the class (class 0: s = 0)
-  uint32_t pad;
-};
+constexpr int NSTENCIL = 18;     // child-0 stencil types
 
 // Maps the hardware workgroup id to a tile so that each XCD works on one contiguous
-// range of tiles: neighbouring elements share nodes, and keeping them on one XCD keeps
-// their scatter-adds in one L2.
+// range of tiles: neighbouring tiles share halo nodes, whose x values then meet in one L2.
 __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
   const int per = (ntiles + NXCD - 1) / NXCD;
   return (bid % NXCD) * per + bid / NXCD;
@@ -33,76 +27,148 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
 
 template <class T>
 __device__ __forceinline__ void atomic_add(T* p, T v) {
-  unsafeAtomicAdd(p, v);   // global_atomic_add_f64 / _f32 (memory from hipMalloc)
+  unsafeAtomicAdd(p, v);
 }
 
-// Scalar element factors exactly as the reference forms them
-// (odaJac.C:271-276, omgJac.C:1665-1672): h = hFac * 2^(maxD - lev), hFac = 2^-(maxD-1),
-// fac1 = kappa h / 2, fac2 = mass h^3 / 8.
+// Element factors exactly as the reference forms them (odaJac.C:271-276,
+// omgJac.C:1665-1672): h = hFac 2^(maxD - lev), fac1 = kappa h / 2, fac2 = mass h^3 / 8.
 template <class T>
-__device__ __forceinline__ void elem_factors(uint32_t lev, uint32_t maxD, double hFac,
-                                             double k1, double k2, T& fac1, T& fac2) {
+__device__ __forceinline__ void elem_factors(uint32_t lev, uint32_t maxD, double hFac, double k1,
+                                             double k2, T& fac1, T& fac2) {
   const double h = hFac * double(1u << (maxD - lev));
   fac1 = T(k1 * h / 2.0);
   fac2 = T(k2 * h * h * h / 8.0);
 }
 
-// y += A x over the tiles.  nl: [E][8] node ids, lev: [E] levels, kappa/mass: [E]
-// (nullptr: uniform lap_fac / mass_fac).  LT/MT: stencils of this precision [144][64].
-template <class T>
-__global__ void __launch_bounds__(TILE)
-matvec_kernel(const TileInfo* __restrict__ tiles, int ntiles, const uint32_t* __restrict__ nl,
-              const uint8_t* __restrict__ lev, const double* __restrict__ kappa,
-              const double* __restrict__ mass, double lap_fac, double mass_fac, uint32_t maxD,
-              double hFac, const T* __restrict__ LT, const T* __restrict__ MT,
-              const T* __restrict__ x, T* __restrict__ y) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t >= ntiles) return;
-  const TileInfo ti = tiles[t];
-  if (threadIdx.x >= ti.count) return;
-  const uint32_t e = ti.start + threadIdx.x;
-  const uint4 a = reinterpret_cast<const uint4*>(nl)[2 * e];
-  const uint4 b = reinterpret_cast<const uint4*>(nl)[2 * e + 1];
-  const uint32_t idx[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+enum OpMode { OP_MATVEC = 0, OP_DIAG = 1 };
+
+// Symmetric 8x8 stencils are kept as their upper triangles (36 entries, row-major):
+// entry (k, j) with k <= j sits at tri(k, j) = k*8 - k*(k-1)/2 + (j - k).
+__host__ __device__ constexpr int tri(int k, int j) {
+  return (k <= j) ? (k * 8 - k * (k - 1) / 2 + (j - k)) : (j * 8 - j * (j - 1) / 2 + (k - j));
+}
+constexpr int KSTRIDE = 72;   // per stencil type: Laplacian triangle then mass triangle
+
+struct Rec {
+  uint32_t l[8];
+  uint32_t mt;
+  double k1, k2;
+};
+
+template <class TC, bool MASS>
+__device__ __forceinline__ void load_rec(Rec& R, uint32_t g, const uint16_t* __restrict__ lid,
+                                         const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
+                                         const TC* __restrict__ mass, double lap_fac, double mass_fac) {
+  const uint4 lp = reinterpret_cast<const uint4*>(lid)[g];
+  R.l[0] = lp.x & 0xffffu; R.l[1] = lp.x >> 16; R.l[2] = lp.y & 0xffffu; R.l[3] = lp.y >> 16;
+  R.l[4] = lp.z & 0xffffu; R.l[5] = lp.z >> 16; R.l[6] = lp.w & 0xffffu; R.l[7] = lp.w >> 16;
+  R.mt = meta[g];
+  R.k1 = kappa ? double(kappa[g]) : lap_fac;
+  R.k2 = kappa ? (MASS ? double(mass[g]) : 0.0) : mass_fac;
+}
+
+// Evaluates one record with stencil triangles Ls/Ms (LDS or, when the stencil type is
+// uniform across the wavefront, global memory read through the scalar cache).
+template <class T, bool MASS, int MODE>
+__device__ __forceinline__ void eval_rec_with(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac,
+                                              const T* Ls, const T* Ms, const T* __restrict__ xs,
+                                              T* __restrict__ ys) {
   T fac1, fac2;
-  elem_factors<T>(lev[e], maxD, hFac, kappa ? kappa[e] : lap_fac,
-                  kappa ? (mass ? mass[e] : 0.0) : mass_fac, fac1, fac2);
-  T xv[8];
+  elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
+  if (MODE == OP_MATVEC) {
+    T xv[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) xv[j] = x[idx[j]];
-  const T* L = LT + ti.s * 64;
-  const T* M = MT + ti.s * 64;
+    for (int j = 0; j < 8; ++j) xv[j] = xs[R.l[j]];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    T acc = T(0);
+    for (int k = 0; k < 8; ++k) {
+      if (R.l[k] < n_own) {
+        T acc = T(0);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) acc += (fac1 * L[k * 8 + j] + fac2 * M[k * 8 + j]) * xv[j];
-    atomic_add(&y[idx[k]], acc);
+        for (int j = 0; j < 8; ++j) {
+          if (MASS)
+            acc += (fac1 * Ls[tri(k, j)] + fac2 * Ms[tri(k, j)]) * xv[j];
+          else
+            acc += (fac1 * Ls[tri(k, j)]) * xv[j];
+        }
+        atomic_add(&ys[R.l[k]], acc);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (R.l[k] < n_own) atomic_add(&ys[R.l[k]], fac1 * Ls[tri(k, k)] + (MASS ? fac2 * Ms[tri(k, k)] : T(0)));
   }
 }
 
-// d += diag(A)
-template <class T>
-__global__ void __launch_bounds__(TILE)
-diag_kernel(const TileInfo* __restrict__ tiles, int ntiles, const uint32_t* __restrict__ nl,
-            const uint8_t* __restrict__ lev, const double* __restrict__ kappa,
-            const double* __restrict__ mass, double lap_fac, double mass_fac, uint32_t maxD,
-            double hFac, const T* __restrict__ LT, const T* __restrict__ MT, T* __restrict__ d) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t >= ntiles) return;
-  const TileInfo ti = tiles[t];
-  if (threadIdx.x >= ti.count) return;
-  const uint32_t e = ti.start + threadIdx.x;
-  T fac1, fac2;
-  elem_factors<T>(lev[e], maxD, hFac, kappa ? kappa[e] : lap_fac,
-                  kappa ? (mass ? mass[e] : 0.0) : mass_fac, fac1, fac2);
-  const T* L = LT + ti.s * 64;
-  const T* M = MT + ti.s * 64;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) atomic_add(&d[nl[8 * e + k]], fac1 * L[k * 9] + fac2 * M[k * 9]);
+template <class T, bool MASS, int MODE>
+__device__ __forceinline__ void eval_rec(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac,
+                                         const T* __restrict__ sK, const T* __restrict__ K0,
+                                         const T* __restrict__ xs, T* __restrict__ ys) {
+  const uint32_t tp = R.mt & 31u;
+  const uint32_t tp0 = __builtin_amdgcn_readfirstlane(tp);
+  if (__all(tp == tp0)) {
+    const T* Ls = K0 + tp0 * KSTRIDE;   // wave-uniform: scalar loads
+    eval_rec_with<T, MASS, MODE>(R, n_own, maxD, hFac, Ls, Ls + 36, xs, ys);
+  } else {
+    const T* Ls = sK + tp * KSTRIDE;
+    eval_rec_with<T, MASS, MODE>(R, n_own, maxD, hFac, Ls, Ls + 36, xs, ys);
+  }
 }
 
-// f_out[j] = f_in[j] + sum_k W[s][pos][k] c[cnl[k]] for every fine node j (prolongation,
+// K0: [18][72] child-0 Laplacian then mass upper triangles of precision T.
+// kappa/mass: per record (nullptr: uniform lap_fac / mass_fac).  MASS: mass term present.
+template <class T, class TC, bool MASS, int MODE, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
+               const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
+               const TC* __restrict__ mass, double lap_fac, double mass_fac, uint32_t maxD,
+               double hFac, const T* __restrict__ K0, const uint32_t* __restrict__ node_tab,
+               uint32_t ys_len, const T* __restrict__ x, T* __restrict__ y) {
+  // LDS: stencils [18][72], owned-node sums [ys_len >= n_own], x of touched nodes
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sK = reinterpret_cast<T*>(smem_raw);
+  T* ys = sK + NSTENCIL * KSTRIDE;
+  T* xs = ys + ys_len;
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t >= ntiles) return;
+  const TileRec rec = tiles[t];
+  const uint32_t* nodes = node_tab + rec.node_off;
+  for (int i = threadIdx.x; i < NSTENCIL * KSTRIDE; i += BLOCK) sK[i] = K0[i];
+  if (MODE == OP_MATVEC) {
+    // 4 independent gathers in flight per thread
+    uint32_t i = threadIdx.x;
+    for (; i + 3 * BLOCK < rec.n_touch; i += 4 * BLOCK) {
+      const uint32_t n0 = nodes[i], n1 = nodes[i + BLOCK], n2 = nodes[i + 2 * BLOCK], n3 = nodes[i + 3 * BLOCK];
+      const T v0 = x[n0], v1 = x[n1], v2 = x[n2], v3 = x[n3];
+      xs[i] = v0;
+      xs[i + BLOCK] = v1;
+      xs[i + 2 * BLOCK] = v2;
+      xs[i + 3 * BLOCK] = v3;
+    }
+    for (; i < rec.n_touch; i += BLOCK) xs[i] = x[nodes[i]];
+  }
+  for (uint32_t i = threadIdx.x; i < rec.n_own; i += BLOCK) ys[i] = T(0);
+  __syncthreads();
+  const uint32_t n_own = rec.n_own;
+  // two records per thread in flight: both loads are issued before either is evaluated
+  uint32_t r = threadIdx.x;
+  for (; r + BLOCK < rec.n_elem; r += 2 * BLOCK) {
+    Rec A, B;
+    load_rec<TC, MASS>(A, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
+    load_rec<TC, MASS>(B, rec.elem_off + r + BLOCK, lid, meta, kappa, mass, lap_fac, mass_fac);
+    eval_rec<T, MASS, MODE>(A, n_own, maxD, hFac, sK, K0, xs, ys);
+    eval_rec<T, MASS, MODE>(B, n_own, maxD, hFac, sK, K0, xs, ys);
+  }
+  if (r < rec.n_elem) {
+    Rec A;
+    load_rec<TC, MASS>(A, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
+    eval_rec<T, MASS, MODE>(A, n_own, maxD, hFac, sK, K0, xs, ys);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) y[nodes[i]] = ys[i];
+}
+
+// f_out[j] = f_in[j] + sum_k W[s][pos][k] c[cnl[k]] for every fine node j (prolongation
 // P = R^T): the coarse shape functions of the coarse element holding node j.
 template <class T>
 __global__ void __launch_bounds__(256)
@@ -126,28 +192,19 @@ prolong_kernel(uint32_t nf, const uint32_t* __restrict__ celem, const uint8_t* _
   f_out[j] = (f_in ? f_in[j] : T(0)) + acc;
 }
 
-// c[cnl[k]] += W[s][pos][k] f[j] for every fine node j (restriction R = P^T); c zeroed.
+// c[i] = sum_p w_p f[col_p] over row i of R = P^T (CSR built on the host; gather, no atomics).
 template <class T>
 __global__ void __launch_bounds__(256)
-restrict_kernel(uint32_t nf, const uint32_t* __restrict__ celem, const uint8_t* __restrict__ pos,
-                const uint32_t* __restrict__ cnl, const uint8_t* __restrict__ cs,
-                const T* __restrict__ W, const T* __restrict__ f, T* __restrict__ c) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nf) return;
-  const uint32_t ce = celem[j];
-  const T* w = W + (size_t(cs[ce]) * 27 + pos[j]) * 8;
-  const uint4 a = reinterpret_cast<const uint4*>(cnl)[2 * ce];
-  const uint4 b = reinterpret_cast<const uint4*>(cnl)[2 * ce + 1];
-  const uint32_t idx[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-  const T fj = f[j];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const T wk = w[k];
-    if (wk != T(0)) atomic_add(&c[idx[k]], wk * fj);
-  }
+restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                const float* __restrict__ w, const T* __restrict__ f, T* __restrict__ c) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  T acc = T(0);
+  const uint32_t p1 = rowptr[i + 1];
+  for (uint32_t p = rowptr[i]; p < p1; ++p) acc += T(w[p]) * f[col[p]];
+  c[i] = acc;
 }
 
-// ---------------------------------------------------------------- vector kernels
 template <class T>
 __global__ void fill_kernel(T* __restrict__ v, T val, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

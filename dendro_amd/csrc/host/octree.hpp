@@ -42,6 +42,27 @@ void build_da(const std::vector<Oct>& tree, uint32_t D, DALevel& da);
 void build_transfer(const DALevel& coarse, const DALevel& fine, std::vector<uint32_t>& celem,
                     std::vector<uint8_t>& pos);
 
+// ---- owner-computes tiling of one level (tiles.cpp)
+struct TileRec {
+  uint32_t elem_off;   // first record of the tile
+  uint32_t n_elem;     // records (own + halo elements)
+  uint32_t node_off;   // first entry of the tile's node table
+  uint32_t n_touch;    // nodes touched by the records (owned first)
+  uint32_t n_own;      // nodes owned (stored) by the tile
+  uint32_t pad[3];
+};
+struct TilePlan {
+  uint32_t tile_elems = 0;
+  std::vector<TileRec> tiles;
+  std::vector<uint16_t> lid;        // [R][8] tile-local node ids, vertex order k^child
+  std::vector<uint16_t> meta;       // [R] child-0 stencil type | level << 5
+  std::vector<uint32_t> rec_elem;   // [R] element of each record
+  std::vector<uint32_t> node_tab;   // global node ids, per tile: owned then others
+  uint32_t max_touch = 0, max_own = 0, max_elem = 0;
+};
+void build_tiles(uint32_t E, uint32_t N, const uint32_t* nl, const uint8_t* child, const uint8_t* hn,
+                 const uint8_t* lev, uint32_t T, TilePlan& plan);
+
 // Matching of coarse elements to fine elements (same element or 8 children), in order:
 // first fine element of each coarse element and whether it is the same element.
 void match_levels(const DALevel& coarse, const DALevel& fine, std::vector<uint32_t>& first_fine,

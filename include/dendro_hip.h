@@ -168,6 +168,15 @@ int dgh_mesh_create(uint32_t n_elem, uint32_t n_node, uint32_t max_depth, const 
 int dgh_mesh_set_coefficients(dgh_mesh m, const double* kappa, const double* mass,
                               double lap_fac, double mass_fac);
 int dgh_mesh_info(dgh_mesh m, uint32_t* n_elem, uint32_t* n_node);
+/* Re-tiles the operator kernel with tiles of at most tile_elems Morton-consecutive elements
+ * (default 2048; reduced automatically until the tile's LDS fits) and block threads per
+ * workgroup (256, 512 or 1024; 0 keeps the current value).  No reference counterpart: the
+ * tuning hook of the tile autotuner. */
+int dgh_mesh_set_tile(dgh_mesh m, uint32_t tile_elems, uint32_t block);
+/* Tiling statistics: tile size, tiles, element records (own + halo), node-table entries,
+ * LDS bytes per FP64 workgroup.  No reference counterpart. */
+int dgh_mesh_tile_info(dgh_mesh m, uint32_t* tile_elems, uint32_t* n_tiles, uint64_t* n_records,
+                       uint64_t* n_node_entries, uint32_t* lds_bytes_f64);
 int dgh_mesh_destroy(dgh_mesh m);
 
 /* y = A x.  Replaces Jacobian1MatMult (odaJac.C:290-374) and Jacobian2MatMult
