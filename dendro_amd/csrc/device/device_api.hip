@@ -125,7 +125,7 @@ namespace dgh {
 
 template <class T>
 static size_t op_lds_bytes(const dgh_mesh m) {
-  return sizeof(T) * (size_t(NSTENCIL) * KSTRIDE + m->ys_len + m->plan.max_touch);
+  return sizeof(double) * m->ys_len + sizeof(T) * (size_t(NSTENCIL) * KSTRIDE + m->plan.max_touch);
 }
 
 template <class T> const T* rk(dgh_mesh m);

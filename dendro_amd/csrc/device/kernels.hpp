@@ -68,11 +68,12 @@ __device__ __forceinline__ void load_rec(Rec& R, uint32_t g, const uint16_t* __r
 }
 
 // Evaluates one record with stencil triangles Ls/Ms (LDS or, when the stencil type is
-// uniform across the wavefront, global memory read through the scalar cache).
-template <class T, bool MASS, int MODE>
+// uniform across the wavefront, global memory read through the scalar cache); row sums of
+// owned nodes are added into ys (accumulator type TA).
+template <class T, class TA, bool MASS, int MODE>
 __device__ __forceinline__ void eval_rec_with(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac,
                                               const T* Ls, const T* Ms, const T* __restrict__ xs,
-                                              T* __restrict__ ys) {
+                                              TA* __restrict__ ys) {
   T fac1, fac2;
   elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
   if (MODE == OP_MATVEC) {
@@ -90,33 +91,37 @@ __device__ __forceinline__ void eval_rec_with(const Rec& R, uint32_t n_own, uint
           else
             acc += (fac1 * Ls[tri(k, j)]) * xv[j];
         }
-        atomic_add(&ys[R.l[k]], acc);
+        atomic_add(&ys[R.l[k]], TA(acc));
       }
     }
   } else {
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-      if (R.l[k] < n_own) atomic_add(&ys[R.l[k]], fac1 * Ls[tri(k, k)] + (MASS ? fac2 * Ms[tri(k, k)] : T(0)));
+      if (R.l[k] < n_own) atomic_add(&ys[R.l[k]], TA(fac1 * Ls[tri(k, k)] + (MASS ? fac2 * Ms[tri(k, k)] : T(0))));
   }
 }
 
-template <class T, bool MASS, int MODE>
+template <class T, class TA, bool MASS, int MODE>
 __device__ __forceinline__ void eval_rec(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac,
                                          const T* __restrict__ sK, const T* __restrict__ K0,
-                                         const T* __restrict__ xs, T* __restrict__ ys) {
+                                         const T* __restrict__ xs, TA* __restrict__ ys) {
   const uint32_t tp = R.mt & 31u;
   const uint32_t tp0 = __builtin_amdgcn_readfirstlane(tp);
   if (__all(tp == tp0)) {
     const T* Ls = K0 + tp0 * KSTRIDE;   // wave-uniform: scalar loads
-    eval_rec_with<T, MASS, MODE>(R, n_own, maxD, hFac, Ls, Ls + 36, xs, ys);
+    eval_rec_with<T, TA, MASS, MODE>(R, n_own, maxD, hFac, Ls, Ls + 36, xs, ys);
   } else {
     const T* Ls = sK + tp * KSTRIDE;
-    eval_rec_with<T, MASS, MODE>(R, n_own, maxD, hFac, Ls, Ls + 36, xs, ys);
+    eval_rec_with<T, TA, MASS, MODE>(R, n_own, maxD, hFac, Ls, Ls + 36, xs, ys);
   }
 }
 
+constexpr int RCHUNK = 2;   // records per thread whose loads are in flight together
+constexpr int XCHUNK = 4;   // x gathers per thread in flight together
+
 // K0: [18][72] child-0 Laplacian then mass upper triangles of precision T.
 // kappa/mass: per record (nullptr: uniform lap_fac / mass_fac).  MASS: mass term present.
+// Row sums accumulate in FP64 LDS for both precisions (TA = double).
 template <class T, class TC, bool MASS, int MODE, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
@@ -124,48 +129,64 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
                const TC* __restrict__ mass, double lap_fac, double mass_fac, uint32_t maxD,
                double hFac, const T* __restrict__ K0, const uint32_t* __restrict__ node_tab,
                uint32_t ys_len, const T* __restrict__ x, T* __restrict__ y) {
-  // LDS: stencils [18][72], owned-node sums [ys_len >= n_own], x of touched nodes
+  using TA = double;
+  // LDS: owned-node sums [ys_len >= n_own] (TA), stencils [18][72], x of touched nodes
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sK = reinterpret_cast<T*>(smem_raw);
-  T* ys = sK + NSTENCIL * KSTRIDE;
-  T* xs = ys + ys_len;
+  TA* ys = reinterpret_cast<TA*>(smem_raw);
+  T* sK = reinterpret_cast<T*>(ys + ys_len);
+  T* xs = sK + NSTENCIL * KSTRIDE;
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t >= ntiles) return;
   const TileRec rec = tiles[t];
   const uint32_t* nodes = node_tab + rec.node_off;
+  const uint32_t n_own = rec.n_own;
+  // first chunk of records: loads issued before the x gather so both chains overlap
+  Rec R[RCHUNK];
+#pragma unroll
+  for (int q = 0; q < RCHUNK; ++q) {
+    const uint32_t r = threadIdx.x + q * BLOCK;
+    if (r < rec.n_elem) load_rec<TC, MASS>(R[q], rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
+  }
   for (int i = threadIdx.x; i < NSTENCIL * KSTRIDE; i += BLOCK) sK[i] = K0[i];
   if (MODE == OP_MATVEC) {
-    // 4 independent gathers in flight per thread
-    uint32_t i = threadIdx.x;
-    for (; i + 3 * BLOCK < rec.n_touch; i += 4 * BLOCK) {
-      const uint32_t n0 = nodes[i], n1 = nodes[i + BLOCK], n2 = nodes[i + 2 * BLOCK], n3 = nodes[i + 3 * BLOCK];
-      const T v0 = x[n0], v1 = x[n1], v2 = x[n2], v3 = x[n3];
-      xs[i] = v0;
-      xs[i + BLOCK] = v1;
-      xs[i + 2 * BLOCK] = v2;
-      xs[i + 3 * BLOCK] = v3;
+    for (uint32_t base = 0; base < rec.n_touch; base += XCHUNK * BLOCK) {
+      uint32_t nid[XCHUNK];
+      T v[XCHUNK];
+#pragma unroll
+      for (int q = 0; q < XCHUNK; ++q) {
+        const uint32_t i = base + threadIdx.x + q * BLOCK;
+        nid[q] = (i < rec.n_touch) ? nodes[i] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < XCHUNK; ++q) {
+        const uint32_t i = base + threadIdx.x + q * BLOCK;
+        v[q] = (i < rec.n_touch) ? x[nid[q]] : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < XCHUNK; ++q) {
+        const uint32_t i = base + threadIdx.x + q * BLOCK;
+        if (i < rec.n_touch) xs[i] = v[q];
+      }
     }
-    for (; i < rec.n_touch; i += BLOCK) xs[i] = x[nodes[i]];
   }
-  for (uint32_t i = threadIdx.x; i < rec.n_own; i += BLOCK) ys[i] = T(0);
+  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) ys[i] = TA(0);
   __syncthreads();
-  const uint32_t n_own = rec.n_own;
-  // two records per thread in flight: both loads are issued before either is evaluated
-  uint32_t r = threadIdx.x;
-  for (; r + BLOCK < rec.n_elem; r += 2 * BLOCK) {
-    Rec A, B;
-    load_rec<TC, MASS>(A, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
-    load_rec<TC, MASS>(B, rec.elem_off + r + BLOCK, lid, meta, kappa, mass, lap_fac, mass_fac);
-    eval_rec<T, MASS, MODE>(A, n_own, maxD, hFac, sK, K0, xs, ys);
-    eval_rec<T, MASS, MODE>(B, n_own, maxD, hFac, sK, K0, xs, ys);
-  }
-  if (r < rec.n_elem) {
-    Rec A;
-    load_rec<TC, MASS>(A, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
-    eval_rec<T, MASS, MODE>(A, n_own, maxD, hFac, sK, K0, xs, ys);
+  for (uint32_t base = 0;;) {
+#pragma unroll
+    for (int q = 0; q < RCHUNK; ++q) {
+      const uint32_t r = base + threadIdx.x + q * BLOCK;
+      if (r < rec.n_elem) eval_rec<T, TA, MASS, MODE>(R[q], n_own, maxD, hFac, sK, K0, xs, ys);
+    }
+    base += RCHUNK * BLOCK;
+    if (base >= rec.n_elem) break;
+#pragma unroll
+    for (int q = 0; q < RCHUNK; ++q) {
+      const uint32_t r = base + threadIdx.x + q * BLOCK;
+      if (r < rec.n_elem) load_rec<TC, MASS>(R[q], rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
+    }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) y[nodes[i]] = ys[i];
+  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) y[nodes[i]] = T(ys[i]);
 }
 
 // f_out[j] = f_in[j] + sum_k W[s][pos][k] c[cnl[k]] for every fine node j (prolongation
