@@ -165,7 +165,8 @@ def main():
         b = torch.tensor(np.random.default_rng(22).uniform(-1, 1, N), dtype=torch.float64, device=dev)
         for name, prec in (("fp64", da.F64), ("mixed", da.F32)):
             p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
-                                  norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=prec)
+                                  norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=prec,
+                                  use_graph=1)
             s = da.Solver(meshes, trans, p)
             xs = torch.zeros(N, dtype=torch.float64, device=dev)
             s.solve(b, xs)                                   # warm-up solve

@@ -136,7 +136,7 @@ template <> const float* rk<float>(dgh_mesh m) { return m->rk32.p; }
 template <> const float* rm<float>(dgh_mesh m) { return m->rm32.p; }
 
 template <class T, int MODE, int BLOCK>
-static void launch_op_b(dgh_mesh m, const T* x, T* y, hipStream_t st) {
+static void launch_op_b(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
   const int nt = int(m->plan.tiles.size());
   const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
   const size_t lds = op_lds_bytes<T>(m);
@@ -145,32 +145,44 @@ static void launch_op_b(dgh_mesh m, const T* x, T* y, hipStream_t st) {
   if (kap && !mas)
     hipLaunchKernelGGL((tile_op_kernel<T, T, false, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->tiles.p,
                        nt, m->lid.p, m->meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
-                       m->node_tab.p, m->ys_len, x, y);
+                       m->node_tab.p, m->ys_len, io);
   else
     hipLaunchKernelGGL((tile_op_kernel<T, T, true, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->tiles.p,
                        nt, m->lid.p, m->meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
-                       m->node_tab.p, m->ys_len, x, y);
+                       m->node_tab.p, m->ys_len, io);
   HIP_CHECK(hipGetLastError());
 }
 
 template <class T, int MODE>
-static void launch_op(dgh_mesh m, const T* x, T* y, hipStream_t st) {
+static void launch_op(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
   if (m->plan.tiles.empty()) return;
   switch (m->block) {
-    case 256: launch_op_b<T, MODE, 256>(m, x, y, st); break;
-    case 512: launch_op_b<T, MODE, 512>(m, x, y, st); break;
-    default: launch_op_b<T, MODE, 1024>(m, x, y, st); break;
+    case 256: launch_op_b<T, MODE, 256>(m, io, st); break;
+    case 512: launch_op_b<T, MODE, 512>(m, io, st); break;
+    default: launch_op_b<T, MODE, 1024>(m, io, st); break;
   }
 }
 
 template <class T>
 static void launch_matvec(dgh_mesh m, const T* x, T* y, hipStream_t st) {
-  launch_op<T, OP_MATVEC>(m, x, y, st);
+  launch_op<T, OP_MATVEC>(m, OpIO<T>{x, y, nullptr, nullptr, T(0)}, st);
 }
 
 template <class T>
 static void launch_diag(dgh_mesh m, T* d, hipStream_t st) {
-  launch_op<T, OP_DIAG>(m, static_cast<const T*>(nullptr), d, st);
+  launch_op<T, OP_DIAG>(m, OpIO<T>{nullptr, d, nullptr, nullptr, T(0)}, st);
+}
+
+// r = b - A x
+template <class T>
+static void launch_resid(dgh_mesh m, const T* x, const T* b, T* r, hipStream_t st) {
+  launch_op<T, OP_RESID>(m, OpIO<T>{x, r, b, nullptr, T(0)}, st);
+}
+
+// x_out = x + omega invd (b - A x)
+template <class T>
+static void launch_smooth(dgh_mesh m, const T* x, const T* b, const T* invd, T omega, T* x_out, hipStream_t st) {
+  launch_op<T, OP_SMOOTH>(m, OpIO<T>{x, x_out, b, invd, omega}, st);
 }
 
 template <class T>
@@ -232,7 +244,7 @@ namespace dgh {
 
 template <class T>
 struct LevelVecs {
-  DevBuf<T> invd, b, x, r, y;
+  DevBuf<T> invd, b, xa, xb, r;
 };
 
 }  // namespace dgh
@@ -247,28 +259,38 @@ struct dgh_solver_s {
   int nc = 0;                     // coarsest size
   DevBuf<double> Ainv64;          // dense coarsest inverse
   DevBuf<float> Ainv32;
-  DevBuf<double> scal;            // device scalars (dot results)
-  // outer CG vectors (finest, FP64)
-  DevBuf<double> r, z, p, w;
+  DevBuf<double> S;               // CG scalars (CgScalar)
+  DevBuf<double> part;            // reduction partials [MAXQ][RED_BLOCKS]
+  DevBuf<double> part_p;          // partials of the null-space projections
+  double* hS = nullptr;           // pinned host mirror of S
+  DevBuf<double> x, r, z, p, w;   // outer CG vectors (finest, FP64)
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  ~dgh_solver_s() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (hS) (void)hipHostFree(hS);
+    if (st) (void)hipStreamDestroy(st);
+  }
 };
 
 namespace dgh {
 
-static void project_mean_dev(uint32_t n, double* v, double* scratch, hipStream_t st) {
-  HIP_CHECK(hipMemsetAsync(scratch, 0, sizeof(double) * 3, st));
-  hipLaunchKernelGGL(dot3_kernel<double>, dim3(std::min(1024u, blocks_for(n))), dim3(VBLOCK), 0, st, n,
-                     v, (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                     (const double*)nullptr, (const double*)nullptr, scratch);
-  hipLaunchKernelGGL(subtract_mean_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n,
-                     (const double*)scratch, v);
+template <class T>
+static void reduce(uint32_t n, const DotArgs<T>& d, int nq, double* part, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(reduce1_kernel<T>, dim3(RED_BLOCKS), dim3(VBLOCK), 0, st, n, d, part);
+  hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(RED_BLOCKS), 0, st, nq, (const double*)part, out);
 }
-static void project_mean_dev(uint32_t n, float* v, double* scratch, hipStream_t st) {
-  HIP_CHECK(hipMemsetAsync(scratch, 0, sizeof(double) * 3, st));
-  hipLaunchKernelGGL(dot3_kernel<float>, dim3(std::min(1024u, blocks_for(n))), dim3(VBLOCK), 0, st, n,
-                     v, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr,
-                     (const float*)nullptr, (const float*)nullptr, scratch);
-  hipLaunchKernelGGL(subtract_mean_kernel<float>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n,
-                     (const double*)scratch, v);
+
+// v -= mean(v)
+template <class T>
+static void project_mean(dgh_solver s, uint32_t n, T* v, hipStream_t st) {
+  DotArgs<T> d{};
+  d.a[0] = v;
+  reduce<T>(n, d, 1, s->part_p.p, s->S.p + S_PROJ, st);
+  hipLaunchKernelGGL(subtract_mean_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n,
+                     (const double*)(s->S.p + S_PROJ), v);
 }
 
 template <class T> std::vector<LevelVecs<T>>& vecs(dgh_solver s);
@@ -278,52 +300,46 @@ template <class T> const T* ainv(dgh_solver s);
 template <> const double* ainv<double>(dgh_solver s) { return s->Ainv64.p; }
 template <> const float* ainv<float>(dgh_solver s) { return s->Ainv32.p; }
 
-// Richardson/Jacobi sweeps on level l (see KSPSolve_Richardson restatement in the oracle).
+// PCMG multiplicative V-cycle on level l for right-hand side b (PCMGMCycle_Private with the
+// options of bin/options:182-206): pre-smoothing from a zero guess, residual, restriction,
+// recursion (coarsest: direct solve), interpolate-add, post-smoothing.  Returns the buffer
+// holding the result (one of the level's two iterate buffers).
 template <class T>
-static void smooth(dgh_solver s, int l, const T* b, T* x, int nu, bool guess_zero, hipStream_t st) {
-  auto& V = vecs<T>(s)[l];
-  dgh_mesh m = s->meshes[l];
-  const uint32_t n = m->N;
-  const T om = T(s->prm.omega);
-  for (int it = 0; it < nu; ++it) {
-    if (it == 0 && guess_zero) {
-      hipLaunchKernelGGL(smooth_first_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, om, b,
-                         (const T*)V.invd.p, x);
-    } else {
-      launch_matvec<T>(m, x, V.y.p, st);
-      hipLaunchKernelGGL(smooth_update_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, om, b,
-                         (const T*)V.y.p, (const T*)V.invd.p, x);
-    }
-  }
-}
-
-// PCMG multiplicative V-cycle on level l: x = B_l b (x overwritten).
-template <class T>
-static void vcycle(dgh_solver s, int l, const T* b, T* x, hipStream_t st) {
+static T* vcycle(dgh_solver s, int l, const T* b, hipStream_t st) {
   auto& VV = vecs<T>(s);
+  auto& V = VV[l];
   if (l == 0) {
     const int n = s->nc;
     const T* bb = b;
     if (s->prm.null_space) {
-      HIP_CHECK(hipMemcpyAsync(VV[0].r.p, b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
-      project_mean_dev(uint32_t(n), VV[0].r.p, s->scal.p + 4, st);
-      bb = VV[0].r.p;
+      hipLaunchKernelGGL(copy_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, uint32_t(n), b, V.r.p);
+      project_mean<T>(s, uint32_t(n), V.r.p, st);
+      bb = V.r.p;
     }
-    hipLaunchKernelGGL(dense_apply_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, ainv<T>(s), bb, x);
-    return;
+    hipLaunchKernelGGL(dense_apply_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, ainv<T>(s), bb, V.xa.p);
+    return V.xa.p;
   }
-  auto& V = VV[l];
-  auto& Vc = VV[l - 1];
   dgh_mesh m = s->meshes[l];
   const uint32_t n = m->N;
-  smooth<T>(s, l, b, x, s->prm.nu_pre, true, st);
-  launch_matvec<T>(m, x, V.y.p, st);
-  hipLaunchKernelGGL(residual_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, b,
-                     (const T*)V.y.p, V.r.p);
-  launch_restrict<T>(s->transfers[l - 1], V.r.p, Vc.b.p, st);
-  vcycle<T>(s, l - 1, Vc.b.p, Vc.x.p, st);
-  launch_prolong<T>(s->transfers[l - 1], Vc.x.p, x, x, st);
-  smooth<T>(s, l, b, x, s->prm.nu_post, false, st);
+  const T om = T(s->prm.omega);
+  T* xc = V.xa.p;
+  T* xn = V.xb.p;
+  // pre-smoothing: the first sweep from x = 0 needs no operator application
+  hipLaunchKernelGGL(smooth_first_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, om, b,
+                     (const T*)V.invd.p, xc);
+  for (int it = 1; it < s->prm.nu_pre; ++it) {
+    launch_smooth<T>(m, xc, b, V.invd.p, om, xn, st);
+    std::swap(xc, xn);
+  }
+  launch_resid<T>(m, xc, b, V.r.p, st);
+  launch_restrict<T>(s->transfers[l - 1], V.r.p, VV[l - 1].b.p, st);
+  T* xcoarse = vcycle<T>(s, l - 1, VV[l - 1].b.p, st);
+  launch_prolong<T>(s->transfers[l - 1], xcoarse, xc, xc, st);
+  for (int it = 0; it < s->prm.nu_post; ++it) {
+    launch_smooth<T>(m, xc, b, V.invd.p, om, xn, st);
+    std::swap(xc, xn);
+  }
+  return xc;
 }
 
 // z = B r at the finest level, FP64 in and out.
@@ -332,24 +348,41 @@ static void precondition(dgh_solver s, const double* r, double* z, hipStream_t s
   const uint32_t n = s->meshes[top]->N;
   if (s->prm.vcycle_prec == DGH_F32) {
     auto& V = s->v32[top];
-    // the finest level's b and x buffers of the FP32 cycle carry r and z
     hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, r, V.b.p);
-    vcycle<float>(s, top, V.b.p, V.x.p, st);
-    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n,
-                       (const float*)V.x.p, z);
+    const float* res = vcycle<float>(s, top, V.b.p, st);
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, res, z);
   } else {
-    vcycle<double>(s, top, r, z, st);
+    const double* res = vcycle<double>(s, top, r, st);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, res, z);
   }
-  if (s->prm.null_space) project_mean_dev(n, z, s->scal.p + 4, st);
+  if (s->prm.null_space) project_mean<double>(s, n, z, st);
 }
 
-static void dots(uint32_t n, const double* a0, const double* b0, const double* a1, const double* b1,
-                 double* dev_out, double* host_out, int cnt, hipStream_t st) {
-  HIP_CHECK(hipMemsetAsync(dev_out, 0, sizeof(double) * 3, st));
-  hipLaunchKernelGGL(dot3_kernel<double>, dim3(std::min(1024u, blocks_for(n))), dim3(VBLOCK), 0, st, n,
-                     a0, b0, a1, b1, (const double*)nullptr, (const double*)nullptr, dev_out);
-  HIP_CHECK(hipMemcpyAsync(host_out, dev_out, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
-  HIP_CHECK(hipStreamSynchronize(st));
+// One CG iteration (everything between two stopping tests), on device scalars only.
+static void cg_iteration(dgh_solver s, hipStream_t st) {
+  const int top = s->nlevels - 1;
+  dgh_mesh m = s->meshes[top];
+  const uint32_t n = m->N;
+  const bool flexible = (s->prm.vcycle_prec == DGH_F32);
+  const bool pnorm = (s->prm.norm_type == DGH_NORM_PRECONDITIONED);
+  double* S = s->S.p;
+  hipLaunchKernelGGL(cg_p_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
+                     (const double*)s->z.p, s->p.p);
+  launch_matvec<double>(m, s->p.p, s->w.p, st);
+  DotArgs<double> d{};
+  d.a[0] = s->p.p;
+  d.b[0] = s->w.p;
+  reduce<double>(n, d, 1, s->part.p, S + S_PW, st);
+  hipLaunchKernelGGL(cg_xr_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, S, (const double*)s->p.p,
+                     (const double*)s->w.p, s->x.p, s->r.p);
+  precondition(s, s->r.p, s->z.p, st);
+  DotArgs<double> e{};
+  e.a[0] = s->z.p; e.b[0] = s->z.p;    // ZZ
+  e.a[1] = s->z.p; e.b[1] = s->r.p;    // ZR
+  e.a[2] = flexible ? s->z.p : nullptr; e.b[2] = s->w.p;   // ZW
+  e.a[3] = pnorm ? nullptr : s->r.p; e.b[3] = s->r.p;      // RR
+  reduce<double>(n, e, 4, s->part.p, S + S_ZZ, st);
+  hipLaunchKernelGGL(cg_scalar_kernel, dim3(1), dim3(64), 0, st, S, int(pnorm), int(flexible));
 }
 
 // Assembles the coarsest operator from its element blocks and inverts it on the device
@@ -699,7 +732,11 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
       if (l > 0 && (s->transfers[l - 1]->coarse != meshes[l - 1] || s->transfers[l - 1]->fine != meshes[l]))
         fail(DGH_ERR_ARG, "transfer %d does not connect levels %d and %d", l - 1, l - 1, l);
     }
-    s->scal.alloc(8);
+    HIP_CHECK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s->hS), sizeof(double) * S_NSCAL, hipHostMallocDefault));
+    s->S.alloc(S_NSCAL);
+    s->part.alloc(MAXQ * RED_BLOCKS);
+    s->part_p.alloc(MAXQ * RED_BLOCKS);
     // per-level vectors and Jacobi diagonals
     s->v64.resize(nlevels);
     if (params->vcycle_prec == DGH_F32) s->v32.resize(nlevels);
@@ -708,26 +745,27 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
       auto& V = s->v64[l];
       V.invd.alloc(n);
       V.b.alloc(n);
-      V.x.alloc(n);
+      V.xa.alloc(n);
+      V.xb.alloc(n);
       V.r.alloc(n);
-      V.y.alloc(n);
-      launch_diag<double>(meshes[l], V.y.p, 0);
-      hipLaunchKernelGGL(reciprocal_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, 0, n,
-                         (const double*)V.y.p, V.invd.p);
+      launch_diag<double>(meshes[l], V.r.p, s->st);
+      hipLaunchKernelGGL(reciprocal_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, s->st, n,
+                         (const double*)V.r.p, V.invd.p);
       if (params->vcycle_prec == DGH_F32) {
         auto& F = s->v32[l];
         F.invd.alloc(n);
         F.b.alloc(n);
-        F.x.alloc(n);
+        F.xa.alloc(n);
+        F.xb.alloc(n);
         F.r.alloc(n);
-        F.y.alloc(n);
-        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n)), dim3(VBLOCK), 0, 0, n,
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n)), dim3(VBLOCK), 0, s->st, n,
                            (const double*)V.invd.p, F.invd.p);
       }
     }
-    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipStreamSynchronize(s->st));
     setup_coarse(s.get());
     const uint32_t nf = meshes[nlevels - 1]->N;
+    s->x.alloc(nf);
     s->r.alloc(nf);
     s->z.alloc(nf);
     s->p.alloc(nf);
@@ -749,73 +787,65 @@ int dgh_solver_vcycle(dgh_solver s, const double* r, double* z, void* stream) {
   });
 }
 
-// PETSc KSPSolve_CG restated (zero initial guess, left preconditioning); in the FP32-V-cycle
-// mode the preconditioner is inexact and beta takes the flexible (Polak-Ribiere) form
-// z_k.(r_k - r_{k-1}) / z_{k-1}.r_{k-1}.
+// PETSc KSPSolve_CG restated (zero initial guess, left preconditioning): see the oracle's
+// or_pcg.  In the FP32-V-cycle mode the preconditioner is inexact and the p-update uses the
+// flexible (Polak-Ribiere) factor z_k.(r_k - r_{k-1}) / z_{k-1}.r_{k-1}.  CG scalars live on
+// the device; the host reads the stopping-test norm once per iteration.
 int dgh_solver_solve(dgh_solver s, const double* b, double* x, dgh_solve_stats* stats, double* history,
                      int history_len, void* stream) {
   return guarded([&] {
     require_init();
     if (!s || !b || !x) fail(DGH_ERR_ARG, "NULL argument");
-    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipStream_t ust = static_cast<hipStream_t>(stream);
+    hipStream_t st = s->st;
     const int top = s->nlevels - 1;
     dgh_mesh m = s->meshes[top];
     const uint32_t n = m->N;
-    const bool flexible = (s->prm.vcycle_prec == DGH_F32);
-    hipEvent_t e0, e1;
+    const bool pnorm = (s->prm.norm_type == DGH_NORM_PRECONDITIONED);
+    double* S = s->S.p;
+    hipEvent_t e0, e1, eu;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventCreateWithFlags(&eu, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(eu, ust));          // b is ready on the caller's stream
+    HIP_CHECK(hipStreamWaitEvent(st, eu, 0));
     HIP_CHECK(hipEventRecord(e0, st));
-    double* sc = s->scal.p;
-    double hs[3];
-    // r = b (projected), x = 0
-    HIP_CHECK(hipMemcpyAsync(s->r.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    if (s->prm.null_space) project_mean_dev(n, s->r.p, sc + 4, st);
-    HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, st));
-    dots(n, s->r.p, s->r.p, nullptr, nullptr, sc, hs, 1, st);
-    const double bnorm = std::sqrt(hs[0]);
+    // r = b (projected), x = 0, p = 0
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, b, s->r.p);
+    if (s->prm.null_space) project_mean<double>(s, n, s->r.p, st);
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, s->x.p, 0.0, n);
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, s->p.p, 0.0, n);
     precondition(s, s->r.p, s->z.p, st);
-    dots(n, s->z.p, s->z.p, s->z.p, s->r.p, sc, hs, 2, st);
-    double dp = std::sqrt(s->prm.norm_type == DGH_NORM_PRECONDITIONED ? hs[0] : bnorm * bnorm);
-    double rz = hs[1], ratio = 0.0;
-    const double dp0 = dp;
-    if (history && history_len > 0) history[0] = dp;
+    DotArgs<double> d{};
+    d.a[0] = s->z.p; d.b[0] = s->z.p;   // ZZ
+    d.a[1] = s->z.p; d.b[1] = s->r.p;   // ZR
+    reduce<double>(n, d, 2, s->part.p, S + S_ZZ, st);
+    DotArgs<double> bb{};
+    bb.a[0] = s->r.p; bb.b[0] = s->r.p;
+    reduce<double>(n, bb, 1, s->part_p.p, S + S_BB, st);
+    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(64), 0, st, S, int(pnorm));
+    HIP_CHECK(hipMemcpyAsync(s->hS, S, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    const double dp0 = s->hS[S_DP];
+    const double bnorm = std::sqrt(s->hS[S_BB]);
+    double dp = dp0;
+    if (history && history_len > 0) history[0] = dp0;
+    if (s->prm.use_graph && !s->gexec) {
+      HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      cg_iteration(s, st);
+      HIP_CHECK(hipStreamEndCapture(st, &s->graph));
+      HIP_CHECK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+    }
     int it = 0;
     bool conv = (dp0 == 0.0), bad = false;
     while (!conv && it < s->prm.max_it) {
-      if (it == 0) {
-        HIP_CHECK(hipMemcpyAsync(s->p.p, s->z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-      } else {
-        hipLaunchKernelGGL(aypx_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, ratio,
-                           (const double*)s->z.p, s->p.p);
-      }
-      launch_matvec<double>(m, s->p.p, s->w.p, st);
-      dots(n, s->p.p, s->w.p, nullptr, nullptr, sc, hs, 1, st);
-      const double a = rz / hs[0];
-      hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)nullptr, a,
-                         1.0, (const double*)s->p.p, x);
-      hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)nullptr, -a,
-                         1.0, (const double*)s->w.p, s->r.p);
-      precondition(s, s->r.p, s->z.p, st);
-      // z.z, z.r and (flexible) z.w in one pass
-      HIP_CHECK(hipMemsetAsync(sc, 0, sizeof(double) * 3, st));
-      hipLaunchKernelGGL(dot3_kernel<double>, dim3(std::min(1024u, blocks_for(n))), dim3(VBLOCK), 0, st, n,
-                         (const double*)s->z.p, (const double*)s->z.p, (const double*)s->z.p,
-                         (const double*)s->r.p, flexible ? (const double*)s->z.p : (const double*)nullptr,
-                         (const double*)s->w.p, sc);
-      HIP_CHECK(hipMemcpyAsync(hs, sc, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+      if (s->gexec)
+        HIP_CHECK(hipGraphLaunch(s->gexec, st));
+      else
+        cg_iteration(s, st);
+      HIP_CHECK(hipMemcpyAsync(&s->hS[S_DP], S + S_DP, sizeof(double), hipMemcpyDeviceToHost, st));
       HIP_CHECK(hipStreamSynchronize(st));
-      double rr = 0.0;
-      if (s->prm.norm_type == DGH_NORM_UNPRECONDITIONED) {
-        double t[1];
-        dots(n, s->r.p, s->r.p, nullptr, nullptr, sc, t, 1, st);
-        rr = t[0];
-      }
-      // standard: beta = z_k.r_k / z_{k-1}.r_{k-1}; flexible: z_k.(r_k - r_{k-1}) / z_{k-1}.r_{k-1}
-      // with r_{k-1} = r_k + a w, i.e. numerator -a z_k.w
-      ratio = flexible ? (-a * hs[2]) / rz : hs[1] / rz;
-      rz = hs[1];
-      dp = std::sqrt(s->prm.norm_type == DGH_NORM_PRECONDITIONED ? hs[0] : rr);
+      dp = s->hS[S_DP];
       ++it;
       if (history && it < history_len) history[it] = dp;
       if (!std::isfinite(dp)) {
@@ -824,28 +854,32 @@ int dgh_solver_solve(dgh_solver s, const double* b, double* x, dgh_solve_stats* 
       }
       if (dp <= s->prm.rtol * dp0) conv = true;
     }
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)s->x.p, x);
     HIP_CHECK(hipEventRecord(e1, st));
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    HIP_CHECK(hipEventDestroy(e0));
-    HIP_CHECK(hipEventDestroy(e1));
     if (stats) {
       // true residual ||b - A x|| / ||b|| in FP64 (b projected when constants are the null space)
-      HIP_CHECK(hipMemcpyAsync(s->r.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-      if (s->prm.null_space) project_mean_dev(n, s->r.p, sc + 4, st);
-      launch_matvec<double>(m, x, s->w.p, st);
-      hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)nullptr, -1.0,
-                         1.0, (const double*)s->w.p, s->r.p);
-      double t[1];
-      dots(n, s->r.p, s->r.p, nullptr, nullptr, sc, t, 1, st);
+      hipLaunchKernelGGL(copy_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, b, s->w.p);
+      if (s->prm.null_space) project_mean<double>(s, n, s->w.p, st);
+      launch_resid<double>(m, x, s->w.p, s->z.p, st);
+      DotArgs<double> q{};
+      q.a[0] = s->z.p; q.b[0] = s->z.p;
+      reduce<double>(n, q, 1, s->part_p.p, S + S_PROJ, st);
+      HIP_CHECK(hipMemcpyAsync(s->hS, S, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
       stats->iterations = it;
       stats->converged = conv ? 1 : 0;
       stats->rnorm0 = dp0;
       stats->rnorm = dp;
-      stats->rel_true = bnorm > 0 ? std::sqrt(t[0]) / bnorm : 0.0;
+      stats->rel_true = bnorm > 0 ? std::sqrt(s->hS[S_PROJ]) / bnorm : 0.0;
       stats->seconds = ms * 1e-3;
     }
+    HIP_CHECK(hipStreamSynchronize(st));
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    HIP_CHECK(hipEventDestroy(eu));
     if (bad) fail(DGH_ERR_DIVERGED, "CG breakdown: non-finite residual at iteration %d", it);
   });
 }

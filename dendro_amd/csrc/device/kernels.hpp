@@ -40,7 +40,18 @@ __device__ __forceinline__ void elem_factors(uint32_t lev, uint32_t maxD, double
   fac2 = T(k2 * h * h * h / 8.0);
 }
 
-enum OpMode { OP_MATVEC = 0, OP_DIAG = 1 };
+// MATVEC: y = A x;  DIAG: y = diag(A);  RESID: y = b - A x;
+// SMOOTH: y = x + omega (invd (b - A x))  (one Richardson + Jacobi sweep, y must not alias x)
+enum OpMode { OP_MATVEC = 0, OP_DIAG = 1, OP_RESID = 2, OP_SMOOTH = 3 };
+
+template <class T>
+struct OpIO {
+  const T* x;      // operator input (MATVEC, RESID, SMOOTH)
+  T* y;            // output
+  const T* b;      // right-hand side (RESID, SMOOTH)
+  const T* invd;   // inverse diagonal (SMOOTH)
+  T omega;         // Richardson scale (SMOOTH)
+};
 
 // Symmetric 8x8 stencils are kept as their upper triangles (36 entries, row-major):
 // entry (k, j) with k <= j sits at tri(k, j) = k*8 - k*(k-1)/2 + (j - k).
@@ -76,7 +87,7 @@ __device__ __forceinline__ void eval_rec_with(const Rec& R, uint32_t n_own, uint
                                               TA* __restrict__ ys) {
   T fac1, fac2;
   elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
-  if (MODE == OP_MATVEC) {
+  if (MODE != OP_DIAG) {
     T xv[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) xv[j] = xs[R.l[j]];
@@ -128,8 +139,9 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
                const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
                const TC* __restrict__ mass, double lap_fac, double mass_fac, uint32_t maxD,
                double hFac, const T* __restrict__ K0, const uint32_t* __restrict__ node_tab,
-               uint32_t ys_len, const T* __restrict__ x, T* __restrict__ y) {
+               uint32_t ys_len, OpIO<T> io) {
   using TA = double;
+  const T* __restrict__ x = io.x;
   // LDS: owned-node sums [ys_len >= n_own] (TA), stencils [18][72], x of touched nodes
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TA* ys = reinterpret_cast<TA*>(smem_raw);
@@ -148,7 +160,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
     if (r < rec.n_elem) load_rec<TC, MASS>(R[q], rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
   }
   for (int i = threadIdx.x; i < NSTENCIL * KSTRIDE; i += BLOCK) sK[i] = K0[i];
-  if (MODE == OP_MATVEC) {
+  if (MODE != OP_DIAG) {
     for (uint32_t base = 0; base < rec.n_touch; base += XCHUNK * BLOCK) {
       uint32_t nid[XCHUNK];
       T v[XCHUNK];
@@ -186,7 +198,18 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) y[nodes[i]] = T(ys[i]);
+  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) {
+    const uint32_t n = nodes[i];
+    const T v = T(ys[i]);
+    if (MODE == OP_MATVEC || MODE == OP_DIAG) {
+      io.y[n] = v;
+    } else if (MODE == OP_RESID) {
+      io.y[n] = io.b[n] - v;
+    } else {
+      const T r = io.b[n] - v;
+      io.y[n] = x[n] + io.omega * (r * io.invd[n]);
+    }
+  }
 }
 
 // f_out[j] = f_in[j] + sum_k W[s][pos][k] c[cnl[k]] for every fine node j (prolongation

@@ -8,18 +8,6 @@ namespace dgh {
 
 constexpr int VBLOCK = 256;
 
-// Richardson + Jacobi sweep after y = A x:  r = b - y;  x = x + omega (r * invd)
-// (KSPSolve_Richardson with PCApply_Jacobi, bin/options:187-193).
-template <class T>
-__global__ void smooth_update_kernel(uint32_t n, T omega, const T* __restrict__ b,
-                                     const T* __restrict__ y, const T* __restrict__ invd,
-                                     T* __restrict__ x) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const T r = b[i] - y[i];
-  x[i] = x[i] + omega * (r * invd[i]);
-}
-
 // First sweep from a zero guess: r = b, x = omega (b * invd).
 template <class T>
 __global__ void smooth_first_kernel(uint32_t n, T omega, const T* __restrict__ b,
@@ -27,14 +15,6 @@ __global__ void smooth_first_kernel(uint32_t n, T omega, const T* __restrict__ b
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   x[i] = T(0) + omega * (b[i] * invd[i]);
-}
-
-// r = b - y
-template <class T>
-__global__ void residual_kernel(uint32_t n, const T* __restrict__ b, const T* __restrict__ y,
-                                T* __restrict__ r) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) r[i] = b[i] - y[i];
 }
 
 // invd = 1 / d
@@ -50,71 +30,119 @@ __global__ void convert_kernel(uint32_t n, const Ti* __restrict__ a, To* __restr
   if (i < n) b[i] = To(a[i]);
 }
 
-// y = y + a x  (a read from device memory when ap != nullptr, scaled by sgn)
-__global__ void axpy_kernel(uint32_t n, const double* __restrict__ ap, double a, double sgn,
-                            const double* __restrict__ x, double* __restrict__ y) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double al = ap ? sgn * ap[0] : a;
-  y[i] = y[i] + al * x[i];
-}
-
-// p = z + b p
-__global__ void aypx_kernel(uint32_t n, double b, const double* __restrict__ z,
-                            double* __restrict__ p) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = z[i] + b * p[i];
-}
-
-// v = v - s/n (constant-null-space projection; s = sum of v precomputed on the device)
-template <class T>
-__global__ void subtract_mean_kernel(uint32_t n, const double* __restrict__ sum, T* __restrict__ v) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = v[i] - T(sum[0] / double(n));
-}
-
-// Wavefront (64 lanes) reduction, then one atomic per workgroup.
+// Wavefront (64 lanes) reduction.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
 }
 
-// out[q] += sum_i a_q[i] b_q[i] for up to 3 pairs at once (b_q == nullptr: sum of a_q;
-// a_q == nullptr: pair unused).  out must be zeroed.
+// Deterministic two-stage reductions (no atomics, no zeroing, safe in graphs).
+// Stage 1: block b writes partial q of sum_i a_q[i] * b_q[i] (b_q == nullptr: sum of a_q;
+// a_q == nullptr: unused) to part[q * RED_BLOCKS + b].  Stage 2 sums the partials of each
+// quantity in a fixed order into out[q].
+constexpr int RED_BLOCKS = 256;
+constexpr int MAXQ = 4;
+
+template <class T>
+struct DotArgs {
+  const T* a[MAXQ];
+  const T* b[MAXQ];
+};
+
 template <class T>
 __global__ void __launch_bounds__(VBLOCK)
-dot3_kernel(uint32_t n, const T* __restrict__ a0, const T* __restrict__ b0,
-            const T* __restrict__ a1, const T* __restrict__ b1, const T* __restrict__ a2,
-            const T* __restrict__ b2, double* __restrict__ out) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (a0) s0 += double(a0[i]) * (b0 ? double(b0[i]) : 1.0);
-    if (a1) s1 += double(a1[i]) * (b1 ? double(b1[i]) : 1.0);
-    if (a2) s2 += double(a2[i]) * (b2 ? double(b2[i]) : 1.0);
+reduce1_kernel(uint32_t n, DotArgs<T> d, double* __restrict__ part) {
+  double s[MAXQ] = {0.0, 0.0, 0.0, 0.0};
+  for (uint32_t i = blockIdx.x * VBLOCK + threadIdx.x; i < n; i += RED_BLOCKS * VBLOCK) {
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q)
+      if (d.a[q]) s[q] += double(d.a[q][i]) * (d.b[q] ? double(d.b[q][i]) : 1.0);
   }
-  __shared__ double red[3][VBLOCK / 64];
-  s0 = wave_sum(s0);
-  s1 = wave_sum(s1);
-  s2 = wave_sum(s2);
+  __shared__ double red[MAXQ][VBLOCK / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-    red[0][w] = s0;
-    red[1][w] = s1;
-    red[2][w] = s2;
+#pragma unroll
+  for (int q = 0; q < MAXQ; ++q) {
+    const double v = wave_sum(s[q]);
+    if (lane == 0) red[q][w] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t0 = 0, t1 = 0, t2 = 0;
-    for (int q = 0; q < VBLOCK / 64; ++q) {
-      t0 += red[0][q];
-      t1 += red[1][q];
-      t2 += red[2][q];
-    }
-    if (a0) atomicAdd(&out[0], t0);
-    if (a1) atomicAdd(&out[1], t1);
-    if (a2) atomicAdd(&out[2], t2);
+  if (threadIdx.x < MAXQ) {
+    double t = 0.0;
+    for (int k = 0; k < VBLOCK / 64; ++k) t += red[threadIdx.x][k];
+    part[threadIdx.x * RED_BLOCKS + blockIdx.x] = t;
   }
+}
+
+__global__ void __launch_bounds__(RED_BLOCKS)
+reduce2_kernel(int nq, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[RED_BLOCKS / 64];
+  for (int q = 0; q < nq; ++q) {
+    const double v = wave_sum(part[q * RED_BLOCKS + threadIdx.x]);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int k = 0; k < RED_BLOCKS / 64; ++k) t += red[k];
+      out[q] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// v = v - sum[0] / n  (constant-null-space projection)
+template <class T>
+__global__ void subtract_mean_kernel(uint32_t n, const double* __restrict__ sum, T* __restrict__ v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = v[i] - T(sum[0] / double(n));
+}
+
+// ---------------------------------------------------------------- CG with device scalars
+// S: RZ = z.r, PW = p.w, DP = stopping-test norm, RATIO = p-update factor, ZZ, ZR, ZW, RR,
+// BB = b.b, ALPHA.
+enum CgScalar { S_RZ = 0, S_PW, S_DP, S_RATIO, S_ZZ, S_ZR, S_ZW, S_RR, S_BB, S_ALPHA, S_PROJ, S_NSCAL = 16 };
+
+// p = z + ratio p  (VecAYPX)
+__global__ void cg_p_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ z,
+                            double* __restrict__ p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + S[S_RATIO] * p[i];
+}
+
+// a = rz / pw;  x = x + a p;  r = r + (-a) w  (VecAXPY twice)
+__global__ void cg_xr_kernel(uint32_t n, double* __restrict__ S, const double* __restrict__ p,
+                             const double* __restrict__ w, double* __restrict__ x, double* __restrict__ r) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double a = S[S_RZ] / S[S_PW];
+  if (i == 0) S[S_ALPHA] = a;
+  if (i < n) {
+    x[i] = x[i] + a * p[i];
+    r[i] = r[i] + (-a) * w[i];
+  }
+}
+
+// after z = B r and the dots: dp, the next p-update factor, rz
+__global__ void cg_scalar_kernel(double* __restrict__ S, int precond_norm, int flexible) {
+  if (threadIdx.x != 0) return;
+  const double dp = sqrt(precond_norm ? S[S_ZZ] : S[S_RR]);
+  const double ratio = flexible ? (-S[S_ALPHA] * S[S_ZW]) / S[S_RZ] : S[S_ZR] / S[S_RZ];
+  S[S_DP] = dp;
+  S[S_RATIO] = ratio;
+  S[S_RZ] = S[S_ZR];
+}
+
+// before the first iteration: rz, dp0, ratio = 0 (p starts at zero)
+__global__ void cg_init_kernel(double* __restrict__ S, int precond_norm) {
+  if (threadIdx.x != 0) return;
+  S[S_RZ] = S[S_ZR];
+  S[S_DP] = sqrt(precond_norm ? S[S_ZZ] : S[S_BB]);
+  S[S_RATIO] = 0.0;
+}
+
+template <class T>
+__global__ void copy_kernel(uint32_t n, const T* __restrict__ a, T* __restrict__ b) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 // x = Ainv b for a dense symmetric n x n inverse (row-major), one wavefront per row.

@@ -166,3 +166,29 @@ def test_pcg_converges_fp64_and_mixed(dev, hier_small):
     err = rel_err(out[da.F32], out[da.F64])
     print(f"mixed-precision solution error vs FP64 solve: {err:.2e}")
     assert err <= 1e-6
+
+
+def test_pcg_graph_replays_match_eager(dev, hier_small):
+    """The hipGraph-captured CG iteration gives the eager iterates on every replay."""
+    h = hier_small
+    coefs = var_coefficients(h, mass_base=1.0)
+    ms, ts = _meshes(h, coefs)
+    b = np.random.default_rng(31).uniform(-1, 1, h.fine.n_node)
+    bt = _t(b, dev)
+    res = {}
+    for g in (0, 1):
+        p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=50, rtol=1e-10,
+                              norm_type=da.NORM_UNPRECONDITIONED, use_graph=g)
+        s = da.Solver(ms, ts, p)
+        outs = []
+        for rep in range(3 if g else 1):
+            x = torch.zeros(h.fine.n_node, dtype=torch.float64, device=dev)
+            st, hist = s.solve(bt, x, history_len=51)
+            _ = (bt * 2.0).sum().item()          # eager work between replays
+            outs.append((_np(x), hist.copy(), st.iterations))
+        res[g] = outs
+    x0, h0, it0 = res[0][0]
+    for x1, h1, it1 in res[1]:
+        assert it1 == it0
+        assert rel_err(x1, x0) <= 1e-12
+        assert np.allclose(h1, h0, rtol=1e-9, atol=0)
