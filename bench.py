@@ -98,6 +98,11 @@ def main():
     mf = meshes[-1]
     if args.tile:
         mf.set_tile(args.tile)
+    else:
+        t0 = time.time()
+        for m in meshes:
+            m.autotune(da.F64)
+        log(f"autotuned tilings in {time.time() - t0:.1f}s: finest {mf.tile_info()}")
     log(f"tiling: {mf.tile_info()}")
 
     # ---- MatVec: K steps, inputs resident in HBM, events on the launch stream
@@ -139,7 +144,7 @@ def main():
             torch.cuda.synchronize()
             ti = mf.tile_info()
             log(f"tile {te} block {blk}: {ev0.elapsed_time(ev1) / 50 * 1e3:.1f} us  {ti}")
-        mf.set_tile(args.tile or 2048, 512)
+        mf.autotune(da.F64)
 
     if args.calib_copy:
         a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev)

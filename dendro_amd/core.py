@@ -28,7 +28,7 @@ EXPORTS = [
     "dgh_damg_free",
     "dgh_init", "dgh_finalize",
     "dgh_mesh_create", "dgh_mesh_set_coefficients", "dgh_mesh_info", "dgh_mesh_destroy",
-    "dgh_mesh_set_tile", "dgh_mesh_tile_info",
+    "dgh_mesh_set_tile", "dgh_mesh_tile_info", "dgh_mesh_autotune",
     "dgh_matvec", "dgh_diagonal", "dgh_matvec_multi",
     "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add",
     "dgh_solver_default_params", "dgh_solver_create", "dgh_solver_destroy", "dgh_solver_solve",
@@ -291,6 +291,11 @@ class Mesh:
                                        C.byref(lds)))
         return {"tile_elems": te.value, "n_tiles": nt.value, "n_records": nr.value,
                 "n_node_entries": nn.value, "lds_bytes_f64": lds.value}
+
+    def autotune(self, prec=F64):
+        te, blk, us = C.c_uint32(), C.c_uint32(), C.c_float()
+        check(lib().dgh_mesh_autotune(self.h, int(prec), C.byref(te), C.byref(blk), C.byref(us)))
+        return {"tile_elems": te.value, "block": blk.value, "us": us.value}
 
     def matvec(self, x, y, prec=F64, stream=None):
         check(lib().dgh_matvec(self.h, int(prec), ptr(x), ptr(y), _stream(stream)))

@@ -125,7 +125,8 @@ namespace dgh {
 
 template <class T>
 static size_t op_lds_bytes(const dgh_mesh m) {
-  return sizeof(double) * m->ys_len + sizeof(T) * (size_t(NSTENCIL) * KSTRIDE + m->plan.max_touch);
+  return sizeof(double) * m->ys_len +
+         sizeof(T) * ((LDS_STENCIL ? size_t(NSTENCIL) * KSTRIDE : 0) + m->plan.max_touch);
 }
 
 template <class T> const T* rk(dgh_mesh m);
@@ -189,8 +190,8 @@ template <class T>
 static void launch_restrict(dgh_transfer t, const T* f, T* c, hipStream_t st) {
   const uint32_t nc = t->coarse->N;
   if (!nc) return;
-  hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(nc)), dim3(VBLOCK), 0, st, nc, t->rowptr.p, t->col.p,
-                     t->w.p, f, c);
+  hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(size_t(nc) * RLANES)), dim3(VBLOCK), 0, st, nc,
+                     t->rowptr.p, t->col.p, t->w.p, f, c);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -280,7 +281,7 @@ namespace dgh {
 template <class T>
 static void reduce(uint32_t n, const DotArgs<T>& d, int nq, double* part, double* out, hipStream_t st) {
   hipLaunchKernelGGL(reduce1_kernel<T>, dim3(RED_BLOCKS), dim3(VBLOCK), 0, st, n, d, part);
-  hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(RED_BLOCKS), 0, st, nq, (const double*)part, out);
+  hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(RED2_THREADS), 0, st, nq, (const double*)part, out);
 }
 
 // v -= mean(v)
@@ -530,6 +531,59 @@ int dgh_mesh_tile_info(dgh_mesh m, uint32_t* tile_elems, uint32_t* n_tiles, uint
     if (n_records) *n_records = m->plan.rec_elem.size();
     if (n_node_entries) *n_node_entries = m->plan.node_tab.size();
     if (lds_bytes_f64) *lds_bytes_f64 = uint32_t(op_lds_bytes<double>(m));
+  });
+}
+
+int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* block, float* us) {
+  return guarded([&] {
+    require_init();
+    if (!m) fail(DGH_ERR_ARG, "NULL mesh");
+    if (prec != DGH_F64 && prec != DGH_F32) fail(DGH_ERR_ARG, "unknown precision %d", prec);
+    const size_t bytes = (prec == DGH_F64 ? 8 : 4) * size_t(m->N);
+    DevBuf<double> xb((bytes + 7) / 8), yb((bytes + 7) / 8);
+    HIP_CHECK(hipMemset(xb.p, 0, bytes));
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    float best = 1e30f;
+    uint32_t bt = 0, bb = 0;
+    for (uint32_t T : {1024u, 2048u, 4096u}) {
+      for (int B : {256, 512, 1024}) {
+        m->block = B;
+        try {
+          plan_mesh(m, T);
+        } catch (const Error&) {
+          continue;
+        }
+        if (m->plan.tile_elems != T) continue;   // reduced to fit LDS: tried already
+        auto run = [&] {
+          if (prec == DGH_F64)
+            launch_matvec<double>(m, xb.p, yb.p, nullptr);
+          else
+            launch_matvec<float>(m, reinterpret_cast<const float*>(xb.p), reinterpret_cast<float*>(yb.p), nullptr);
+        };
+        for (int i = 0; i < 3; ++i) run();
+        HIP_CHECK(hipEventRecord(e0, nullptr));
+        for (int i = 0; i < 10; ++i) run();
+        HIP_CHECK(hipEventRecord(e1, nullptr));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          bt = T;
+          bb = uint32_t(B);
+        }
+      }
+    }
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    if (!bt) fail(DGH_ERR_STATE, "no tiling candidate fits");
+    m->block = int(bb);
+    plan_mesh(m, bt);
+    if (tile_elems) *tile_elems = bt;
+    if (block) *block = bb;
+    if (us) *us = best * 100.f;   // ms per 10 launches -> us per launch
   });
 }
 

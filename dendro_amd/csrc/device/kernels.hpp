@@ -112,21 +112,39 @@ __device__ __forceinline__ void eval_rec_with(const Rec& R, uint32_t n_own, uint
   }
 }
 
+// Every stencil read goes through the scalar cache: the wavefront walks over the distinct
+// stencil types among its active lanes (one in class-pure wavefronts, two at the class
+// boundaries of a tile) and evaluates the lanes of each type with a wave-uniform pointer.
 template <class T, class TA, bool MASS, int MODE>
 __device__ __forceinline__ void eval_rec(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac,
                                          const T* __restrict__ sK, const T* __restrict__ K0,
                                          const T* __restrict__ xs, TA* __restrict__ ys) {
+  (void)sK;
   const uint32_t tp = R.mt & 31u;
-  const uint32_t tp0 = __builtin_amdgcn_readfirstlane(tp);
-  if (__all(tp == tp0)) {
-    const T* Ls = K0 + tp0 * KSTRIDE;   // wave-uniform: scalar loads
-    eval_rec_with<T, TA, MASS, MODE>(R, n_own, maxD, hFac, Ls, Ls + 36, xs, ys);
-  } else {
-    const T* Ls = sK + tp * KSTRIDE;
-    eval_rec_with<T, TA, MASS, MODE>(R, n_own, maxD, hFac, Ls, Ls + 36, xs, ys);
+  bool done = false;
+  for (;;) {
+    const uint64_t pend = __ballot(!done);
+    if (pend == 0) break;
+    // stencil type of the first pending lane: wave-uniform, so the stencil is read with
+    // scalar loads
+    const uint32_t tu = __builtin_amdgcn_readlane(tp, __builtin_ctzll(pend));
+    const T* Ls = K0 + tu * KSTRIDE;
+    T Lr[36], Mr[36];
+#pragma unroll
+    for (int q = 0; q < 36; ++q) {
+      Lr[q] = Ls[q];
+      if (MASS) Mr[q] = Ls[36 + q];
+    }
+    if (!done && tp == tu) {
+      eval_rec_with<T, TA, MASS, MODE>(R, n_own, maxD, hFac, Lr, Mr, xs, ys);
+      done = true;
+    }
   }
 }
 
+// Keep a copy of the 18 stencil triangles in LDS for wavefronts with mixed stencil types
+// (false: those few wavefronts read the global table through L1).
+constexpr bool LDS_STENCIL = false;
 constexpr int RCHUNK = 2;   // records per thread whose loads are in flight together
 constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 
@@ -145,8 +163,8 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   // LDS: owned-node sums [ys_len >= n_own] (TA), stencils [18][72], x of touched nodes
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TA* ys = reinterpret_cast<TA*>(smem_raw);
-  T* sK = reinterpret_cast<T*>(ys + ys_len);
-  T* xs = sK + NSTENCIL * KSTRIDE;
+  T* sK = LDS_STENCIL ? reinterpret_cast<T*>(ys + ys_len) : nullptr;
+  T* xs = LDS_STENCIL ? sK + NSTENCIL * KSTRIDE : reinterpret_cast<T*>(ys + ys_len);
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t >= ntiles) return;
   const TileRec rec = tiles[t];
@@ -159,7 +177,8 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
     const uint32_t r = threadIdx.x + q * BLOCK;
     if (r < rec.n_elem) load_rec<TC, MASS>(R[q], rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
   }
-  for (int i = threadIdx.x; i < NSTENCIL * KSTRIDE; i += BLOCK) sK[i] = K0[i];
+  if (LDS_STENCIL)
+    for (int i = threadIdx.x; i < NSTENCIL * KSTRIDE; i += BLOCK) sK[i] = K0[i];
   if (MODE != OP_DIAG) {
     for (uint32_t base = 0; base < rec.n_touch; base += XCHUNK * BLOCK) {
       uint32_t nid[XCHUNK];
@@ -236,17 +255,24 @@ prolong_kernel(uint32_t nf, const uint32_t* __restrict__ celem, const uint8_t* _
   f_out[j] = (f_in ? f_in[j] : T(0)) + acc;
 }
 
-// c[i] = sum_p w_p f[col_p] over row i of R = P^T (CSR built on the host; gather, no atomics).
+// c[i] = sum_p w_p f[col_p] over row i of R = P^T (CSR built on the host; gather, no
+// atomics).  RLANES lanes share a row (rows hold about 4-30 entries) and combine their
+// partial sums with shuffles; the sum order is fixed.
+constexpr int RLANES = 8;
 template <class T>
 __global__ void __launch_bounds__(256)
 restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
                 const float* __restrict__ w, const T* __restrict__ f, T* __restrict__ c) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nc) return;
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = gid / RLANES, lane = gid % RLANES;
   T acc = T(0);
-  const uint32_t p1 = rowptr[i + 1];
-  for (uint32_t p = rowptr[i]; p < p1; ++p) acc += T(w[p]) * f[col[p]];
-  c[i] = acc;
+  if (i < nc) {
+    const uint32_t p1 = rowptr[i + 1];
+    for (uint32_t p = rowptr[i] + lane; p < p1; p += RLANES) acc += T(w[p]) * f[col[p]];
+  }
+#pragma unroll
+  for (int off = RLANES / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, RLANES);
+  if (i < nc && lane == 0) c[i] = acc;
 }
 
 template <class T>

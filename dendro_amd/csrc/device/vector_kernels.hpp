@@ -41,7 +41,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Stage 1: block b writes partial q of sum_i a_q[i] * b_q[i] (b_q == nullptr: sum of a_q;
 // a_q == nullptr: unused) to part[q * RED_BLOCKS + b].  Stage 2 sums the partials of each
 // quantity in a fixed order into out[q].
-constexpr int RED_BLOCKS = 256;
+constexpr int RED_BLOCKS = 1024;
 constexpr int MAXQ = 4;
 
 template <class T>
@@ -74,16 +74,19 @@ reduce1_kernel(uint32_t n, DotArgs<T> d, double* __restrict__ part) {
   }
 }
 
-__global__ void __launch_bounds__(RED_BLOCKS)
+constexpr int RED2_THREADS = 256;
+__global__ void __launch_bounds__(RED2_THREADS)
 reduce2_kernel(int nq, const double* __restrict__ part, double* __restrict__ out) {
-  __shared__ double red[RED_BLOCKS / 64];
+  __shared__ double red[RED2_THREADS / 64];
   for (int q = 0; q < nq; ++q) {
-    const double v = wave_sum(part[q * RED_BLOCKS + threadIdx.x]);
+    double v = 0.0;
+    for (int k = threadIdx.x; k < RED_BLOCKS; k += RED2_THREADS) v += part[q * RED_BLOCKS + k];
+    v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
-      for (int k = 0; k < RED_BLOCKS / 64; ++k) t += red[k];
+      for (int k = 0; k < RED2_THREADS / 64; ++k) t += red[k];
       out[q] = t;
     }
     __syncthreads();

@@ -177,6 +177,11 @@ int dgh_mesh_set_tile(dgh_mesh m, uint32_t tile_elems, uint32_t block);
  * LDS bytes per FP64 workgroup.  No reference counterpart. */
 int dgh_mesh_tile_info(dgh_mesh m, uint32_t* tile_elems, uint32_t* n_tiles, uint64_t* n_records,
                        uint64_t* n_node_entries, uint32_t* lds_bytes_f64);
+/* Autotuner: times the operator kernel of precision prec on this mesh for every candidate
+ * (tile size, block) pair and keeps the fastest.  Synchronous; uses scratch vectors it
+ * allocates itself.  Outputs (any may be NULL): the chosen pair and its time per launch.
+ * No reference counterpart. */
+int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* block, float* us);
 int dgh_mesh_destroy(dgh_mesh m);
 
 /* y = A x.  Replaces Jacobian1MatMult (odaJac.C:290-374) and Jacobian2MatMult
