@@ -61,6 +61,67 @@ def cpu_baseline(F, kappa, budget_s=15.0):
                       f"(E={F.n_elem}, N={F.n_node}), {dt:.1f}s"}
 
 
+def bench_multi(args, meshes, trans, N, dev, stream, extra):
+    """32 right-hand sides (seeds 400..431): the batched FP32 operator and the batched
+    PCG (one FP32 V-cycle per iteration for all columns, FP64 outer CG, one hipGraph)."""
+    import torch
+    import dendro_amd as da
+    k = 32
+    tuned = [m.autotune_multi(k) for m in meshes[1:]]
+    mf = meshes[-1]
+    log(f"batched tiles (autotuned, k={k}): {[t['tile_elems'] for t in tuned]}")
+    X = torch.tensor(np.random.default_rng(11).uniform(-1, 1, (N, k)), dtype=torch.float32, device=dev)
+    Y = torch.empty_like(X)
+    for _ in range(5):
+        mf.matvec_multi(X, Y, k, prec=da.F32, stream=stream)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(10, args.steps // 10)
+    ev0.record(stream)
+    for _ in range(reps):
+        mf.matvec_multi(X, Y, k, prec=da.F32, stream=stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    del X, Y
+    B = torch.tensor(np.stack([np.random.default_rng(400 + j).uniform(-1, 1, N) for j in range(k)], 1),
+                     dtype=torch.float64, device=dev)
+    Xs = torch.zeros_like(B)
+    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
+                          norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=da.F32, use_graph=1)
+    s = da.Solver(meshes, trans, p)
+    s.solve_multi(B, Xs, k)                    # warm-up (buffers, graph)
+    st = s.solve_multi(B, Xs, k)
+    secs = st[0].seconds
+    single = extra.get("pcg_mixed", {}).get("seconds")
+    out = {"k": k, "matvec_f32_us": round(ms * 1e3, 2), "matvec_f32_gdofs": round(N * k / (ms * 1e-3) / 1e9, 3),
+           "tile_elems": [t["tile_elems"] for t in tuned], "solve_seconds": round(secs, 5),
+           "seconds_per_rhs": round(secs / k, 6), "iterations": [x.iterations for x in st],
+           "max_rel_true": max(x.rel_true for x in st), "all_converged": all(x.converged for x in st)}
+    if single:
+        out["speedup_per_rhs_vs_single_mixed"] = round(single / (secs / k), 2)
+    log(f"batched k={k}: MatVec FP32 {ms*1e3:.1f} us ({out['matvec_f32_gdofs']} GDoF/s), "
+        f"PCG {secs*1e3:.1f} ms for {k} RHS, its {min(out['iterations'])}-{max(out['iterations'])}, "
+        f"max rel_true {out['max_rel_true']:.2e}")
+    del s
+    return out
+
+
+def measured_traffic(tile_elems, block):
+    """HBM bytes per FP64 MatVec launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes over this workload (already scaled by the copy-kernel calibration), when they were
+    taken with the same tiling; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "tile_op_matvec_f64_lognormal1M_d12_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("tile_elems") != tile_elems or t.get("block") != block:
+        return None
+    return t.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,8 +132,10 @@ def main():
     ap.add_argument("--nlevels", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
-    ap.add_argument("--tile", type=int, default=0, help="operator tile size (0: library default)")
+    ap.add_argument("--tile", type=int, default=0, help="operator tile size (0: autotune)")
+    ap.add_argument("--block", type=int, default=0, help="workgroup size with --tile (0: library default)")
     ap.add_argument("--sweep-tiles", action="store_true", help="time the MatVec for several tile sizes")
+    ap.add_argument("--no-multi", action="store_true", help="skip the 32 right-hand-side batch")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -96,13 +159,14 @@ def main():
         meshes.append(m)
     trans = [da.Transfer(meshes[l - 1], meshes[l], *trans_arr[l - 1]) for l in range(1, len(levels))]
     mf = meshes[-1]
+    tuned = {"tile_elems": args.tile, "block": args.block or 512}
     if args.tile:
-        mf.set_tile(args.tile)
+        mf.set_tile(args.tile, args.block)
     else:
         t0 = time.time()
         for m in meshes:
-            m.autotune(da.F64)
-        log(f"autotuned tilings in {time.time() - t0:.1f}s: finest {mf.tile_info()}")
+            tuned = m.autotune(da.F64)
+        log(f"autotuned tilings in {time.time() - t0:.1f}s: finest {tuned} {mf.tile_info()}")
     log(f"tiling: {mf.tile_info()}")
 
     # ---- MatVec: K steps, inputs resident in HBM, events on the launch stream
@@ -182,6 +246,10 @@ def main():
             del s
         extra["pcg_speedup_mixed_vs_fp64"] = round(extra["pcg_fp64"]["seconds"] / extra["pcg_mixed"]["seconds"], 3)
 
+    if not args.no_multi:
+        extra["multi_rhs"] = bench_multi(args, meshes, trans, N, dev, stream, extra)
+
+    traffic = measured_traffic(mf.tile_info()["tile_elems"], tuned["block"])
     out = {
         "metric": METRIC,
         "value": round(gdofs, 4),
@@ -199,8 +267,10 @@ def main():
                                "max depth 12, kappa=10^U(-1,1), FP64",
                    "n_elem": E, "n_node": N, "n_levels": len(levels), "n_hanging_elem": F.n_hanging},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": alg_bytes, "kernel": "matvec_kernel<double>"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "kernel": f"dgh::tile_op_kernel<double, double, false, 0, {tuned['block']}>",
+                     "tile_elems": mf.tile_info()["tile_elems"]},
         "matvec_fp32_ms": round(ms32, 5),
         "matvec_fp32_gdofs": round(N / (ms32 * 1e-3) / 1e9, 4),
     }

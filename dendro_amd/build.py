@@ -8,6 +8,7 @@ objects are linked into dendro_amd/lib/libdendrohip.so.  Always recompiles.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -18,10 +19,11 @@ LIB = os.path.join(LIBDIR, "libdendrohip.so")
 
 HOST_SOURCES = ["host/stencils.cpp", "host/octree.cpp", "host/da.cpp", "host/tiles.cpp",
                 "host/host_api.cpp"]
-HIP_SOURCES = ["device/device_api.hip"]
+HIP_SOURCES = ["device/device_api.hip", "device/multi_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-# packed-FP32 instructions are disabled for every HIP translation unit
+# packed-FP32 instructions are disabled for every HIP translation unit: some of their forms
+# have returned wrong results on this hardware without any error
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
@@ -37,15 +39,16 @@ def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
-    objs = []
+    jobs = []
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-        _run(["g++"] + CXX_FLAGS + inc + ["-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append((["g++"] + CXX_FLAGS + inc + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-        _run([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append(([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        list(ex.map(lambda j: _run(j[0]), jobs))
+    objs = [o for _, o in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs +
          ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-pthread", "-Wl,-rpath,/opt/rocm/lib"])

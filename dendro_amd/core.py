@@ -29,10 +29,11 @@ EXPORTS = [
     "dgh_init", "dgh_finalize",
     "dgh_mesh_create", "dgh_mesh_set_coefficients", "dgh_mesh_info", "dgh_mesh_destroy",
     "dgh_mesh_set_tile", "dgh_mesh_tile_info", "dgh_mesh_autotune",
-    "dgh_matvec", "dgh_diagonal", "dgh_matvec_multi",
+    "dgh_matvec", "dgh_diagonal", "dgh_matvec_multi", "dgh_mesh_set_tile_multi",
+    "dgh_mesh_autotune_multi",
     "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add",
     "dgh_solver_default_params", "dgh_solver_create", "dgh_solver_destroy", "dgh_solver_solve",
-    "dgh_solver_vcycle",
+    "dgh_solver_solve_multi", "dgh_solver_vcycle",
 ]
 
 
@@ -300,6 +301,18 @@ class Mesh:
     def matvec(self, x, y, prec=F64, stream=None):
         check(lib().dgh_matvec(self.h, int(prec), ptr(x), ptr(y), _stream(stream)))
 
+    def matvec_multi(self, x, y, k, prec=F64, stream=None):
+        """Y = A X for k columns stored node-major ([N][k])."""
+        check(lib().dgh_matvec_multi(self.h, int(prec), int(k), ptr(x), ptr(y), _stream(stream)))
+
+    def set_tile_multi(self, tile_elems):
+        check(lib().dgh_mesh_set_tile_multi(self.h, int(tile_elems)))
+
+    def autotune_multi(self, k):
+        te, us = C.c_uint32(), C.c_float()
+        check(lib().dgh_mesh_autotune_multi(self.h, int(k), C.byref(te), C.byref(us)))
+        return {"tile_elems": te.value, "us": us.value}
+
     def diagonal(self, d, prec=F64, stream=None):
         check(lib().dgh_diagonal(self.h, int(prec), ptr(d), _stream(stream)))
 
@@ -355,6 +368,12 @@ class Solver:
                                      ptr(hist) if history_len else None, int(history_len),
                                      _stream(stream)))
         return st, hist[: min(history_len, st.iterations + 1)]
+
+    def solve_multi(self, B, X, k, stream=None):
+        """k independent solves; B, X FP64 [N][k] node-major.  Returns k SolveStats."""
+        st = (SolveStats * int(k))()
+        check(lib().dgh_solver_solve_multi(self.h, int(k), ptr(B), ptr(X), st, _stream(stream)))
+        return list(st)
 
     def vcycle(self, r, z, stream=None):
         check(lib().dgh_solver_vcycle(self.h, ptr(r), ptr(z), _stream(stream)))

@@ -11,152 +11,48 @@
 #include <memory>
 #include <vector>
 
-#include "../host/octree.hpp"
-#include "kernels.hpp"
-#include "vector_kernels.hpp"
+#include "internal.hpp"
 
 namespace dgh {
-
-#define HIP_CHECK(expr)                                                                  \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess) fail(DGH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-// ---------------------------------------------------------------- device buffers
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  explicit DevBuf(size_t count) { alloc(count); }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-  DevBuf& operator=(DevBuf&& o) noexcept {
-    release();
-    p = o.p;
-    n = o.n;
-    o.p = nullptr;
-    o.n = 0;
-    return *this;
-  }
-  ~DevBuf() { release(); }
-  void alloc(size_t count) {
-    release();
-    if (count) {
-      if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
-        p = nullptr;
-        fail(DGH_ERR_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
-      }
-    }
-    n = count;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  void upload(const T* h, size_t count) {
-    alloc(count);
-    if (count) HIP_CHECK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
-  }
-};
-
-static inline unsigned blocks_for(size_t n, int bs = VBLOCK) { return unsigned((n + bs - 1) / bs); }
-
-// ---------------------------------------------------------------- library state
-struct State {
-  bool ready = false;
-  int device = 0;
-  int max_lds = 65536;            // dynamic LDS bytes a workgroup may use
-  DevBuf<double> K64, W64;        // child-0 stencil triangles [18][72]; transfer weights [144][27][8]
-  DevBuf<float> K32, W32;
-  rocblas_handle blas = nullptr;
-};
-static State g;
-
-static void require_init() {
-  if (!g.ready) fail(DGH_ERR_STATE, "dgh_init has not been called");
-}
-
-template <class T> const T* Ktab();
-template <class T> const T* Wtab();
-template <> const double* Ktab<double>() { return g.K64.p; }
-template <> const double* Wtab<double>() { return g.W64.p; }
-template <> const float* Ktab<float>() { return g.K32.p; }
-template <> const float* Wtab<float>() { return g.W32.p; }
-
+State g;
 }  // namespace dgh
-
-using namespace dgh;
-
-// ---------------------------------------------------------------- mesh
-struct dgh_mesh_s {
-  uint32_t E = 0, N = 0, maxD = 0;
-  double hFac = 1.0;
-  // element arrays in the caller's (Morton) order: transfers and coarse assembly
-  std::vector<uint32_t> h_nl;
-  std::vector<uint8_t> h_child, h_hn, h_lev, h_cs;
-  std::vector<double> h_kappa, h_mass;
-  DevBuf<uint32_t> nl;          // [E][8]
-  DevBuf<uint8_t> cs;           // [E] stencil index 18c+t
-  double lap_fac = 1.0, mass_fac = 1.0;
-  // owner-computes tiling (tiles.cpp)
-  TilePlan plan;
-  uint32_t ys_len = 0;
-  int block = 512;              // threads per operator workgroup (256, 512 or 1024)
-  DevBuf<TileRec> tiles;
-  DevBuf<uint16_t> lid, meta;
-  DevBuf<uint32_t> node_tab;
-  DevBuf<double> rk64, rm64;    // per-record coefficients
-  DevBuf<float> rk32, rm32;
-};
-
-struct dgh_transfer_s {
-  dgh_mesh coarse = nullptr, fine = nullptr;
-  DevBuf<uint32_t> celem;       // [N_f] coarse element holding each fine node
-  DevBuf<uint8_t> pos;          // [N_f] position code inside it
-  DevBuf<uint32_t> rowptr, col; // R = P^T in CSR (rows: coarse nodes)
-  DevBuf<float> w;              // exact dyadic weights
-};
 
 namespace dgh {
 
 template <class T>
 static size_t op_lds_bytes(const dgh_mesh m) {
-  return sizeof(double) * m->ys_len +
-         sizeof(T) * ((LDS_STENCIL ? size_t(NSTENCIL) * KSTRIDE : 0) + m->plan.max_touch);
+  return sizeof(double) * m->op.ys_len +
+         sizeof(T) * ((LDS_STENCIL ? size_t(NSTENCIL) * KSTRIDE : 0) + m->op.plan.max_touch);
 }
 
 template <class T> const T* rk(dgh_mesh m);
 template <class T> const T* rm(dgh_mesh m);
-template <> const double* rk<double>(dgh_mesh m) { return m->rk64.p; }
-template <> const double* rm<double>(dgh_mesh m) { return m->rm64.p; }
-template <> const float* rk<float>(dgh_mesh m) { return m->rk32.p; }
-template <> const float* rm<float>(dgh_mesh m) { return m->rm32.p; }
+template <> const double* rk<double>(dgh_mesh m) { return m->op.rk64.p; }
+template <> const double* rm<double>(dgh_mesh m) { return m->op.rm64.p; }
+template <> const float* rk<float>(dgh_mesh m) { return m->op.rk32.p; }
+template <> const float* rm<float>(dgh_mesh m) { return m->op.rm32.p; }
 
 template <class T, int MODE, int BLOCK>
 static void launch_op_b(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
-  const int nt = int(m->plan.tiles.size());
+  const int nt = int(m->op.plan.tiles.size());
   const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
   const size_t lds = op_lds_bytes<T>(m);
   const T* kap = rk<T>(m);
   const T* mas = rm<T>(m);
   if (kap && !mas)
-    hipLaunchKernelGGL((tile_op_kernel<T, T, false, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->tiles.p,
-                       nt, m->lid.p, m->meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
-                       m->node_tab.p, m->ys_len, io);
+    hipLaunchKernelGGL((tile_op_kernel<T, T, false, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->op.tiles.p,
+                       nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
+                       m->op.node_tab.p, m->op.ys_len, io);
   else
-    hipLaunchKernelGGL((tile_op_kernel<T, T, true, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->tiles.p,
-                       nt, m->lid.p, m->meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
-                       m->node_tab.p, m->ys_len, io);
+    hipLaunchKernelGGL((tile_op_kernel<T, T, true, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->op.tiles.p,
+                       nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
+                       m->op.node_tab.p, m->op.ys_len, io);
   HIP_CHECK(hipGetLastError());
 }
 
 template <class T, int MODE>
 static void launch_op(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
-  if (m->plan.tiles.empty()) return;
+  if (m->op.plan.tiles.empty()) return;
   switch (m->block) {
     case 256: launch_op_b<T, MODE, 256>(m, io, st); break;
     case 512: launch_op_b<T, MODE, 512>(m, io, st); break;
@@ -204,78 +100,53 @@ static void launch_prolong(dgh_transfer t, const T* c, const T* f_in, T* f_out, 
   HIP_CHECK(hipGetLastError());
 }
 
-// (Re)builds the tiling of a mesh with the largest tile size <= T whose LDS fits.
-static void plan_mesh(dgh_mesh m, uint32_t T) {
-  for (;;) {
-    build_tiles(m->E, m->N, m->h_nl.data(), m->h_child.data(), m->h_hn.data(), m->h_lev.data(), T, m->plan);
-    m->ys_len = (m->plan.max_own + 1) & ~1u;
-    if (op_lds_bytes<double>(m) <= size_t(g.max_lds) || T <= 64) break;
-    T /= 2;
-  }
-  if (op_lds_bytes<double>(m) > size_t(g.max_lds))
-    fail(DGH_ERR_ARG, "operator tiles need %zu bytes of LDS (limit %d)", op_lds_bytes<double>(m), g.max_lds);
-  m->tiles.upload(m->plan.tiles.data(), m->plan.tiles.size());
-  m->lid.upload(m->plan.lid.data(), m->plan.lid.size());
-  m->meta.upload(m->plan.meta.data(), m->plan.meta.size());
-  m->node_tab.upload(m->plan.node_tab.data(), m->plan.node_tab.size());
-  // per-record coefficient copies follow the new record order
-  m->rk64.release();
-  m->rm64.release();
-  m->rk32.release();
-  m->rm32.release();
-  const size_t R = m->plan.rec_elem.size();
-  if (!m->h_kappa.empty()) {
-    std::vector<double> k(R);
-    std::vector<float> kf(R);
-    for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = m->h_kappa[m->plan.rec_elem[r]]);
-    m->rk64.upload(k.data(), R);
-    m->rk32.upload(kf.data(), R);
-    if (!m->h_mass.empty()) {
-      for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = m->h_mass[m->plan.rec_elem[r]]);
-      m->rm64.upload(k.data(), R);
-      m->rm32.upload(kf.data(), R);
-    }
+// Per-record coefficient copies of a tiling (they follow its record order).
+void upload_coefs(dgh_mesh m, DevTiles& d) {
+  d.rk64.release();
+  d.rm64.release();
+  d.rk32.release();
+  d.rm32.release();
+  const size_t R = d.plan.rec_elem.size();
+  if (m->h_kappa.empty() || R == 0) return;
+  std::vector<double> k(R);
+  std::vector<float> kf(R);
+  for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = m->h_kappa[d.plan.rec_elem[r]]);
+  d.rk64.upload(k.data(), R);
+  d.rk32.upload(kf.data(), R);
+  if (!m->h_mass.empty()) {
+    for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = m->h_mass[d.plan.rec_elem[r]]);
+    d.rm64.upload(k.data(), R);
+    d.rm32.upload(kf.data(), R);
   }
 }
+
+// (Re)builds a tiling with the largest tile size <= T whose LDS need (lds(d)) fits.
+void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&)) {
+  for (;;) {
+    build_tiles(m->E, m->N, m->h_nl.data(), m->h_child.data(), m->h_hn.data(), m->h_lev.data(), T, d.plan);
+    d.ys_len = (d.plan.max_own + 1) & ~1u;
+    if (lds(d) <= size_t(g.max_lds) || T <= 32) break;
+    T /= 2;
+  }
+  if (lds(d) > size_t(g.max_lds))
+    fail(DGH_ERR_ARG, "operator tiles need %zu bytes of LDS (limit %d)", lds(d), g.max_lds);
+  d.tiles.upload(d.plan.tiles.data(), d.plan.tiles.size());
+  d.lid.upload(d.plan.lid.data(), d.plan.lid.size());
+  d.meta.upload(d.plan.meta.data(), d.plan.meta.size());
+  d.node_tab.upload(d.plan.node_tab.data(), d.plan.node_tab.size());
+  upload_coefs(m, d);
+}
+
+static size_t op_lds_f64(const DevTiles& d) {
+  return sizeof(double) * (d.ys_len + (LDS_STENCIL ? size_t(NSTENCIL) * KSTRIDE : 0) + d.plan.max_touch);
+}
+
+// (Re)builds the single right-hand-side tiling with the largest tile size <= T that fits.
+static void plan_mesh(dgh_mesh m, uint32_t T) { build_devtiles(m, m->op, T, op_lds_f64); }
 
 }  // namespace dgh
 
 // ---------------------------------------------------------------- solver
-namespace dgh {
-
-template <class T>
-struct LevelVecs {
-  DevBuf<T> invd, b, xa, xb, r;
-};
-
-}  // namespace dgh
-
-struct dgh_solver_s {
-  int nlevels = 0;
-  std::vector<dgh_mesh> meshes;
-  std::vector<dgh_transfer> transfers;
-  dgh_solver_params prm{};
-  std::vector<LevelVecs<double>> v64;
-  std::vector<LevelVecs<float>> v32;
-  int nc = 0;                     // coarsest size
-  DevBuf<double> Ainv64;          // dense coarsest inverse
-  DevBuf<float> Ainv32;
-  DevBuf<double> S;               // CG scalars (CgScalar)
-  DevBuf<double> part;            // reduction partials [MAXQ][RED_BLOCKS]
-  DevBuf<double> part_p;          // partials of the null-space projections
-  double* hS = nullptr;           // pinned host mirror of S
-  DevBuf<double> x, r, z, p, w;   // outer CG vectors (finest, FP64)
-  hipStream_t st = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  ~dgh_solver_s() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (hS) (void)hipHostFree(hS);
-    if (st) (void)hipStreamDestroy(st);
-  }
-};
-
 namespace dgh {
 
 template <class T>
@@ -317,7 +188,8 @@ static T* vcycle(dgh_solver s, int l, const T* b, hipStream_t st) {
       project_mean<T>(s, uint32_t(n), V.r.p, st);
       bb = V.r.p;
     }
-    hipLaunchKernelGGL(dense_apply_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, ainv<T>(s), bb, V.xa.p);
+    if (reinterpret_cast<uintptr_t>(bb) % 16) fail(DGH_ERR_ARG, "coarse right-hand side is not 16-byte aligned");
+    hipLaunchKernelGGL(dense_apply_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, s->ldc, ainv<T>(s), bb, V.xa.p);
     return V.xa.p;
   }
   dgh_mesh m = s->meshes[l];
@@ -395,7 +267,9 @@ static void setup_coarse(dgh_solver s) {
   if (n > 40000) fail(DGH_ERR_ARG, "coarsest level has %d nodes; use more levels (dense solve limit 40000)", n);
   std::vector<double> L(9216), M(9216);
   generate_stencils(L.data(), M.data(), nullptr, nullptr);
-  std::vector<double> A(size_t(n) * n, 0.0);
+  const int ld = (n + DENSE_PAD - 1) / DENSE_PAD * DENSE_PAD;
+  s->ldc = ld;
+  std::vector<double> A(size_t(n) * ld, 0.0);
   const double hFac = m->hFac;
   for (uint32_t e = 0; e < m->E; ++e) {
     const double h = hFac * double(1u << (m->maxD - m->h_lev[e]));
@@ -406,34 +280,35 @@ static void setup_coarse(dgh_solver s) {
     const double* Ms = M.data() + size_t(m->h_cs[e]) * 64;
     const uint32_t* idx = m->h_nl.data() + size_t(e) * 8;
     for (int k = 0; k < 8; ++k)
-      for (int j = 0; j < 8; ++j) A[size_t(idx[k]) * n + idx[j]] += fac1 * Ls[k * 8 + j] + fac2 * Ms[k * 8 + j];
+      for (int j = 0; j < 8; ++j) A[size_t(idx[k]) * ld + idx[j]] += fac1 * Ls[k * 8 + j] + fac2 * Ms[k * 8 + j];
   }
   if (s->prm.null_space) {
     double tr = 0.0;
-    for (int i = 0; i < n; ++i) tr += A[size_t(i) * n + i];
+    for (int i = 0; i < n; ++i) tr += A[size_t(i) * ld + i];
     const double a = tr / (double(n) * n);
-    for (double& v : A) v += a;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) A[size_t(i) * ld + j] += a;
   }
   s->Ainv64.upload(A.data(), A.size());
   if (!g.blas) {
     if (rocblas_create_handle(&g.blas) != rocblas_status_success) fail(DGH_ERR_HIP, "rocblas_create_handle failed");
   }
   DevBuf<rocblas_int> info(1);
-  if (rocsolver_dpotrf(g.blas, rocblas_fill_lower, n, s->Ainv64.p, n, info.p) != rocblas_status_success)
+  if (rocsolver_dpotrf(g.blas, rocblas_fill_lower, n, s->Ainv64.p, ld, info.p) != rocblas_status_success)
     fail(DGH_ERR_HIP, "rocsolver_dpotrf failed");
   rocblas_int hinfo = 0;
   HIP_CHECK(hipMemcpy(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost));
   if (hinfo != 0) fail(DGH_ERR_DIVERGED, "coarsest operator is not positive definite (pivot %d)", int(hinfo));
-  if (rocsolver_dpotri(g.blas, rocblas_fill_lower, n, s->Ainv64.p, n, info.p) != rocblas_status_success)
+  if (rocsolver_dpotri(g.blas, rocblas_fill_lower, n, s->Ainv64.p, ld, info.p) != rocblas_status_success)
     fail(DGH_ERR_HIP, "rocsolver_dpotri failed");
   HIP_CHECK(hipMemcpy(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost));
   if (hinfo != 0) fail(DGH_ERR_DIVERGED, "coarsest inverse failed (%d)", int(hinfo));
-  hipLaunchKernelGGL(symmetrize_kernel<double>, dim3((n + 255) / 256, n), dim3(256), 0, 0, n, s->Ainv64.p);
+  hipLaunchKernelGGL(symmetrize_kernel<double>, dim3((n + 255) / 256, n), dim3(256), 0, 0, n, ld, s->Ainv64.p);
   HIP_CHECK(hipGetLastError());
   if (s->prm.vcycle_prec == DGH_F32) {
-    s->Ainv32.alloc(size_t(n) * n);
-    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(size_t(n) * n)), dim3(VBLOCK), 0, 0,
-                       uint32_t(size_t(n) * n), (const double*)s->Ainv64.p, s->Ainv32.p);
+    s->Ainv32.alloc(size_t(n) * ld);
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(size_t(n) * ld)), dim3(VBLOCK), 0, 0,
+                       uint32_t(size_t(n) * ld), (const double*)s->Ainv64.p, s->Ainv32.p);
   }
   HIP_CHECK(hipDeviceSynchronize());
 }
@@ -526,10 +401,10 @@ int dgh_mesh_tile_info(dgh_mesh m, uint32_t* tile_elems, uint32_t* n_tiles, uint
                        uint64_t* n_node_entries, uint32_t* lds_bytes_f64) {
   return guarded([&] {
     if (!m) fail(DGH_ERR_ARG, "NULL mesh");
-    if (tile_elems) *tile_elems = m->plan.tile_elems;
-    if (n_tiles) *n_tiles = uint32_t(m->plan.tiles.size());
-    if (n_records) *n_records = m->plan.rec_elem.size();
-    if (n_node_entries) *n_node_entries = m->plan.node_tab.size();
+    if (tile_elems) *tile_elems = m->op.plan.tile_elems;
+    if (n_tiles) *n_tiles = uint32_t(m->op.plan.tiles.size());
+    if (n_records) *n_records = m->op.plan.rec_elem.size();
+    if (n_node_entries) *n_node_entries = m->op.plan.node_tab.size();
     if (lds_bytes_f64) *lds_bytes_f64 = uint32_t(op_lds_bytes<double>(m));
   });
 }
@@ -555,7 +430,7 @@ int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* bloc
         } catch (const Error&) {
           continue;
         }
-        if (m->plan.tile_elems != T) continue;   // reduced to fit LDS: tried already
+        if (m->op.plan.tile_elems != T) continue;   // reduced to fit LDS: tried already
         auto run = [&] {
           if (prec == DGH_F64)
             launch_matvec<double>(m, xb.p, yb.p, nullptr);
@@ -599,23 +474,8 @@ int dgh_mesh_set_coefficients(dgh_mesh m, const double* kappa, const double* mas
     m->h_mass.clear();
     if (kappa) m->h_kappa.assign(kappa, kappa + m->E);
     if (mass) m->h_mass.assign(mass, mass + m->E);
-    const size_t R = m->plan.rec_elem.size();
-    m->rk64.release();
-    m->rm64.release();
-    m->rk32.release();
-    m->rm32.release();
-    if (kappa) {
-      std::vector<double> k(R);
-      std::vector<float> kf(R);
-      for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = kappa[m->plan.rec_elem[r]]);
-      m->rk64.upload(k.data(), R);
-      m->rk32.upload(kf.data(), R);
-      if (mass) {
-        for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = mass[m->plan.rec_elem[r]]);
-        m->rm64.upload(k.data(), R);
-        m->rm32.upload(kf.data(), R);
-      }
-    }
+    upload_coefs(m, m->op);
+    if (!m->mop.plan.tiles.empty()) upload_coefs(m, m->mop);
   });
 }
 
@@ -658,13 +518,6 @@ int dgh_diagonal(dgh_mesh m, int prec, void* d, void* stream) {
       launch_diag<float>(m, static_cast<float*>(d), st);
     else
       fail(DGH_ERR_ARG, "unknown precision %d", prec);
-  });
-}
-
-int dgh_matvec_multi(dgh_mesh m, int prec, int k, const void* x, void* y, void* stream) {
-  return guarded([&] {
-    (void)m; (void)prec; (void)k; (void)x; (void)y; (void)stream;
-    fail(DGH_ERR_STATE, "dgh_matvec_multi is not available in this build");
   });
 }
 

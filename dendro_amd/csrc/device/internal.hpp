@@ -1,0 +1,182 @@
+// Internal state shared by the device translation units of libdendrohip: device buffers,
+// library state, and the mesh / transfer / solver objects behind the opaque C handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <vector>
+
+#include "../host/octree.hpp"
+#include "kernels.hpp"
+#include "vector_kernels.hpp"
+
+namespace dgh {
+
+#define HIP_CHECK(expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) fail(DGH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------- device buffers
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    release();
+    p = o.p;
+    n = o.n;
+    o.p = nullptr;
+    o.n = 0;
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count) {
+      if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+        p = nullptr;
+        fail(DGH_ERR_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+      }
+    }
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T* h, size_t count) {
+    alloc(count);
+    if (count) HIP_CHECK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+};
+
+static inline unsigned blocks_for(size_t n, int bs = VBLOCK) { return unsigned((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------- library state
+struct State {
+  bool ready = false;
+  int device = 0;
+  int max_lds = 65536;            // dynamic LDS bytes a workgroup may use
+  DevBuf<double> K64, W64;        // child-0 stencil triangles [18][72]; transfer weights [144][27][8]
+  DevBuf<float> K32, W32;
+  rocblas_handle blas = nullptr;
+};
+extern State g;
+
+inline void require_init() {
+  if (!g.ready) fail(DGH_ERR_STATE, "dgh_init has not been called");
+}
+
+template <class T> const T* Ktab();
+template <class T> const T* Wtab();
+template <> inline const double* Ktab<double>() { return g.K64.p; }
+template <> inline const double* Wtab<double>() { return g.W64.p; }
+template <> inline const float* Ktab<float>() { return g.K32.p; }
+template <> inline const float* Wtab<float>() { return g.W32.p; }
+
+}  // namespace dgh
+
+namespace dgh {
+
+// One owner-computes tiling of a mesh (tiles.cpp) and its device copies.
+struct DevTiles {
+  TilePlan plan;
+  uint32_t ys_len = 0;          // owned-node slots per tile (>= max_own, even)
+  DevBuf<TileRec> tiles;
+  DevBuf<uint16_t> lid, meta;
+  DevBuf<uint32_t> node_tab;
+  DevBuf<double> rk64, rm64;    // per-record coefficients
+  DevBuf<float> rk32, rm32;
+};
+
+}  // namespace dgh
+
+using namespace dgh;
+
+struct dgh_mesh_s {
+  uint32_t E = 0, N = 0, maxD = 0;
+  double hFac = 1.0;
+  // element arrays in the caller's (Morton) order: transfers and coarse assembly
+  std::vector<uint32_t> h_nl;
+  std::vector<uint8_t> h_child, h_hn, h_lev, h_cs;
+  std::vector<double> h_kappa, h_mass;
+  DevBuf<uint32_t> nl;          // [E][8]
+  DevBuf<uint8_t> cs;           // [E] stencil index 18c+t
+  double lap_fac = 1.0, mass_fac = 1.0;
+  // owner-computes tilings: single right-hand side, and batched right-hand sides (smaller
+  // tiles: the batched kernel stages several values per node in LDS)
+  DevTiles op;
+  int block = 512;              // threads per operator workgroup (256, 512 or 1024)
+  DevTiles mop;
+  uint32_t mtile = 0;           // batched tile size (0: default)
+};
+
+struct dgh_transfer_s {
+  dgh_mesh coarse = nullptr, fine = nullptr;
+  DevBuf<uint32_t> celem;       // [N_f] coarse element holding each fine node
+  DevBuf<uint8_t> pos;          // [N_f] position code inside it
+  DevBuf<uint32_t> rowptr, col; // R = P^T in CSR (rows: coarse nodes)
+  DevBuf<float> w;              // exact dyadic weights
+};
+
+namespace dgh {
+
+template <class T>
+struct LevelVecs {
+  DevBuf<T> invd, b, xa, xb, r;
+};
+
+}  // namespace dgh
+
+struct dgh_solver_s {
+  int nlevels = 0;
+  std::vector<dgh_mesh> meshes;
+  std::vector<dgh_transfer> transfers;
+  dgh_solver_params prm{};
+  std::vector<LevelVecs<double>> v64;
+  std::vector<LevelVecs<float>> v32;
+  int nc = 0;                     // coarsest size
+  int ldc = 0;                    // row pitch of the dense inverse
+  DevBuf<double> Ainv64;          // dense coarsest inverse
+  DevBuf<float> Ainv32;
+  DevBuf<double> S;               // CG scalars (CgScalar)
+  DevBuf<double> part;            // reduction partials [MAXQ][RED_BLOCKS]
+  DevBuf<double> part_p;          // partials of the null-space projections
+  double* hS = nullptr;           // pinned host mirror of S
+  DevBuf<double> x, r, z, p, w;   // outer CG vectors (finest, FP64)
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  // batched right-hand sides (sized for mk columns on first use)
+  int mk = 0;
+  std::vector<LevelVecs<float>> m32;   // per level [N_l][mk]: b, xa, xb, r
+  DevBuf<double> mx, mr, mz, mp, mw;   // outer CG vectors [N][mk]
+  DevBuf<double> mS;                   // per-column CG scalars
+  DevBuf<double> mpart, mpart_p;       // column reduction partials
+  double* hmS = nullptr;
+  hipGraph_t mgraph = nullptr;
+  hipGraphExec_t mgexec = nullptr;
+  ~dgh_solver_s() {
+    if (mgexec) (void)hipGraphExecDestroy(mgexec);
+    if (mgraph) (void)hipGraphDestroy(mgraph);
+    if (hmS) (void)hipHostFree(hmS);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (hS) (void)hipHostFree(hS);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace dgh {
+void upload_coefs(dgh_mesh m, DevTiles& d);
+void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&));
+}  // namespace dgh

@@ -151,7 +151,7 @@ constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 // K0: [18][72] child-0 Laplacian then mass upper triangles of precision T.
 // kappa/mass: per record (nullptr: uniform lap_fac / mass_fac).  MASS: mass term present.
 // Row sums accumulate in FP64 LDS for both precisions (TA = double).
-template <class T, class TC, bool MASS, int MODE, int BLOCK>
+template <class T, class TC, bool MASS, int MODE, int BLOCK, int RCH = RCHUNK, int XCH = XCHUNK>
 __global__ void __launch_bounds__(BLOCK)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
                const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
@@ -171,30 +171,30 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   const uint32_t* nodes = node_tab + rec.node_off;
   const uint32_t n_own = rec.n_own;
   // first chunk of records: loads issued before the x gather so both chains overlap
-  Rec R[RCHUNK];
+  Rec R[RCH];
 #pragma unroll
-  for (int q = 0; q < RCHUNK; ++q) {
+  for (int q = 0; q < RCH; ++q) {
     const uint32_t r = threadIdx.x + q * BLOCK;
     if (r < rec.n_elem) load_rec<TC, MASS>(R[q], rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
   }
   if (LDS_STENCIL)
     for (int i = threadIdx.x; i < NSTENCIL * KSTRIDE; i += BLOCK) sK[i] = K0[i];
   if (MODE != OP_DIAG) {
-    for (uint32_t base = 0; base < rec.n_touch; base += XCHUNK * BLOCK) {
-      uint32_t nid[XCHUNK];
-      T v[XCHUNK];
+    for (uint32_t base = 0; base < rec.n_touch; base += XCH * BLOCK) {
+      uint32_t nid[XCH];
+      T v[XCH];
 #pragma unroll
-      for (int q = 0; q < XCHUNK; ++q) {
+      for (int q = 0; q < XCH; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         nid[q] = (i < rec.n_touch) ? nodes[i] : 0u;
       }
 #pragma unroll
-      for (int q = 0; q < XCHUNK; ++q) {
+      for (int q = 0; q < XCH; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         v[q] = (i < rec.n_touch) ? x[nid[q]] : T(0);
       }
 #pragma unroll
-      for (int q = 0; q < XCHUNK; ++q) {
+      for (int q = 0; q < XCH; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         if (i < rec.n_touch) xs[i] = v[q];
       }
@@ -204,14 +204,14 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   __syncthreads();
   for (uint32_t base = 0;;) {
 #pragma unroll
-    for (int q = 0; q < RCHUNK; ++q) {
+    for (int q = 0; q < RCH; ++q) {
       const uint32_t r = base + threadIdx.x + q * BLOCK;
       if (r < rec.n_elem) eval_rec<T, TA, MASS, MODE>(R[q], n_own, maxD, hFac, sK, K0, xs, ys);
     }
-    base += RCHUNK * BLOCK;
+    base += RCH * BLOCK;
     if (base >= rec.n_elem) break;
 #pragma unroll
-    for (int q = 0; q < RCHUNK; ++q) {
+    for (int q = 0; q < RCH; ++q) {
       const uint32_t r = base + threadIdx.x + q * BLOCK;
       if (r < rec.n_elem) load_rec<TC, MASS>(R[q], rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
     }

@@ -1,0 +1,354 @@
+// Batched (multi right-hand side) kernels of libdendrohip.
+//
+// k right-hand sides are stored node-major: X[n * k + j], so the k values of a node are one
+// contiguous row and every node gather is a coalesced row segment.  The operator kernel
+// keeps the owner-computes tiling of the single right-hand-side kernel; one workgroup
+// handles one tile and one slice of KS right-hand sides, staging the slice of every touched
+// node in LDS.  In FP32 the element products Y_e = K_e X_e ([8x8] x [8xKS]) run on the
+// matrix cores as v_mfma_f32_4x4x1_16b_f32 (16 independent 4x4 blocks per instruction:
+// 4 elements x 2 row blocks x 2 column blocks); FP64 uses the vector ALUs with the same
+// staging.  Row sums of owned nodes accumulate in LDS and are stored once per node.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "kernels.hpp"
+#include "vector_kernels.hpp"
+
+namespace dgh {
+
+constexpr int KS = 8;            // right-hand sides per workgroup slice
+constexpr int MAXK = 32;         // right-hand sides per batch
+constexpr int MBLOCK = 256;      // threads per batched-operator workgroup
+
+template <class T>
+struct MultiIO {
+  const T* x;      // [N][k] operator input
+  T* y;            // [N][k] output
+  const T* b;      // [N][k] right-hand sides (RESID, SMOOTH)
+  const T* invd;   // [N] inverse diagonal (SMOOTH)
+  T omega;         // Richardson scale (SMOOTH)
+  int k;           // right-hand sides per node row
+};
+
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+template <class T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+
+// LDS of one batched workgroup: owned sums [ys_len][KS], full child-0 stencils
+// [18][1 + MASS][64], touched x [max_touch][KS].
+template <class T, bool MASS>
+__host__ __device__ constexpr size_t multi_stencil_len() {
+  return size_t(NSTENCIL) * (MASS ? 2 : 1) * 64;
+}
+
+template <class T, bool MASS, int MODE>
+__global__ void __launch_bounds__(MBLOCK)
+tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
+                  const uint16_t* __restrict__ lid, const uint16_t* __restrict__ meta,
+                  const T* __restrict__ kappa, const T* __restrict__ mass, double lap_fac,
+                  double mass_fac, uint32_t maxD, double hFac, const T* __restrict__ K0,
+                  const uint32_t* __restrict__ node_tab, uint32_t ys_len, MultiIO<T> io) {
+  static_assert(MODE != OP_DIAG, "the diagonal does not depend on the right-hand side");
+  constexpr int NM = MASS ? 2 : 1;
+  constexpr int VW = 16 / sizeof(T);          // values per 16-byte vector
+  constexpr int NV = KS / VW;                 // vectors per node slice
+  using V = typename Vec16<T>::type;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* ys = reinterpret_cast<T*>(smem_raw);
+  T* sK = ys + size_t(ys_len) * KS;
+  T* xs = sK + multi_stencil_len<T, MASS>();
+  const int np = ntiles * nslices;
+  const int pidx = xcd_tile(blockIdx.x, np);   // the slices of a tile stay on one XCD
+  if (pidx >= np) return;
+  const int t = pidx / nslices, sl = pidx % nslices;
+  const TileRec rec = tiles[t];
+  const uint32_t* nodes = node_tab + rec.node_off;
+  const uint32_t n_own = rec.n_own;
+  const size_t kv = size_t(io.k / VW);        // node row length in vectors
+  // full 8x8 child-0 stencils from the upper triangles [18][72]
+  for (int i = threadIdx.x; i < int(multi_stencil_len<T, MASS>()); i += MBLOCK) {
+    const int tp = i / (NM * 64), mm = (i / 64) % NM, kj = i % 64;
+    sK[i] = K0[tp * KSTRIDE + mm * 36 + tri(kj >> 3, kj & 7)];
+  }
+  {
+    const V* X = reinterpret_cast<const V*>(io.x);
+    V* xsv = reinterpret_cast<V*>(xs);
+    for (uint32_t i = threadIdx.x; i < rec.n_touch * NV; i += MBLOCK) {
+      const uint32_t nd = i / NV, part = i % NV;
+      xsv[i] = X[size_t(nodes[nd]) * kv + size_t(sl) * NV + part];
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < n_own * KS; i += MBLOCK) ys[i] = T(0);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int NW = MBLOCK / 64;
+  if constexpr (std::is_same<T, float>::value) {
+    // lane = 4 * block + q; block = 4 * es + 2 * ib + jb: element es of the round, row
+    // block ib (nodes 4ib..4ib+3), column block jb (right-hand sides 4jb..4jb+3).  MFMA
+    // operands: A = K[4ib + q][kk], B = X[kk][4jb + q]; result lane 4*block + q, register
+    // r = Y[4ib + r][4jb + q].
+    const int blk = lane >> 2, q = lane & 3;
+    const int es = blk >> 2, ib = (blk >> 1) & 1, jb = blk & 1;
+    for (uint32_t r0 = uint32_t(wv) * 4; r0 < rec.n_elem; r0 += NW * 4) {
+      const uint32_t r = r0 + es;
+      const bool act = r < rec.n_elem;
+      uint32_t L[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      float a[8], bv[8];
+      if (act) {
+        Rec R;
+        load_rec<T, MASS>(R, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
+        float f1, f2;
+        elem_factors<float>(R.mt >> 5, maxD, hFac, R.k1, R.k2, f1, f2);
+        const float* Lrow = sK + (R.mt & 31u) * (NM * 64) + (4 * ib + q) * 8;
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+          L[kk] = R.l[kk];
+          a[kk] = MASS ? f1 * Lrow[kk] + f2 * Lrow[64 + kk] : f1 * Lrow[kk];
+          bv[kk] = xs[R.l[kk] * KS + 4 * jb + q];
+        }
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) a[kk] = bv[kk] = 0.f;
+      }
+      mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(a[kk], bv[kk], acc, 0, 0, 0);
+      if (act) {
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const uint32_t nd = L[4 * ib + rr];
+          if (nd < n_own) atomic_add(&ys[nd * KS + 4 * jb + q], acc[rr]);
+        }
+      }
+    }
+  } else {
+    // lane = 16 * es + 8 * ib + c: element es, rows 4ib..4ib+3, right-hand side c
+    const int es = lane >> 4, ib = (lane >> 3) & 1, c = lane & 7;
+    for (uint32_t r0 = uint32_t(wv) * 4; r0 < rec.n_elem; r0 += NW * 4) {
+      const uint32_t r = r0 + es;
+      if (r >= rec.n_elem) continue;
+      Rec R;
+      load_rec<T, MASS>(R, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
+      T f1, f2;
+      elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, f1, f2);
+      const T* Ks = sK + (R.mt & 31u) * (NM * 64) + (4 * ib) * 8;
+      T xv[8];
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) xv[kk] = xs[R.l[kk] * KS + c];
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        T acc = T(0);
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+          const T kv2 = MASS ? f1 * Ks[rr * 8 + kk] + f2 * Ks[64 + rr * 8 + kk] : f1 * Ks[rr * 8 + kk];
+          acc += kv2 * xv[kk];
+        }
+        const uint32_t nd = R.l[4 * ib + rr];
+        if (nd < n_own) atomic_add(&ys[nd * KS + c], acc);
+      }
+    }
+  }
+  __syncthreads();
+  const size_t k = size_t(io.k);
+  for (uint32_t i = threadIdx.x; i < n_own * KS; i += MBLOCK) {
+    const uint32_t nd = i / KS, c = i % KS;
+    const size_t gi = size_t(nodes[nd]) * k + size_t(sl) * KS + c;
+    const T v = ys[i];
+    if (MODE == OP_MATVEC) {
+      io.y[gi] = v;
+    } else if (MODE == OP_RESID) {
+      io.y[gi] = io.b[gi] - v;
+    } else {
+      const T rr = io.b[gi] - v;
+      io.y[gi] = xs[i] + io.omega * (rr * io.invd[nodes[nd]]);
+    }
+  }
+}
+
+// x[n][j] = omega (b[n][j] invd[n])  (first sweep from a zero guess)
+template <class T>
+__global__ void smooth_first_multi_kernel(uint32_t n, int k, T omega, const T* __restrict__ b,
+                                          const T* __restrict__ invd, T* __restrict__ x) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(n) * k) return;
+  x[i] = T(0) + omega * (b[i] * invd[i / k]);
+}
+
+// c[i][j] = sum_p w_p f[col_p][j]: lanes run over the right-hand sides of a row.
+template <class T>
+static __global__ void __launch_bounds__(256)
+restrict_multi_kernel(uint32_t nc, int k, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                      const float* __restrict__ w, const T* __restrict__ f, T* __restrict__ c) {
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t i = gid / k;
+  const int j = int(gid % k);
+  if (i >= nc) return;
+  T acc = T(0);
+  const uint32_t p1 = rowptr[i + 1];
+  for (uint32_t p = rowptr[i]; p < p1; ++p) acc += T(w[p]) * f[size_t(col[p]) * k + j];
+  c[i * k + j] = acc;
+}
+
+// f_out[j][q] = f_in[j][q] + sum_m W[s][pos_j][m] c[cnl[m]][q]
+template <class T>
+static __global__ void __launch_bounds__(256)
+prolong_multi_kernel(uint32_t nf, int k, const uint32_t* __restrict__ celem, const uint8_t* __restrict__ pos,
+                     const uint32_t* __restrict__ cnl, const uint8_t* __restrict__ cs,
+                     const T* __restrict__ W, const T* __restrict__ c, const T* __restrict__ f_in,
+                     T* __restrict__ f_out) {
+  const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t j = gid / k;
+  const int q = int(gid % k);
+  if (j >= nf) return;
+  const uint32_t ce = celem[j];
+  const T* wv = W + (size_t(cs[ce]) * 27 + pos[j]) * 8;
+  T acc = T(0);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const T wk = wv[m];
+    if (wk != T(0)) acc += wk * c[size_t(cnl[size_t(ce) * 8 + m]) * k + q];
+  }
+  f_out[gid] = (f_in ? f_in[gid] : T(0)) + acc;
+}
+
+// X = Ainv B for the dense coarsest inverse (row pitch ld) and k right-hand sides
+// ([n][k]).  A workgroup computes 32 rows; Ainv and B chunks of 64 columns pass through LDS.
+constexpr int DM_ROWS = 32, DM_CH = 64;
+template <class T>
+static __global__ void __launch_bounds__(256)
+dense_multi_kernel(int n, int ld, int k, const T* __restrict__ Ainv, const T* __restrict__ B, T* __restrict__ X) {
+  __shared__ T sa[DM_ROWS][DM_CH + 1];
+  __shared__ T sb[DM_CH][MAXK];
+  const int row0 = blockIdx.x * DM_ROWS;
+  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;   // row, column group (k / 8 columns)
+  const int cpt = k / 8;
+  double acc[MAXK / 8] = {0.0, 0.0, 0.0, 0.0};
+  for (int m0 = 0; m0 < n; m0 += DM_CH) {
+    for (int i = threadIdx.x; i < DM_ROWS * DM_CH; i += 256) {
+      const int r = i / DM_CH, m = i % DM_CH;
+      sa[r][m] = (row0 + r < n && m0 + m < n) ? Ainv[size_t(row0 + r) * ld + m0 + m] : T(0);
+    }
+    for (int i = threadIdx.x; i < DM_CH * k; i += 256) {
+      const int m = i / k, j = i % k;
+      sb[m][j] = (m0 + m < n) ? B[size_t(m0 + m) * k + j] : T(0);
+    }
+    __syncthreads();
+    for (int m = 0; m < DM_CH; ++m) {
+      const double a = double(sa[tr][m]);
+#pragma unroll
+      for (int u = 0; u < MAXK / 8; ++u)
+        if (u < cpt) acc[u] += a * double(sb[m][tc * cpt + u]);
+    }
+    __syncthreads();
+  }
+  if (row0 + tr < n)
+    for (int u = 0; u < cpt; ++u) X[size_t(row0 + tr) * k + tc * cpt + u] = T(acc[u]);
+}
+
+// ---------------------------------------------------------------- column reductions
+// out[q][j] = sum_n a_q[n][j] * b_q[n][j] (b_q == nullptr: sum of a_q), deterministic:
+// stage 1 writes part[(q * RED_BLOCKS_M + block) * k + j], stage 2 sums over blocks in order.
+constexpr int RED_BLOCKS_M = 256;
+template <class T>
+static __global__ void __launch_bounds__(256)
+reduce1_multi_kernel(uint32_t n, int k, DotArgs<T> d, double* __restrict__ part) {
+  const int j = threadIdx.x % k, rs = threadIdx.x / k, nrs = 256 / k;
+  double s[MAXQ] = {0.0, 0.0, 0.0, 0.0};
+  const uint32_t per = (n + RED_BLOCKS_M - 1) / RED_BLOCKS_M;
+  const uint32_t lo = blockIdx.x * per, hi = min(n, lo + per);
+  if (rs < nrs)
+    for (uint32_t i = lo + rs; i < hi; i += nrs) {
+      const size_t g = size_t(i) * k + j;
+#pragma unroll
+      for (int q = 0; q < MAXQ; ++q)
+        if (d.a[q]) s[q] += double(d.a[q][g]) * (d.b[q] ? double(d.b[q][g]) : 1.0);
+    }
+  __shared__ double red[MAXQ][256];
+#pragma unroll
+  for (int q = 0; q < MAXQ; ++q) red[q][threadIdx.x] = s[q];
+  __syncthreads();
+  if (threadIdx.x < k) {
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+      double t = 0.0;
+      for (int r = 0; r < nrs; ++r) t += red[q][r * k + threadIdx.x];
+      part[(size_t(q) * RED_BLOCKS_M + blockIdx.x) * k + threadIdx.x] = t;
+    }
+  }
+}
+
+// out[q * ostride + j] for q < nq, j < k
+static __global__ void __launch_bounds__(256)
+reduce2_multi_kernel(int nq, int k, const double* __restrict__ part, double* __restrict__ out, int ostride) {
+  const int q = threadIdx.x / MAXK, j = threadIdx.x % MAXK;
+  if (q >= nq || j >= k) return;
+  double t = 0.0;
+  for (int b = 0; b < RED_BLOCKS_M; ++b) t += part[(size_t(q) * RED_BLOCKS_M + b) * k + j];
+  out[q * ostride + j] = t;
+}
+
+// v[n][j] -= sum[j] / n
+template <class T>
+__global__ void subtract_mean_multi_kernel(uint32_t n, int k, const double* __restrict__ sum, T* __restrict__ v) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < size_t(n) * k) v[i] = v[i] - T(sum[i % k] / double(n));
+}
+
+// ---------------------------------------------------------------- batched CG
+// Scalars per right-hand side j: SM[q * MAXK + j] with q a CgScalar, plus SM_DP0 (initial
+// norm) and SM_ACT (1 while the column iterates).  A converged column is frozen: its
+// step length is zero, so its iterate stays the one that met the test.
+enum { SM_DP0 = S_NSCAL, SM_ACT, SM_NSCAL };
+
+static __global__ void cg_p_multi_kernel(uint32_t n, int k, const double* __restrict__ S, const double* __restrict__ z,
+                                  double* __restrict__ p) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int j = int(i % k);
+  if (i < size_t(n) * k && S[SM_ACT * MAXK + j] != 0.0) p[i] = z[i] + S[S_RATIO * MAXK + j] * p[i];
+}
+
+static __global__ void cg_xr_multi_kernel(uint32_t n, int k, double* __restrict__ S, const double* __restrict__ p,
+                                   const double* __restrict__ w, double* __restrict__ x, double* __restrict__ r) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int j = int(i % k);
+  const bool act = S[SM_ACT * MAXK + j] != 0.0;
+  const double a = act ? S[S_RZ * MAXK + j] / S[S_PW * MAXK + j] : 0.0;
+  if (i < size_t(k)) S[S_ALPHA * MAXK + j] = a;
+  if (act && i < size_t(n) * k) {
+    x[i] = x[i] + a * p[i];
+    r[i] = r[i] + (-a) * w[i];
+  }
+}
+
+static __global__ void cg_scalar_multi_kernel(int k, double* __restrict__ S, int precond_norm, int flexible, double rtol) {
+  const int j = threadIdx.x;
+  if (j >= k) return;
+  if (S[SM_ACT * MAXK + j] == 0.0) return;
+  const double dp = sqrt(precond_norm ? S[S_ZZ * MAXK + j] : S[S_RR * MAXK + j]);
+  const double ratio = flexible ? (-S[S_ALPHA * MAXK + j] * S[S_ZW * MAXK + j]) / S[S_RZ * MAXK + j]
+                                : S[S_ZR * MAXK + j] / S[S_RZ * MAXK + j];
+  S[S_DP * MAXK + j] = dp;
+  S[S_RATIO * MAXK + j] = ratio;
+  S[S_RZ * MAXK + j] = S[S_ZR * MAXK + j];
+  if (!(dp > rtol * S[SM_DP0 * MAXK + j])) S[SM_ACT * MAXK + j] = 0.0;
+}
+
+static __global__ void cg_init_multi_kernel(int k, double* __restrict__ S, int precond_norm, double rtol) {
+  const int j = threadIdx.x;
+  if (j >= k) return;
+  S[S_RZ * MAXK + j] = S[S_ZR * MAXK + j];
+  const double dp = sqrt(precond_norm ? S[S_ZZ * MAXK + j] : S[S_BB * MAXK + j]);
+  S[S_DP * MAXK + j] = dp;
+  S[SM_DP0 * MAXK + j] = dp;
+  S[S_RATIO * MAXK + j] = 0.0;
+  S[SM_ACT * MAXK + j] = dp > 0.0 ? 1.0 : 0.0;
+  (void)rtol;
+}
+
+}  // namespace dgh

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace dgh {
 
@@ -75,7 +76,7 @@ reduce1_kernel(uint32_t n, DotArgs<T> d, double* __restrict__ part) {
 }
 
 constexpr int RED2_THREADS = 256;
-__global__ void __launch_bounds__(RED2_THREADS)
+static __global__ void __launch_bounds__(RED2_THREADS)
 reduce2_kernel(int nq, const double* __restrict__ part, double* __restrict__ out) {
   __shared__ double red[RED2_THREADS / 64];
   for (int q = 0; q < nq; ++q) {
@@ -106,14 +107,14 @@ __global__ void subtract_mean_kernel(uint32_t n, const double* __restrict__ sum,
 enum CgScalar { S_RZ = 0, S_PW, S_DP, S_RATIO, S_ZZ, S_ZR, S_ZW, S_RR, S_BB, S_ALPHA, S_PROJ, S_NSCAL = 16 };
 
 // p = z + ratio p  (VecAYPX)
-__global__ void cg_p_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ z,
+static __global__ void cg_p_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ z,
                             double* __restrict__ p) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = z[i] + S[S_RATIO] * p[i];
 }
 
 // a = rz / pw;  x = x + a p;  r = r + (-a) w  (VecAXPY twice)
-__global__ void cg_xr_kernel(uint32_t n, double* __restrict__ S, const double* __restrict__ p,
+static __global__ void cg_xr_kernel(uint32_t n, double* __restrict__ S, const double* __restrict__ p,
                              const double* __restrict__ w, double* __restrict__ x, double* __restrict__ r) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const double a = S[S_RZ] / S[S_PW];
@@ -125,7 +126,7 @@ __global__ void cg_xr_kernel(uint32_t n, double* __restrict__ S, const double* _
 }
 
 // after z = B r and the dots: dp, the next p-update factor, rz
-__global__ void cg_scalar_kernel(double* __restrict__ S, int precond_norm, int flexible) {
+static __global__ void cg_scalar_kernel(double* __restrict__ S, int precond_norm, int flexible) {
   if (threadIdx.x != 0) return;
   const double dp = sqrt(precond_norm ? S[S_ZZ] : S[S_RR]);
   const double ratio = flexible ? (-S[S_ALPHA] * S[S_ZW]) / S[S_RZ] : S[S_ZR] / S[S_RZ];
@@ -135,7 +136,7 @@ __global__ void cg_scalar_kernel(double* __restrict__ S, int precond_norm, int f
 }
 
 // before the first iteration: rz, dp0, ratio = 0 (p starts at zero)
-__global__ void cg_init_kernel(double* __restrict__ S, int precond_norm) {
+static __global__ void cg_init_kernel(double* __restrict__ S, int precond_norm) {
   if (threadIdx.x != 0) return;
   S[S_RZ] = S[S_ZR];
   S[S_DP] = sqrt(precond_norm ? S[S_ZZ] : S[S_BB]);
@@ -148,27 +149,54 @@ __global__ void copy_kernel(uint32_t n, const T* __restrict__ a, T* __restrict__
   if (i < n) b[i] = a[i];
 }
 
-// x = Ainv b for a dense symmetric n x n inverse (row-major), one wavefront per row.
+// x = Ainv b for a dense symmetric n x n inverse stored row-major with row pitch ld (a
+// multiple of DENSE_PAD elements, 16-byte aligned rows), one wavefront per row.  Each lane
+// streams 16-byte vectors, four in flight; sums in FP64 in a fixed order.
+constexpr int DENSE_PAD = 8;
 template <class T>
 __global__ void __launch_bounds__(256)
-dense_apply_kernel(int n, const T* __restrict__ Ainv, const T* __restrict__ b, T* __restrict__ x) {
+dense_apply_kernel(int n, int ld, const T* __restrict__ Ainv, const T* __restrict__ b, T* __restrict__ x) {
+  constexpr int VEC = 16 / sizeof(T);
+  using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const T* a = Ainv + size_t(row) * n;
-  double s = 0.0;
-  for (int j = lane; j < n; j += 64) s += double(a[j]) * double(b[j]);
-  s = wave_sum(s);
-  if (lane == 0) x[row] = T(s);
+  const V* a = reinterpret_cast<const V*>(Ainv + size_t(row) * ld);
+  const V* bv = reinterpret_cast<const V*>(b);
+  const int nv = n / VEC;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int v0 = lane; v0 < nv; v0 += 256) {
+    V av[4], xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int v = v0 + 64 * u;
+      if (v < nv) { av[u] = a[v]; xv[u] = bv[v]; }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (v0 + 64 * u < nv) {
+        const T* pa = reinterpret_cast<const T*>(&av[u]);
+        const T* px = reinterpret_cast<const T*>(&xv[u]);
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) s[u] += double(pa[c]) * double(px[c]);
+      }
+    }
+  }
+  double t = (s[0] + s[1]) + (s[2] + s[3]);
+  const int j = nv * VEC + lane;    // tail
+  if (j < n) t += double(Ainv[size_t(row) * ld + j]) * double(b[j]);
+  t = wave_sum(t);
+  if (lane == 0) x[row] = T(t);
 }
 
 // copy the lower triangle (column-major lower == row-major upper) to a full symmetric matrix
+// of pitch ld
 template <class T>
-__global__ void symmetrize_kernel(int n, T* __restrict__ A) {
+__global__ void symmetrize_kernel(int n, int ld, T* __restrict__ A) {
   const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n || i >= n || j <= i) return;
-  // column-major lower: element (r=j, c=i) stored at i*n + j, row-major view (i, j)
-  A[size_t(j) * n + i] = A[size_t(i) * n + j];
+  // column-major lower: element (r=j, c=i) stored at i*ld + j, row-major view (i, j)
+  A[size_t(j) * ld + i] = A[size_t(i) * ld + j];
 }
 
 }  // namespace dgh

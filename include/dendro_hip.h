@@ -190,8 +190,19 @@ int dgh_matvec(dgh_mesh m, int prec, const void* x_dev, void* y_dev, void* strea
 /* d = diag(A).  Replaces Jacobian1MatGetDiagonal (odaJac.C:160-223) and
  * Jacobian2MatGetDiagonal (omgJac.C:1594-1662). */
 int dgh_diagonal(dgh_mesh m, int prec, void* d_dev, void* stream);
-/* k right-hand sides at once: X, Y are [N][k] row-major (node-major, RHS fastest). */
+/* Y = A X for k right-hand sides at once (k a multiple of 8, at most 32): X and Y are
+ * [N][k] row-major (node-major, right-hand side fastest).  The reference applies
+ * Jacobian1MatMult / Jacobian2MatMult (odaJac.C:290-374, omgJac.C:1687-1775) to one vector
+ * per call; each column here equals that product.  FP32 runs the element products on the
+ * matrix cores. */
 int dgh_matvec_multi(dgh_mesh m, int prec, int k, const void* x_dev, void* y_dev, void* stream);
+/* Tile size (elements per tile) of the batched operator; rounded down until its LDS fits.
+ * No reference counterpart (tuning hook). */
+int dgh_mesh_set_tile_multi(dgh_mesh m, uint32_t tile_elems);
+/* Times the FP32 batched operator for k columns over the candidate tile sizes and keeps the
+ * fastest; outputs (may be NULL) the tile size and its time per launch.  No reference
+ * counterpart. */
+int dgh_mesh_autotune_multi(dgh_mesh m, int k, uint32_t* tile_elems, float* us);
 
 /* ================================================================ device: intergrid transfer */
 typedef struct dgh_transfer_s* dgh_transfer;
@@ -254,6 +265,14 @@ int dgh_solver_destroy(dgh_solver s);
  * starting with the initial one. */
 int dgh_solver_solve(dgh_solver s, const double* b_dev, double* x_dev, dgh_solve_stats* stats,
                      double* history, int history_len, void* stream);
+/* k independent solves at once (k a multiple of 8, at most 32): B and X are FP64 [N][k]
+ * row-major on the device.  Each column runs the KSPSolve CG of dgh_solver_solve with its
+ * own scalars and stopping test; all columns share one batched FP32 V-cycle per iteration
+ * (FP64 outer CG, flexible update), captured as one hipGraph when params.use_graph is set.
+ * A converged column is frozen.  stats: array of k (may be NULL).  Replaces k calls of
+ * DAMGSolve (omg.C:1121) on one hierarchy. */
+int dgh_solver_solve_multi(dgh_solver s, int k, const double* B_dev, double* X_dev, dgh_solve_stats* stats,
+                           void* stream);
 /* z = B r: one application of the V-cycle preconditioner (PCApply_MG), FP64 vectors. */
 int dgh_solver_vcycle(dgh_solver s, const double* r_dev, double* z_dev, void* stream);
 
