@@ -19,7 +19,7 @@ LIB = os.path.join(LIBDIR, "libdendrohip.so")
 
 HOST_SOURCES = ["host/stencils.cpp", "host/octree.cpp", "host/da.cpp", "host/tiles.cpp",
                 "host/host_api.cpp"]
-HIP_SOURCES = ["device/device_api.hip", "device/multi_api.hip"]
+HIP_SOURCES = ["device/device_api.hip", "device/multi_api.hip", "device/learn_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed-FP32 instructions are disabled for every HIP translation unit: some of their forms

@@ -34,6 +34,8 @@ EXPORTS = [
     "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add",
     "dgh_solver_default_params", "dgh_solver_create", "dgh_solver_destroy", "dgh_solver_solve",
     "dgh_solver_solve_multi", "dgh_solver_vcycle",
+    "dgh_learn_default_params", "dgh_learned_init", "dgh_learned_get_params", "dgh_learned_set_params",
+    "dgh_learned_train", "dgh_learned_apply", "dgh_learned_eval", "dgh_learned_loss",
 ]
 
 
@@ -60,6 +62,17 @@ class SolveStats(C.Structure):
                 ("rnorm", C.c_double), ("rel_true", C.c_double), ("seconds", C.c_double)]
 
 
+class LearnParams(C.Structure):
+    _fields_ = [("steps", C.c_int), ("samples", C.c_int), ("nu", C.c_int), ("refresh", C.c_int),
+                ("lr", C.c_float), ("seed", C.c_uint64)]
+
+
+class LearnStats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("loss_first", C.c_double), ("loss_last", C.c_double),
+                ("seconds", C.c_double)]
+
+
+LEARN_NPARAM = 6984
 _lib = None
 
 
@@ -377,6 +390,51 @@ class Solver:
 
     def vcycle(self, r, z, stream=None):
         check(lib().dgh_solver_vcycle(self.h, ptr(r), ptr(z), _stream(stream)))
+
+    # ---- learned Jacobi damping
+    def learned_init(self, seed):
+        check(lib().dgh_learned_init(self.h, C.c_uint64(seed)))
+
+    def learned_params(self):
+        th = np.zeros(LEARN_NPARAM, np.float32)
+        check(lib().dgh_learned_get_params(self.h, ptr(th), LEARN_NPARAM))
+        return th
+
+    def set_learned_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        check(lib().dgh_learned_set_params(self.h, ptr(th), int(th.size)))
+
+    def learned_train(self, **kw):
+        p = LearnParams()
+        check(lib().dgh_learn_default_params(C.byref(p)))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        st = LearnStats()
+        check(lib().dgh_learned_train(self.h, C.byref(p), C.byref(st)))
+        return st
+
+    def learned_apply(self, enable=True, samples=8, seed=6000):
+        mask = C.c_int()
+        check(lib().dgh_learned_apply(self.h, int(enable), int(samples), C.c_uint64(seed), C.byref(mask)))
+        return mask.value
+
+    def learned_eval(self, level, n_elem, dO=None):
+        O = np.zeros((n_elem, 8), np.float32)
+        g = np.zeros(LEARN_NPARAM, np.float32)
+        d = None if dO is None else np.ascontiguousarray(dO, np.float32)
+        check(lib().dgh_learned_eval(self.h, int(level), ptr(d), ptr(O), ptr(g) if d is not None else None))
+        return (O, g) if d is not None else O
+
+    def learned_loss(self, level, n_node, omega=None, samples=None, S=8, nu=2, seed=0, grad=True):
+        om = None if omega is None else np.ascontiguousarray(omega, np.float32)
+        sm = None if samples is None else np.ascontiguousarray(samples, np.float32)
+        if sm is not None:
+            S = sm.shape[1]
+        loss = C.c_double()
+        g = np.zeros(n_node, np.float32) if grad else None
+        check(lib().dgh_learned_loss(self.h, int(level), ptr(om), ptr(sm), int(S), int(nu), C.c_uint64(seed),
+                                     C.byref(loss), ptr(g)))
+        return (loss.value, g) if grad else loss.value
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

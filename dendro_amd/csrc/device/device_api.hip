@@ -194,7 +194,7 @@ static T* vcycle(dgh_solver s, int l, const T* b, hipStream_t st) {
   }
   dgh_mesh m = s->meshes[l];
   const uint32_t n = m->N;
-  const T om = T(s->prm.omega);
+  const T om = T(s->om_level[l]);
   T* xc = V.xa.p;
   T* xn = V.xb.p;
   // pre-smoothing: the first sweep from x = 0 needs no operator application
@@ -628,10 +628,12 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
     if (params->nu_pre < 1 || params->nu_post < 0 || params->max_it < 0 || !(params->omega > 0))
       fail(DGH_ERR_ARG, "invalid smoother parameters");
     if (params->vcycle_prec != DGH_F64 && params->vcycle_prec != DGH_F32) fail(DGH_ERR_ARG, "bad vcycle_prec");
-    if (params->smoother != DGH_SMOOTHER_JACOBI) fail(DGH_ERR_STATE, "learned smoother not available in this build");
+    if (params->smoother != DGH_SMOOTHER_JACOBI && params->smoother != DGH_SMOOTHER_LEARNED)
+      fail(DGH_ERR_ARG, "unknown smoother %d", params->smoother);
     auto s = std::make_unique<dgh_solver_s>();
     s->nlevels = nlevels;
     s->prm = *params;
+    s->om_level.assign(nlevels, params->omega);
     s->meshes.assign(meshes, meshes + nlevels);
     if (nlevels > 1) s->transfers.assign(transfers, transfers + nlevels - 1);
     for (int l = 0; l < nlevels; ++l) {

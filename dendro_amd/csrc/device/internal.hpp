@@ -137,6 +137,35 @@ struct LevelVecs {
 
 }  // namespace dgh
 
+namespace dgh {
+
+// Learned damping of one level (element features, node CSR of (e, k) slots, caps, current
+// logits and weights, training samples) and the network state of a solver.
+struct LearnLevel {
+  bool ready = false;
+  DevBuf<uint8_t> lev;
+  DevBuf<float> lke, lkbar, cap, dinv32;   // dinv32: plain 1 / A_nn
+  DevBuf<double> dinv64;
+  DevBuf<uint32_t> rowptr, slot;
+  DevBuf<float> O, dO, theta_n, omega, gomega;
+  DevBuf<float> e0;                        // samples [N][S]
+  int S = 0;
+  int age = 0;                             // training steps since the samples were drawn
+};
+
+struct LearnState {
+  bool init = false;
+  DevBuf<float> theta, m1, m2, grad, gpart;
+  DevBuf<uint16_t> wbf;
+  int step = 0;
+  std::vector<LearnLevel> lv;
+  DevBuf<float> t, g, u, es, z;            // objective scratch, sized for the finest level
+  DevBuf<double> dots;
+  std::vector<int> on;                     // levels whose smoother uses the learned weights
+};
+
+}  // namespace dgh
+
 struct dgh_solver_s {
   int nlevels = 0;
   std::vector<dgh_mesh> meshes;
@@ -165,7 +194,10 @@ struct dgh_solver_s {
   double* hmS = nullptr;
   hipGraph_t mgraph = nullptr;
   hipGraphExec_t mgexec = nullptr;
+  std::vector<double> om_level;        // Richardson scale per level (1 where learned weights sit in invd)
+  LearnState* learn = nullptr;
   ~dgh_solver_s() {
+    delete learn;
     if (mgexec) (void)hipGraphExecDestroy(mgexec);
     if (mgraph) (void)hipGraphDestroy(mgraph);
     if (hmS) (void)hipHostFree(hmS);
@@ -179,4 +211,18 @@ struct dgh_solver_s {
 namespace dgh {
 void upload_coefs(dgh_mesh m, DevTiles& d);
 void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&));
+}  // namespace dgh
+
+namespace dgh {
+// batched pieces shared by the batched solver and the learned-damping trainer (multi_api.hip)
+void ensure_multi(dgh_mesh m);
+void ensure_multi_buffers(dgh_solver s, int k);
+void multi_op_f32(dgh_mesh m, int mode, const float* x, float* y, const float* b, const float* invd, float om,
+                  int k, hipStream_t st);
+float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st);
+void restrict_multi_f32(dgh_transfer t, int k, const float* f, float* c, hipStream_t st);
+void prolong_multi_f32(dgh_transfer t, int k, const float* c, const float* f_in, float* f_out, hipStream_t st);
+void project_mean_multi_f32(dgh_solver s, uint32_t n, int k, float* v, hipStream_t st);
+void column_dots_f32(uint32_t n, int k, const float* a, const float* b, double* part, double* out, hipStream_t st);
+void drop_graphs(dgh_solver s);
 }  // namespace dgh

@@ -31,7 +31,7 @@ static void plan_multi(dgh_mesh m, uint32_t T) {
   m->mtile = m->mop.plan.tile_elems;
 }
 
-static void ensure_multi(dgh_mesh m) {
+void ensure_multi(dgh_mesh m) {
   if (m->mop.plan.tiles.empty()) plan_multi(m, m->mtile ? m->mtile : 512);
 }
 
@@ -96,7 +96,7 @@ static void project_mean_multi(dgh_solver s, uint32_t n, int k, T* v, hipStream_
 }
 
 // Batched FP32 V-cycle on level l (the single right-hand-side vcycle, column by column).
-static float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st) {
+float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st) {
   auto& V = s->m32[l];
   if (l == 0) {
     const int n = s->nc;
@@ -114,7 +114,7 @@ static float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st
   }
   dgh_mesh m = s->meshes[l];
   const uint32_t n = m->N;
-  const float om = float(s->prm.omega);
+  const float om = float(s->om_level[l]);
   const float* invd = s->v32[l].invd.p;
   float* xc = V.xa.p;
   float* xn = V.xb.p;
@@ -173,7 +173,45 @@ static void mcg_iteration(dgh_solver s, int k, hipStream_t st) {
 
 // Sizes the batched buffers of a solver for k columns (FP32 per-level vectors, FP64 outer
 // CG vectors, FP32 level diagonals and coarse inverse).
-static void ensure_multi_buffers(dgh_solver s, int k) {
+void drop_graphs(dgh_solver s) {
+  if (s->mgexec) (void)hipGraphExecDestroy(s->mgexec);
+  if (s->mgraph) (void)hipGraphDestroy(s->mgraph);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->mgexec = nullptr;
+  s->mgraph = nullptr;
+  s->gexec = nullptr;
+  s->graph = nullptr;
+}
+
+void multi_op_f32(dgh_mesh m, int mode, const float* x, float* y, const float* b, const float* invd, float om,
+                  int k, hipStream_t st) {
+  const MultiIO<float> io{x, y, b, invd, om, k};
+  if (mode == OP_MATVEC) launch_multi<float, OP_MATVEC>(m, io, st);
+  else if (mode == OP_RESID) launch_multi<float, OP_RESID>(m, io, st);
+  else launch_multi<float, OP_SMOOTH>(m, io, st);
+}
+
+void restrict_multi_f32(dgh_transfer t, int k, const float* f, float* c, hipStream_t st) {
+  restrict_multi<float>(t, k, f, c, st);
+}
+
+void prolong_multi_f32(dgh_transfer t, int k, const float* c, const float* f_in, float* f_out, hipStream_t st) {
+  prolong_multi<float>(t, k, c, f_in, f_out, st);
+}
+
+void project_mean_multi_f32(dgh_solver s, uint32_t n, int k, float* v, hipStream_t st) {
+  project_mean_multi<float>(s, n, k, v, st);
+}
+
+void column_dots_f32(uint32_t n, int k, const float* a, const float* b, double* part, double* out, hipStream_t st) {
+  DotArgs<float> d{};
+  d.a[0] = a;
+  d.b[0] = b;
+  reduce_multi<float>(n, k, d, 1, part, out, st);
+}
+
+void ensure_multi_buffers(dgh_solver s, int k) {
   if (s->mk == k) return;
   if (s->mgexec) (void)hipGraphExecDestroy(s->mgexec);
   if (s->mgraph) (void)hipGraphDestroy(s->mgraph);

@@ -276,6 +276,53 @@ int dgh_solver_solve_multi(dgh_solver s, int k, const double* B_dev, double* X_d
 /* z = B r: one application of the V-cycle preconditioner (PCApply_MG), FP64 vectors. */
 int dgh_solver_vcycle(dgh_solver s, const double* r_dev, double* z_dev, void* stream);
 
+/* ================================================================ learned Jacobi damping
+ * No counterpart in the reference (its smoother is damped Jacobi, bin/options:187-193,
+ * which is also the fallback here).  A per-element MLP (features: level, hanging type,
+ * child number, 8 nodal log-coefficient ratios; 35 -> 64 -> 64 -> 8, ReLU, BF16 weights,
+ * FP32 accumulation) predicts one damping logit per element vertex; per node the logits are
+ * averaged and omega_n = cap_n * sigmoid(mean), with cap_n = 2 A_nn / sum |element row
+ * entries| (every weighted sweep is then an A-norm contraction).  The smoother of a level
+ * with learned weights runs x += omega_n / A_nn (b - A x).  Training runs on the device:
+ * samples are the errors the solver leaves after pre-smoothing and coarse correction, the
+ * objective is their A-norm reduction by nu weighted sweeps, optimiser Adam. */
+#define DGH_LEARN_NPARAM 6984
+typedef struct {
+  int steps;       /* training steps (one level per step, finest first, round robin)   */
+  int samples;     /* sample columns per step: a multiple of 8, at most 32              */
+  int nu;          /* weighted sweeps in the objective                                  */
+  int refresh;     /* steps of a level between regenerations of its samples             */
+  float lr;        /* Adam step size                                                    */
+  uint64_t seed;   /* sample seeds                                                      */
+} dgh_learn_params;
+typedef struct {
+  int steps;
+  double loss_first;   /* mean objective over the levels at their first step           */
+  double loss_last;    /* mean objective over the levels at their last step            */
+  double seconds;
+} dgh_learn_stats;
+int dgh_learn_default_params(dgh_learn_params* p);
+/* (Re)initialises the network (He-normal hidden weights from seed, output bias at the
+ * configured omega). */
+int dgh_learned_init(dgh_solver s, uint64_t seed);
+/* FP32 master parameters [DGH_LEARN_NPARAM] (host): W1[64][35] b1[64] W2[64][64] b2[64]
+ * W3[8][64] b3[8]. */
+int dgh_learned_get_params(dgh_solver s, float* theta, int n);
+int dgh_learned_set_params(dgh_solver s, const float* theta, int n);
+int dgh_learned_train(dgh_solver s, const dgh_learn_params* p, dgh_learn_stats* stats);
+/* Per level 1..nlevels-1 (coarse to fine): draws `samples` fresh samples and keeps the
+ * learned weights only if their objective beats plain damped Jacobi (params.omega) on them;
+ * levels_on (may be NULL) receives the bit mask of levels using learned weights.
+ * enable = 0 restores plain Jacobi everywhere.  Solves after this call use the result. */
+int dgh_learned_apply(dgh_solver s, int enable, int samples, uint64_t seed, int* levels_on);
+/* Test hooks.  eval: network outputs O [E][8] of a level (host, may be NULL) and, when dO
+ * [E][8] (host) is given, the gradient of sum(dO . O) [DGH_LEARN_NPARAM] (host, may be
+ * NULL).  loss: the objective of a level for weights omega [N] (host; NULL: the network's)
+ * on samples [N][S] (host; NULL: drawn from seed), with dL/domega [N] (host, may be NULL). */
+int dgh_learned_eval(dgh_solver s, int level, const float* dO, float* O, float* grad);
+int dgh_learned_loss(dgh_solver s, int level, const float* omega, const float* samples, int S, int nu,
+                     uint64_t seed, double* loss, float* g_omega);
+
 #ifdef __cplusplus
 }
 #endif

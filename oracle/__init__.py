@@ -30,6 +30,7 @@ def lib():
         _lib.or_mg_free.argtypes = [C.c_void_p]
         for f in ["or_mg_set_level", "or_mg_setup", "or_mg_vcycle", "or_pcg", "or_mg_apply"]:
             getattr(_lib, f).argtypes = None
+        _lib.or_learn_loss.restype = C.c_double
     return _lib
 
 
@@ -170,3 +171,86 @@ class MG:
         if getattr(self, "h", None):
             lib().or_mg_free(C.c_void_p(self.h))
             self.h = None
+
+
+# ------------------------------------------------------------------ learned Jacobi damping
+NF, NH, NO = 35, 64, 8
+NPARAM = NH * NF + NH + NH * NH + NH + NO * NH + NO
+
+
+def learn_node_logk(level, kappa=None, lap_fac=1.0):
+    k = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+    lkbar = np.zeros(level.n_node, np.float32)
+    lke = np.zeros(level.n_elem, np.float32)
+    lib().or_learn_node_logk(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), _p(level.nlist), _p(k),
+                             C.c_double(lap_fac), _p(lkbar), _p(lke))
+    return lkbar, lke
+
+
+def learn_features(level, kappa=None, lap_fac=1.0):
+    lkbar, lke = learn_node_logk(level, kappa, lap_fac)
+    X = np.zeros((level.n_elem, NF), np.float32)
+    lib().or_learn_features(C.c_uint32(level.n_elem), C.c_uint32(level.max_depth), _p(level.nlist),
+                            _p(level.child), _p(level.hnmask), _p(level.lev), _p(lkbar), _p(lke), _p(X))
+    return X
+
+
+def mlp_forward(theta, X, round_bf16=True):
+    theta = np.ascontiguousarray(theta, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    O = np.zeros((X.shape[0], NO))
+    lib().or_mlp_forward(_p(theta), C.c_int(int(round_bf16)), C.c_uint32(X.shape[0]), _p(X), _p(O))
+    return O
+
+
+def mlp_backward(theta, X, dO, round_bf16=True):
+    theta = np.ascontiguousarray(theta, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    dO = np.ascontiguousarray(dO, np.float64)
+    g = np.zeros(NPARAM)
+    lib().or_mlp_backward(_p(theta), C.c_int(int(round_bf16)), C.c_uint32(X.shape[0]), _p(X), _p(dO), _p(g))
+    return g
+
+
+def learn_cap(level, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0):
+    st = _st()
+    k, m = _coef(level, kappa, mass)
+    cap = np.zeros(level.n_node)
+    lib().or_learn_cap(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), C.c_uint32(level.max_depth),
+                       _p(level.nlist), _p(level.child), _p(level.hnmask), _p(level.lev), _p(k), _p(m),
+                       C.c_double(lap_fac), C.c_double(mass_fac), _p(st["lap2"]), _p(st["mass2"]), _p(cap))
+    return cap
+
+
+def learn_nodal(level, O, cap):
+    O = np.ascontiguousarray(O, np.float64)
+    cap = np.ascontiguousarray(cap, np.float64)
+    th = np.zeros(level.n_node)
+    om = np.zeros(level.n_node)
+    lib().or_learn_nodal(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), _p(level.nlist), _p(O), _p(cap),
+                         _p(th), _p(om))
+    return th, om
+
+
+def learn_nodal_grad(level, theta_n, cap, g_omega):
+    dO = np.zeros((level.n_elem, NO))
+    lib().or_learn_nodal_grad(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), _p(level.nlist),
+                              _p(np.ascontiguousarray(theta_n, np.float64)), _p(np.ascontiguousarray(cap, np.float64)),
+                              _p(np.ascontiguousarray(g_omega, np.float64)), _p(dO))
+    return dO
+
+
+def learn_loss(level, omega, E0, nu=2, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0, grad=True):
+    """L = mean_s |S_w^nu e_s|_A^2 / |e_s|_A^2 and dL/domega (E0: [N][S])."""
+    st = _st()
+    k, m = _coef(level, kappa, mass)
+    invd = 1.0 / diagonal(level, kappa, mass, lap_fac, mass_fac)
+    E0 = np.ascontiguousarray(E0, np.float64)
+    S = E0.shape[1]
+    g = np.zeros(level.n_node) if grad else None
+    L = lib().or_learn_loss(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), C.c_uint32(level.max_depth),
+                            _p(level.nlist), _p(level.child), _p(level.hnmask), _p(level.lev), _p(k), _p(m),
+                            C.c_double(lap_fac), C.c_double(mass_fac), _p(st["lap2"]), _p(st["mass2"]),
+                            _p(np.ascontiguousarray(omega, np.float64)), _p(invd), C.c_int(nu), C.c_int(S),
+                            _p(E0), _p(g))
+    return (L, g) if grad else L

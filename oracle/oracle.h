@@ -74,6 +74,34 @@ int or_pcg(or_mg* mg, const double* b, double* x, double rtol, int max_it, int n
 /* y = A x on level `level` of the hierarchy (the level's own operator). */
 int or_mg_apply(or_mg* mg, int level, const double* x, double* y);
 
+/* ---- learned Jacobi damping (learned.c; no reference counterpart) ---- */
+#define OR_NF 35                 /* element features */
+#define OR_NH 64                 /* hidden width (two layers) */
+#define OR_NO 8                  /* outputs: one damping logit per element vertex */
+#define OR_NPARAM (OR_NH * OR_NF + OR_NH + OR_NH * OR_NH + OR_NH + OR_NO * OR_NH + OR_NO)
+
+/* lkbar[n] = log10(mean kappa over the element vertices at node n), lke[e] = log10(kappa_e) */
+void or_learn_node_logk(uint32_t E, uint32_t N, const uint32_t* nlist, const double* kappa, double lap_fac,
+                        float* lkbar, float* lke);
+/* X[e] = [ (lev-1)/(maxD-1), onehot18(etype), onehot8(child), lkbar[n_k] - lke[e] (k<8) ] */
+void or_learn_features(uint32_t E, uint32_t max_depth, const uint32_t* nlist, const uint8_t* child,
+                       const uint8_t* hnmask, const uint8_t* lev, const float* lkbar, const float* lke,
+                       float* X);
+void or_mlp_forward(const float* theta, int round_bf16, uint32_t E, const float* X, double* O);
+void or_mlp_backward(const float* theta, int round_bf16, uint32_t E, const float* X, const double* dO,
+                     double* grad);
+void or_learn_cap(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist, const uint8_t* child,
+                  const uint8_t* hnmask, const uint8_t* lev, const double* kappa, const double* mass,
+                  double lap_fac, double mass_fac, const double* lap2, const double* mass2, double* cap);
+void or_learn_nodal(uint32_t E, uint32_t N, const uint32_t* nlist, const double* O, const double* cap,
+                    double* theta_n, double* omega);
+void or_learn_nodal_grad(uint32_t E, uint32_t N, const uint32_t* nlist, const double* theta_n, const double* cap,
+                         const double* g_omega, double* dO);
+double or_learn_loss(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist, const uint8_t* child,
+                     const uint8_t* hnmask, const uint8_t* lev, const double* kappa, const double* mass,
+                     double lap_fac, double mass_fac, const double* lap2, const double* mass2,
+                     const double* omega, const double* invd, int nu, int S, const double* E0, double* g_omega);
+
 #ifdef __cplusplus
 }
 #endif
