@@ -61,6 +61,35 @@ def cpu_baseline(F, kappa, budget_s=15.0):
                       f"(E={F.n_elem}, N={F.n_node}), {dt:.1f}s"}
 
 
+def bench_learned(meshes, trans, N, dev, extra):
+    """FP32 V-cycle PCG with the learned damping: the network is trained on the device on
+    200 batches of solver-generated samples (seeds 5000..5199), then every level keeps the
+    learned weights only where they beat plain Jacobi on fresh samples."""
+    import torch
+    import dendro_amd as da
+    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
+                          norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=da.F32,
+                          use_graph=1, smoother=da.SMOOTHER_LEARNED)
+    s = da.Solver(meshes, trans, p)
+    s.learned_init(1)
+    tr = s.learned_train(steps=200, samples=8, nu=2, refresh=1, lr=3e-3, seed=5000)
+    mask = s.learned_apply(True, samples=8, seed=6000)
+    b = torch.tensor(np.random.default_rng(22).uniform(-1, 1, N), dtype=torch.float64, device=dev)
+    xs = torch.zeros(N, dtype=torch.float64, device=dev)
+    s.solve(b, xs)
+    st, _ = s.solve(b, xs)
+    out = {"seconds": round(st.seconds, 5), "iterations": st.iterations, "rel_true": st.rel_true,
+           "converged": bool(st.converged), "levels_learned_mask": mask,
+           "train_seconds": round(tr.seconds, 3), "objective_first": round(tr.loss_first, 5),
+           "objective_last": round(tr.loss_last, 5)}
+    if "pcg_fp64" in extra:
+        out["speedup_vs_fp64"] = round(extra["pcg_fp64"]["seconds"] / st.seconds, 3)
+    log(f"PCG mixed+learned: {st.iterations} its, {st.seconds*1e3:.1f} ms (levels {mask:#x}); training "
+        f"{tr.seconds:.2f}s objective {tr.loss_first:.4f} -> {tr.loss_last:.4f}")
+    del s
+    return out
+
+
 def bench_multi(args, meshes, trans, N, dev, stream, extra):
     """32 right-hand sides (seeds 400..431): the batched FP32 operator and the batched
     PCG (one FP32 V-cycle per iteration for all columns, FP64 outer CG, one hipGraph)."""
@@ -136,6 +165,7 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="workgroup size with --tile (0: library default)")
     ap.add_argument("--sweep-tiles", action="store_true", help="time the MatVec for several tile sizes")
     ap.add_argument("--no-multi", action="store_true", help="skip the 32 right-hand-side batch")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-damping PCG")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -245,6 +275,9 @@ def main():
             log(f"PCG {name}: {st.iterations} its, {st.seconds*1e3:.1f} ms, rel_true {st.rel_true:.2e}")
             del s
         extra["pcg_speedup_mixed_vs_fp64"] = round(extra["pcg_fp64"]["seconds"] / extra["pcg_mixed"]["seconds"], 3)
+
+    if not args.no_pcg and not args.no_learned:
+        extra["pcg_mixed_learned"] = bench_learned(meshes, trans, N, dev, extra)
 
     if not args.no_multi:
         extra["multi_rhs"] = bench_multi(args, meshes, trans, N, dev, stream, extra)

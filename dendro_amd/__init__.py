@@ -4,7 +4,8 @@ The compute path is libdendrohip.so (HIP kernels for gfx950 behind the C ABI of
 include/dendro_hip.h); this package holds its build recipe and Python bindings.
 """
 from .core import (  # noqa: F401
-    DAMG, F32, F64, Mesh, NORM_PRECONDITIONED, NORM_UNPRECONDITIONED, Octree, PTS_CLUSTERS,
-    PTS_GAUSS, PTS_LOGNORMAL, PTS_MIXED, PTS_UNIFORM, Solver, DendroError, Transfer,
-    default_params, etype, init, lib, points, stencils,
+    DAMG, F32, F64, LEARN_NPARAM, Mesh, NORM_PRECONDITIONED, NORM_UNPRECONDITIONED, Octree,
+    PTS_CLUSTERS, PTS_GAUSS, PTS_LOGNORMAL, PTS_MIXED, PTS_UNIFORM, SMOOTHER_JACOBI,
+    SMOOTHER_LEARNED, Solver, DendroError, Transfer, default_params, etype, init, lib, points,
+    stencils,
 )
