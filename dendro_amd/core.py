@@ -318,13 +318,14 @@ class Mesh:
         """Y = A X for k columns stored node-major ([N][k])."""
         check(lib().dgh_matvec_multi(self.h, int(prec), int(k), ptr(x), ptr(y), _stream(stream)))
 
-    def set_tile_multi(self, tile_elems):
-        check(lib().dgh_mesh_set_tile_multi(self.h, int(tile_elems)))
+    def set_tile_multi(self, tile_elems, variant=0):
+        """variant: 0 vector-ALU element products, 1 matrix-core (MFMA) products (FP32)"""
+        check(lib().dgh_mesh_set_tile_multi(self.h, int(tile_elems), int(variant)))
 
     def autotune_multi(self, k):
-        te, us = C.c_uint32(), C.c_float()
-        check(lib().dgh_mesh_autotune_multi(self.h, int(k), C.byref(te), C.byref(us)))
-        return {"tile_elems": te.value, "us": us.value}
+        te, var, us = C.c_uint32(), C.c_int(), C.c_float()
+        check(lib().dgh_mesh_autotune_multi(self.h, int(k), C.byref(te), C.byref(var), C.byref(us)))
+        return {"tile_elems": te.value, "variant": var.value, "us": us.value}
 
     def diagonal(self, d, prec=F64, stream=None):
         check(lib().dgh_diagonal(self.h, int(prec), ptr(d), _stream(stream)))

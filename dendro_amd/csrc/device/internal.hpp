@@ -118,6 +118,7 @@ struct dgh_mesh_s {
   int block = 512;              // threads per operator workgroup (256, 512 or 1024)
   DevTiles mop;
   uint32_t mtile = 0;           // batched tile size (0: default)
+  int mvariant = 0;             // batched FP32 products: 0 vector ALUs, 1 matrix cores
 };
 
 struct dgh_transfer_s {
@@ -191,6 +192,7 @@ struct dgh_solver_s {
   DevBuf<double> mx, mr, mz, mp, mw;   // outer CG vectors [N][mk]
   DevBuf<double> mS;                   // per-column CG scalars
   DevBuf<double> mpart, mpart_p;       // column reduction partials
+  DevBuf<double> mdense;               // split partial sums of the batched coarse solve
   double* hmS = nullptr;
   hipGraph_t mgraph = nullptr;
   hipGraphExec_t mgexec = nullptr;

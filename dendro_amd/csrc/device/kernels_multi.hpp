@@ -20,9 +20,10 @@
 
 namespace dgh {
 
-constexpr int KS = 8;            // right-hand sides per workgroup slice
+constexpr int KS = 8;            // right-hand sides per workgroup slice (FP32)
+constexpr int KS64 = 4;          // right-hand sides per workgroup slice (FP64: same LDS bytes)
 constexpr int MAXK = 32;         // right-hand sides per batch
-constexpr int MBLOCK = 256;      // threads per batched-operator workgroup
+constexpr int MBLOCK = 512;      // threads per batched-operator workgroup (default)
 
 template <class T>
 struct MultiIO {
@@ -36,33 +37,31 @@ struct MultiIO {
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
-template <class T> struct Vec16;
-template <> struct Vec16<float> { using type = float4; };
-template <> struct Vec16<double> { using type = double2; };
+// Vector-ALU batched operator.  Lane = (element of the round, right-hand side c of the slice):
+// the KSL lanes of an element share its record, read the element's 8 node values of their
+// column from LDS and accumulate its 8 rows; the stencil comes through the scalar cache
+// (per-type wavefront loop of eval_rec), so LDS carries only the minimum per element and
+// column: 8 reads and at most 8 additions.  Sums accumulate in FP64 LDS for both precisions
+// (FP32 LDS additions are several times slower on this device).  LDS rows are padded to
+// KSL + 1 values so the lanes of different elements spread over the banks.
+// LDS: owned sums [ys_len][KSL + 1] (double), touched x [max_touch][KSL + 1] (T).
+template <class T>
+__host__ __device__ constexpr int multi_row(int ksl) { return ksl + 1; }
 
-// LDS of one batched workgroup: owned sums [ys_len][KS], full child-0 stencils
-// [18][1 + MASS][64], touched x [max_touch][KS].
-template <class T, bool MASS>
-__host__ __device__ constexpr size_t multi_stencil_len() {
-  return size_t(NSTENCIL) * (MASS ? 2 : 1) * 64;
-}
-
-template <class T, bool MASS, int MODE>
-__global__ void __launch_bounds__(MBLOCK)
+template <class T, bool MASS, int MODE, int KSL, int MB>
+__global__ void __launch_bounds__(MB)
 tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
                   const uint16_t* __restrict__ lid, const uint16_t* __restrict__ meta,
                   const T* __restrict__ kappa, const T* __restrict__ mass, double lap_fac,
                   double mass_fac, uint32_t maxD, double hFac, const T* __restrict__ K0,
                   const uint32_t* __restrict__ node_tab, uint32_t ys_len, MultiIO<T> io) {
   static_assert(MODE != OP_DIAG, "the diagonal does not depend on the right-hand side");
-  constexpr int NM = MASS ? 2 : 1;
-  constexpr int VW = 16 / sizeof(T);          // values per 16-byte vector
-  constexpr int NV = KS / VW;                 // vectors per node slice
-  using V = typename Vec16<T>::type;
+  static_assert(64 % KSL == 0, "a wavefront holds whole elements");
+  using TA = double;
+  constexpr int RS = multi_row<T>(KSL);      // padded LDS row
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* ys = reinterpret_cast<T*>(smem_raw);
-  T* sK = ys + size_t(ys_len) * KS;
-  T* xs = sK + multi_stencil_len<T, MASS>();
+  TA* ys = reinterpret_cast<TA*>(smem_raw);
+  T* xs = reinterpret_cast<T*>(ys + size_t(ys_len) * RS);
   const int np = ntiles * nslices;
   const int pidx = xcd_tile(blockIdx.x, np);   // the slices of a tile stay on one XCD
   if (pidx >= np) return;
@@ -70,102 +69,165 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   const TileRec rec = tiles[t];
   const uint32_t* nodes = node_tab + rec.node_off;
   const uint32_t n_own = rec.n_own;
-  const size_t kv = size_t(io.k / VW);        // node row length in vectors
-  // full 8x8 child-0 stencils from the upper triangles [18][72]
-  for (int i = threadIdx.x; i < int(multi_stencil_len<T, MASS>()); i += MBLOCK) {
-    const int tp = i / (NM * 64), mm = (i / 64) % NM, kj = i % 64;
-    sK[i] = K0[tp * KSTRIDE + mm * 36 + tri(kj >> 3, kj & 7)];
-  }
-  {
-    const V* X = reinterpret_cast<const V*>(io.x);
-    V* xsv = reinterpret_cast<V*>(xs);
-    for (uint32_t i = threadIdx.x; i < rec.n_touch * NV; i += MBLOCK) {
-      const uint32_t nd = i / NV, part = i % NV;
-      xsv[i] = X[size_t(nodes[nd]) * kv + size_t(sl) * NV + part];
-    }
-  }
-  for (uint32_t i = threadIdx.x; i < n_own * KS; i += MBLOCK) ys[i] = T(0);
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  constexpr int NW = MBLOCK / 64;
-  if constexpr (std::is_same<T, float>::value) {
-    // lane = 4 * block + q; block = 4 * es + 2 * ib + jb: element es of the round, row
-    // block ib (nodes 4ib..4ib+3), column block jb (right-hand sides 4jb..4jb+3).  MFMA
-    // operands: A = K[4ib + q][kk], B = X[kk][4jb + q]; result lane 4*block + q, register
-    // r = Y[4ib + r][4jb + q].
-    const int blk = lane >> 2, q = lane & 3;
-    const int es = blk >> 2, ib = (blk >> 1) & 1, jb = blk & 1;
-    for (uint32_t r0 = uint32_t(wv) * 4; r0 < rec.n_elem; r0 += NW * 4) {
-      const uint32_t r = r0 + es;
-      const bool act = r < rec.n_elem;
-      uint32_t L[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      float a[8], bv[8];
-      if (act) {
-        Rec R;
-        load_rec<T, MASS>(R, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
-        float f1, f2;
-        elem_factors<float>(R.mt >> 5, maxD, hFac, R.k1, R.k2, f1, f2);
-        const float* Lrow = sK + (R.mt & 31u) * (NM * 64) + (4 * ib + q) * 8;
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) {
-          L[kk] = R.l[kk];
-          a[kk] = MASS ? f1 * Lrow[kk] + f2 * Lrow[64 + kk] : f1 * Lrow[kk];
-          bv[kk] = xs[R.l[kk] * KS + 4 * jb + q];
-        }
-      } else {
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) a[kk] = bv[kk] = 0.f;
-      }
-      mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kk = 0; kk < 8; ++kk) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(a[kk], bv[kk], acc, 0, 0, 0);
-      if (act) {
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          const uint32_t nd = L[4 * ib + rr];
-          if (nd < n_own) atomic_add(&ys[nd * KS + 4 * jb + q], acc[rr]);
-        }
-      }
-    }
-  } else {
-    // lane = 16 * es + 8 * ib + c: element es, rows 4ib..4ib+3, right-hand side c
-    const int es = lane >> 4, ib = (lane >> 3) & 1, c = lane & 7;
-    for (uint32_t r0 = uint32_t(wv) * 4; r0 < rec.n_elem; r0 += NW * 4) {
-      const uint32_t r = r0 + es;
-      if (r >= rec.n_elem) continue;
-      Rec R;
-      load_rec<T, MASS>(R, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
-      T f1, f2;
-      elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, f1, f2);
-      const T* Ks = sK + (R.mt & 31u) * (NM * 64) + (4 * ib) * 8;
-      T xv[8];
-#pragma unroll
-      for (int kk = 0; kk < 8; ++kk) xv[kk] = xs[R.l[kk] * KS + c];
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        T acc = T(0);
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) {
-          const T kv2 = MASS ? f1 * Ks[rr * 8 + kk] + f2 * Ks[64 + rr * 8 + kk] : f1 * Ks[rr * 8 + kk];
-          acc += kv2 * xv[kk];
-        }
-        const uint32_t nd = R.l[4 * ib + rr];
-        if (nd < n_own) atomic_add(&ys[nd * KS + c], acc);
-      }
-    }
-  }
-  __syncthreads();
   const size_t k = size_t(io.k);
-  for (uint32_t i = threadIdx.x; i < n_own * KS; i += MBLOCK) {
-    const uint32_t nd = i / KS, c = i % KS;
-    const size_t gi = size_t(nodes[nd]) * k + size_t(sl) * KS + c;
-    const T v = ys[i];
+  for (uint32_t i = threadIdx.x; i < rec.n_touch * KSL; i += MB) {
+    const uint32_t nd = i / KSL, c = i % KSL;
+    xs[nd * RS + c] = io.x[size_t(nodes[nd]) * k + size_t(sl) * KSL + c];
+  }
+  for (uint32_t i = threadIdx.x; i < n_own * RS; i += MB) ys[i] = TA(0);
+  __syncthreads();
+  constexpr int EPW = 64 / KSL;               // elements per wavefront round
+  constexpr int STRIDE = (MB / 64) * EPW;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int es = lane / KSL, c = lane % KSL;
+  // the records of the next round are loaded while the current round is evaluated; the
+  // stencil of the wavefront's current type stays in scalar registers across rounds
+  Rec Rn;
+  uint32_t r0 = uint32_t(wv) * EPW;
+  bool hn = r0 + es < rec.n_elem;
+  if (hn) load_rec<T, MASS>(Rn, rec.elem_off + r0 + es, lid, meta, kappa, mass, lap_fac, mass_fac);
+  uint32_t cur = 0xffffffffu;
+  T Lr[36], Mr[36];
+  for (; r0 < rec.n_elem; r0 += STRIDE) {
+    const Rec R = Rn;
+    const bool h = hn;
+    const uint32_t rn = r0 + STRIDE + es;
+    hn = rn < rec.n_elem;
+    if (hn) load_rec<T, MASS>(Rn, rec.elem_off + rn, lid, meta, kappa, mass, lap_fac, mass_fac);
+    const uint32_t tp = R.mt & 31u;
+    bool done = !h;
+    for (;;) {
+      const uint64_t pend = __ballot(!done);
+      if (pend == 0) break;
+      const uint32_t tu = __builtin_amdgcn_readlane(tp, __builtin_ctzll(pend));
+      if (tu != cur) {
+        const T* Ls = K0 + tu * KSTRIDE;
+#pragma unroll
+        for (int q = 0; q < 36; ++q) {
+          Lr[q] = Ls[q];
+          if (MASS) Mr[q] = Ls[36 + q];
+        }
+        cur = tu;
+      }
+      if (!done && tp == tu) {
+        Rec Q = R;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Q.l[q] *= RS;
+        eval_rec_with<T, TA, MASS, MODE>(Q, n_own * RS, maxD, hFac, Lr, Mr, xs + c, ys + c);
+        done = true;
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n_own * KSL; i += MB) {
+    const uint32_t nd = i / KSL, cc = i % KSL;
+    const size_t gi = size_t(nodes[nd]) * k + size_t(sl) * KSL + cc;
+    const T v = T(ys[nd * RS + cc]);
     if (MODE == OP_MATVEC) {
       io.y[gi] = v;
     } else if (MODE == OP_RESID) {
       io.y[gi] = io.b[gi] - v;
     } else {
       const T rr = io.b[gi] - v;
+      io.y[gi] = xs[nd * RS + cc] + io.omega * (rr * io.invd[nodes[nd]]);
+    }
+  }
+}
+
+// Matrix-core variant (FP32): lane = 4 * block + q, block = 4 * es + 2 * ib + jb: element
+// es of the round, row block ib (nodes 4ib..4ib+3), column block jb (right-hand sides
+// 4jb..4jb+3).  v_mfma_f32_4x4x1_16b_f32 operands: A = K[4ib + q][kk], B = X[kk][4jb + q];
+// result lane 4 * block + q, register r = Y[4ib + r][4jb + q].  The full child-0 stencils
+// sit in LDS after the owned sums (FP64): [18][1 + MASS][64].
+template <bool MASS>
+__host__ __device__ constexpr size_t multi_stencil_len() {
+  return size_t(NSTENCIL) * (MASS ? 2 : 1) * 64;
+}
+
+template <bool MASS, int MODE, int MB>
+__global__ void __launch_bounds__(MB)
+tile_multi_mfma_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
+                       const uint16_t* __restrict__ lid, const uint16_t* __restrict__ meta,
+                       const float* __restrict__ kappa, const float* __restrict__ mass, double lap_fac,
+                       double mass_fac, uint32_t maxD, double hFac, const float* __restrict__ K0,
+                       const uint32_t* __restrict__ node_tab, uint32_t ys_len, MultiIO<float> io) {
+  static_assert(MODE != OP_DIAG, "the diagonal does not depend on the right-hand side");
+  constexpr int NM = MASS ? 2 : 1;
+  constexpr int NV = KS / 4;                  // float4 vectors per node slice
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* ys = reinterpret_cast<double*>(smem_raw);
+  float* sK = reinterpret_cast<float*>(ys + size_t(ys_len) * KS);
+  float* xs = sK + multi_stencil_len<MASS>();
+  const int np = ntiles * nslices;
+  const int pidx = xcd_tile(blockIdx.x, np);
+  if (pidx >= np) return;
+  const int t = pidx / nslices, sl = pidx % nslices;
+  const TileRec rec = tiles[t];
+  const uint32_t* nodes = node_tab + rec.node_off;
+  const uint32_t n_own = rec.n_own;
+  const size_t kv = size_t(io.k / 4);
+  for (int i = threadIdx.x; i < int(multi_stencil_len<MASS>()); i += MB) {
+    const int tp = i / (NM * 64), mm = (i / 64) % NM, kj = i % 64;
+    sK[i] = K0[tp * KSTRIDE + mm * 36 + tri(kj >> 3, kj & 7)];
+  }
+  {
+    const float4* X = reinterpret_cast<const float4*>(io.x);
+    float4* xsv = reinterpret_cast<float4*>(xs);
+    for (uint32_t i = threadIdx.x; i < rec.n_touch * NV; i += MB) {
+      const uint32_t nd = i / NV, part = i % NV;
+      xsv[i] = X[size_t(nodes[nd]) * kv + size_t(sl) * NV + part];
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < n_own * KS; i += MB) ys[i] = 0.0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int NW = MB / 64;
+  const int blk = lane >> 2, q = lane & 3;
+  const int es = blk >> 2, ib = (blk >> 1) & 1, jb = blk & 1;
+  for (uint32_t r0 = uint32_t(wv) * 4; r0 < rec.n_elem; r0 += NW * 4) {
+    const uint32_t r = r0 + es;
+    const bool act = r < rec.n_elem;
+    uint32_t L[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    float a[8], bv[8];
+    if (act) {
+      Rec R;
+      load_rec<float, MASS>(R, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
+      float f1, f2;
+      elem_factors<float>(R.mt >> 5, maxD, hFac, R.k1, R.k2, f1, f2);
+      const float* Lrow = sK + (R.mt & 31u) * (NM * 64) + (4 * ib + q) * 8;
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) {
+        L[kk] = R.l[kk];
+        a[kk] = MASS ? f1 * Lrow[kk] + f2 * Lrow[64 + kk] : f1 * Lrow[kk];
+        bv[kk] = xs[R.l[kk] * KS + 4 * jb + q];
+      }
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) a[kk] = bv[kk] = 0.f;
+    }
+    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(a[kk], bv[kk], acc, 0, 0, 0);
+    if (act) {
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const uint32_t nd = L[4 * ib + rr];
+        if (nd < n_own) atomic_add(&ys[nd * KS + 4 * jb + q], double(acc[rr]));
+      }
+    }
+  }
+  __syncthreads();
+  const size_t k = size_t(io.k);
+  for (uint32_t i = threadIdx.x; i < n_own * KS; i += MB) {
+    const uint32_t nd = i / KS, c = i % KS;
+    const size_t gi = size_t(nodes[nd]) * k + size_t(sl) * KS + c;
+    const float v = float(ys[i]);
+    if (MODE == OP_MATVEC) {
+      io.y[gi] = v;
+    } else if (MODE == OP_RESID) {
+      io.y[gi] = io.b[gi] - v;
+    } else {
+      const float rr = io.b[gi] - v;
       io.y[gi] = xs[i] + io.omega * (rr * io.invd[nodes[nd]]);
     }
   }
@@ -217,28 +279,35 @@ prolong_multi_kernel(uint32_t nf, int k, const uint32_t* __restrict__ celem, con
   f_out[gid] = (f_in ? f_in[gid] : T(0)) + acc;
 }
 
-// X = Ainv B for the dense coarsest inverse (row pitch ld) and k right-hand sides
-// ([n][k]).  A workgroup computes 32 rows; Ainv and B chunks of 64 columns pass through LDS.
-constexpr int DM_ROWS = 32, DM_CH = 64;
+// X = Ainv B for the dense coarsest inverse (row pitch ld) and k right-hand sides ([n][k]),
+// split over DM_SPLIT column ranges of Ainv (enough workgroups to stream Ainv at full
+// bandwidth): part[s][i][j] = sum over m in range s of Ainv[i][m] B[m][j] (FP64), then
+// dense_multi_sum_kernel adds the ranges in order.  A workgroup computes 32 rows x k
+// columns of one range; Ainv and B chunks of 64 columns pass through LDS.
+constexpr int DM_ROWS = 32, DM_CH = 64, DM_SPLIT = 4;
 template <class T>
-static __global__ void __launch_bounds__(256)
-dense_multi_kernel(int n, int ld, int k, const T* __restrict__ Ainv, const T* __restrict__ B, T* __restrict__ X) {
+__global__ void __launch_bounds__(256)
+dense_multi_kernel(int n, int ld, int k, const T* __restrict__ Ainv, const T* __restrict__ B,
+                   double* __restrict__ part) {
   __shared__ T sa[DM_ROWS][DM_CH + 1];
   __shared__ T sb[DM_CH][MAXK];
   const int row0 = blockIdx.x * DM_ROWS;
+  const int per = (n + DM_SPLIT - 1) / DM_SPLIT;
+  const int mlo = blockIdx.y * per, mhi = min(n, mlo + per);
   const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;   // row, column group (k / 8 columns)
   const int cpt = k / 8;
   double acc[MAXK / 8] = {0.0, 0.0, 0.0, 0.0};
-  for (int m0 = 0; m0 < n; m0 += DM_CH) {
+  for (int m0 = mlo; m0 < mhi; m0 += DM_CH) {
     for (int i = threadIdx.x; i < DM_ROWS * DM_CH; i += 256) {
       const int r = i / DM_CH, m = i % DM_CH;
-      sa[r][m] = (row0 + r < n && m0 + m < n) ? Ainv[size_t(row0 + r) * ld + m0 + m] : T(0);
+      sa[r][m] = (row0 + r < n && m0 + m < mhi) ? Ainv[size_t(row0 + r) * ld + m0 + m] : T(0);
     }
     for (int i = threadIdx.x; i < DM_CH * k; i += 256) {
       const int m = i / k, j = i % k;
-      sb[m][j] = (m0 + m < n) ? B[size_t(m0 + m) * k + j] : T(0);
+      sb[m][j] = (m0 + m < mhi) ? B[size_t(m0 + m) * k + j] : T(0);
     }
     __syncthreads();
+#pragma unroll 8
     for (int m = 0; m < DM_CH; ++m) {
       const double a = double(sa[tr][m]);
 #pragma unroll
@@ -248,27 +317,51 @@ dense_multi_kernel(int n, int ld, int k, const T* __restrict__ Ainv, const T* __
     __syncthreads();
   }
   if (row0 + tr < n)
-    for (int u = 0; u < cpt; ++u) X[size_t(row0 + tr) * k + tc * cpt + u] = T(acc[u]);
+    for (int u = 0; u < cpt; ++u)
+      part[(size_t(blockIdx.y) * n + row0 + tr) * k + tc * cpt + u] = acc[u];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+dense_multi_sum_kernel(int n, int k, const double* __restrict__ part, T* __restrict__ X) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(n) * k) return;
+  double s = 0.0;
+  for (int q = 0; q < DM_SPLIT; ++q) s += part[size_t(q) * n * k + i];
+  X[i] = T(s);
 }
 
 // ---------------------------------------------------------------- column reductions
 // out[q][j] = sum_n a_q[n][j] * b_q[n][j] (b_q == nullptr: sum of a_q), deterministic:
 // stage 1 writes part[(q * RED_BLOCKS_M + block) * k + j], stage 2 sums over blocks in order.
-constexpr int RED_BLOCKS_M = 256;
+constexpr int RED_BLOCKS_M = 1024;
 template <class T>
-static __global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256)
 reduce1_multi_kernel(uint32_t n, int k, DotArgs<T> d, double* __restrict__ part) {
   const int j = threadIdx.x % k, rs = threadIdx.x / k, nrs = 256 / k;
   double s[MAXQ] = {0.0, 0.0, 0.0, 0.0};
   const uint32_t per = (n + RED_BLOCKS_M - 1) / RED_BLOCKS_M;
   const uint32_t lo = blockIdx.x * per, hi = min(n, lo + per);
-  if (rs < nrs)
-    for (uint32_t i = lo + rs; i < hi; i += nrs) {
+  if (rs < nrs) {
+    uint32_t i = lo + rs;
+    for (; i + uint32_t(nrs) < hi; i += 2 * nrs) {       // two rows in flight per thread
+      const size_t g0 = size_t(i) * k + j, g1 = size_t(i + nrs) * k + j;
+#pragma unroll
+      for (int q = 0; q < MAXQ; ++q)
+        if (d.a[q]) {
+          const double v0 = double(d.a[q][g0]) * (d.b[q] ? double(d.b[q][g0]) : 1.0);
+          const double v1 = double(d.a[q][g1]) * (d.b[q] ? double(d.b[q][g1]) : 1.0);
+          s[q] += v0;
+          s[q] += v1;
+        }
+    }
+    for (; i < hi; i += nrs) {
       const size_t g = size_t(i) * k + j;
 #pragma unroll
       for (int q = 0; q < MAXQ; ++q)
         if (d.a[q]) s[q] += double(d.a[q][g]) * (d.b[q] ? double(d.b[q][g]) : 1.0);
     }
+  }
   __shared__ double red[MAXQ][256];
 #pragma unroll
   for (int q = 0; q < MAXQ; ++q) red[q][threadIdx.x] = s[q];
@@ -283,14 +376,18 @@ reduce1_multi_kernel(uint32_t n, int k, DotArgs<T> d, double* __restrict__ part)
   }
 }
 
-// out[q * ostride + j] for q < nq, j < k
+// out[q * ostride + j]: one workgroup per (q, j), partials summed in a fixed tree order
 static __global__ void __launch_bounds__(256)
 reduce2_multi_kernel(int nq, int k, const double* __restrict__ part, double* __restrict__ out, int ostride) {
-  const int q = threadIdx.x / MAXK, j = threadIdx.x % MAXK;
-  if (q >= nq || j >= k) return;
+  const int q = blockIdx.x / k, j = blockIdx.x % k;
+  if (q >= nq) return;
   double t = 0.0;
-  for (int b = 0; b < RED_BLOCKS_M; ++b) t += part[(size_t(q) * RED_BLOCKS_M + b) * k + j];
-  out[q * ostride + j] = t;
+  for (int b = threadIdx.x; b < RED_BLOCKS_M; b += 256) t += part[(size_t(q) * RED_BLOCKS_M + b) * k + j];
+  t = wave_sum(t);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) out[q * ostride + j] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // v[n][j] -= sum[j] / n

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "internal.hpp"
@@ -19,12 +20,24 @@ template <> const double* dm<double>(const DevTiles& d) { return d.rm64.p; }
 template <> const float* dk<float>(const DevTiles& d) { return d.rk32.p; }
 template <> const float* dm<float>(const DevTiles& d) { return d.rm32.p; }
 
-template <class T>
-static size_t multi_lds(const DevTiles& d, bool mass) {
-  return sizeof(T) * (size_t(d.ys_len) * KS + size_t(NSTENCIL) * (mass ? 2 : 1) * 64 + size_t(d.plan.max_touch) * KS);
+// LDS bytes of one batched workgroup: FP64 sums of the owned nodes and the staged x of the
+// touched nodes, rows of a slice padded by one value (vector-ALU kernel: FP32 slices of 8
+// columns, FP64 slices of 4); the matrix-core variant adds the full stencil table.
+template <class T, int KSL>
+static size_t multi_lds_valu(const DevTiles& d) {
+  return size_t(multi_row<T>(KSL)) * (sizeof(double) * d.ys_len + sizeof(T) * d.plan.max_touch);
 }
-// one tiling serves both precisions: sized for the larger (FP64, mass present) footprint
-static size_t multi_lds_worst(const DevTiles& d) { return multi_lds<double>(d, true); }
+static size_t multi_lds_mfma(const DevTiles& d, bool mass) {
+  return sizeof(double) * KS * d.ys_len + sizeof(float) * KS * d.plan.max_touch +
+         sizeof(float) * multi_stencil_len<true>() / (mass ? 1 : 2);
+}
+static size_t multi_lds_worst(const DevTiles& d) {
+  return std::max({multi_lds_valu<float, KS>(d), multi_lds_valu<double, KS64>(d), multi_lds_mfma(d, true)});
+}
+
+// FP32 product variants: 0 vector ALUs (8 columns per slice, 512 threads), 1 matrix cores,
+// 2 vector ALUs with 1024 threads, 3 vector ALUs with 4 columns per slice (half the LDS)
+constexpr int NVARIANT = 4;
 
 static void plan_multi(dgh_mesh m, uint32_t T) {
   build_devtiles(m, m->mop, T, multi_lds_worst);
@@ -32,33 +45,60 @@ static void plan_multi(dgh_mesh m, uint32_t T) {
 }
 
 void ensure_multi(dgh_mesh m) {
-  if (m->mop.plan.tiles.empty()) plan_multi(m, m->mtile ? m->mtile : 512);
+  if (m->mop.plan.tiles.empty()) plan_multi(m, m->mtile ? m->mtile : 1024);
 }
 
 static void check_k(int k) {
   if (k < KS || k > MAXK || k % KS) fail(DGH_ERR_ARG, "k = %d right-hand sides: must be a multiple of %d in [%d, %d]", k, KS, KS, MAXK);
 }
 
+template <class T, bool MASS, int MODE, int KSL, int MB>
+static void launch_valu(dgh_mesh m, const DevTiles& d, const T* kap, const T* mas, const MultiIO<T>& io,
+                        hipStream_t st) {
+  const int nt = int(d.plan.tiles.size());
+  const int np = nt * (io.k / KSL);
+  const unsigned grid = unsigned(NXCD * ((np + NXCD - 1) / NXCD));
+  const size_t lds = multi_lds_valu<T, KSL>(d);
+  hipLaunchKernelGGL((tile_multi_kernel<T, MASS, MODE, KSL, MB>), dim3(grid), dim3(MB), lds, st,
+                     d.tiles.p, nt, io.k / KSL, d.lid.p, d.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
+                     m->hFac, Ktab<T>(), d.node_tab.p, d.ys_len, io);
+}
+
+template <bool MASS, int MODE>
+static void launch_mfma(dgh_mesh m, const DevTiles& d, const float* kap, const float* mas,
+                        const MultiIO<float>& io, hipStream_t st) {
+  const int nt = int(d.plan.tiles.size());
+  const int np = nt * (io.k / KS);
+  const unsigned grid = unsigned(NXCD * ((np + NXCD - 1) / NXCD));
+  hipLaunchKernelGGL((tile_multi_mfma_kernel<MASS, MODE, MBLOCK>), dim3(grid), dim3(MBLOCK), multi_lds_mfma(d, MASS), st,
+                     d.tiles.p, nt, io.k / KS, d.lid.p, d.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
+                     m->hFac, Ktab<float>(), d.node_tab.p, d.ys_len, io);
+}
+
 template <class T, int MODE>
 static void launch_multi(dgh_mesh m, const MultiIO<T>& io, hipStream_t st) {
   ensure_multi(m);
   const DevTiles& d = m->mop;
-  const int nt = int(d.plan.tiles.size());
-  if (!nt) return;
-  const int np = nt * (io.k / KS);
-  const unsigned grid = unsigned(NXCD * ((np + NXCD - 1) / NXCD));
+  if (d.plan.tiles.empty()) return;
   const T* kap = dk<T>(d);
   const T* mas = dm<T>(d);
   const bool mass = !(kap && !mas);
-  const size_t lds = multi_lds<T>(d, mass);
-  if (!mass)
-    hipLaunchKernelGGL((tile_multi_kernel<T, false, MODE>), dim3(grid), dim3(MBLOCK), lds, st, d.tiles.p, nt,
-                       io.k / KS, d.lid.p, d.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac,
-                       Ktab<T>(), d.node_tab.p, d.ys_len, io);
-  else
-    hipLaunchKernelGGL((tile_multi_kernel<T, true, MODE>), dim3(grid), dim3(MBLOCK), lds, st, d.tiles.p, nt,
-                       io.k / KS, d.lid.p, d.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac,
-                       Ktab<T>(), d.node_tab.p, d.ys_len, io);
+  if constexpr (std::is_same<T, double>::value) {
+    if (mass) launch_valu<double, true, MODE, KS64, MBLOCK>(m, d, kap, mas, io, st);
+    else launch_valu<double, false, MODE, KS64, MBLOCK>(m, d, kap, mas, io, st);
+  } else if (m->mvariant == 1) {
+    if (mass) launch_mfma<true, MODE>(m, d, kap, mas, io, st);
+    else launch_mfma<false, MODE>(m, d, kap, mas, io, st);
+  } else if (m->mvariant == 2) {
+    if (mass) launch_valu<float, true, MODE, KS, 1024>(m, d, kap, mas, io, st);
+    else launch_valu<float, false, MODE, KS, 1024>(m, d, kap, mas, io, st);
+  } else if (m->mvariant == 3) {
+    if (mass) launch_valu<float, true, MODE, 4, MBLOCK>(m, d, kap, mas, io, st);
+    else launch_valu<float, false, MODE, 4, MBLOCK>(m, d, kap, mas, io, st);
+  } else {
+    if (mass) launch_valu<float, true, MODE, KS, MBLOCK>(m, d, kap, mas, io, st);
+    else launch_valu<float, false, MODE, KS, MBLOCK>(m, d, kap, mas, io, st);
+  }
   HIP_CHECK(hipGetLastError());
 }
 
@@ -82,7 +122,7 @@ template <class T>
 static void reduce_multi(uint32_t n, int k, const DotArgs<T>& d, int nq, double* part, double* out,
                          hipStream_t st) {
   hipLaunchKernelGGL(reduce1_multi_kernel<T>, dim3(RED_BLOCKS_M), dim3(256), 0, st, n, k, d, part);
-  hipLaunchKernelGGL(reduce2_multi_kernel, dim3(1), dim3(MAXQ * MAXK), 0, st, nq, k, (const double*)part, out, MAXK);
+  hipLaunchKernelGGL(reduce2_multi_kernel, dim3(nq * k), dim3(256), 0, st, nq, k, (const double*)part, out, MAXK);
 }
 
 // v[:, j] -= mean(v[:, j]) for every column
@@ -107,8 +147,10 @@ float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st) {
       project_mean_multi<float>(s, uint32_t(n), k, V.r.p, st);
       bb = V.r.p;
     }
-    hipLaunchKernelGGL(dense_multi_kernel<float>, dim3((n + DM_ROWS - 1) / DM_ROWS), dim3(256), 0, st, n, s->ldc,
-                       k, (const float*)s->Ainv32.p, bb, V.xa.p);
+    hipLaunchKernelGGL(dense_multi_kernel<float>, dim3((n + DM_ROWS - 1) / DM_ROWS, DM_SPLIT), dim3(256), 0, st, n,
+                       s->ldc, k, (const float*)s->Ainv32.p, bb, s->mdense.p);
+    hipLaunchKernelGGL(dense_multi_sum_kernel<float>, dim3(blocks_for(size_t(n) * k)), dim3(256), 0, st, n, k,
+                       (const double*)s->mdense.p, V.xa.p);
     HIP_CHECK(hipGetLastError());
     return V.xa.p;
   }
@@ -249,6 +291,7 @@ void ensure_multi_buffers(dgh_solver s, int k) {
   s->mz.alloc(nk);
   s->mp.alloc(nk);
   s->mw.alloc(nk);
+  s->mdense.alloc(size_t(DM_SPLIT) * s->nc * k);
   if (!s->mS.p) {
     s->mS.alloc(size_t(SM_NSCAL) * MAXK);
     s->mpart.alloc(size_t(MAXQ) * RED_BLOCKS_M * MAXK);
@@ -281,15 +324,16 @@ int dgh_matvec_multi(dgh_mesh m, int prec, int k, const void* x, void* y, void* 
   });
 }
 
-int dgh_mesh_set_tile_multi(dgh_mesh m, uint32_t tile_elems) {
+int dgh_mesh_set_tile_multi(dgh_mesh m, uint32_t tile_elems, int variant) {
   return guarded([&] {
     require_init();
-    if (!m || tile_elems < 32) fail(DGH_ERR_ARG, "invalid arguments");
+    if (!m || tile_elems < 32 || variant < 0 || variant >= NVARIANT) fail(DGH_ERR_ARG, "invalid arguments");
     plan_multi(m, tile_elems);
+    m->mvariant = variant;
   });
 }
 
-int dgh_mesh_autotune_multi(dgh_mesh m, int k, uint32_t* tile_elems, float* us) {
+int dgh_mesh_autotune_multi(dgh_mesh m, int k, uint32_t* tile_elems, int* variant, float* us) {
   return guarded([&] {
     require_init();
     if (!m) fail(DGH_ERR_ARG, "NULL mesh");
@@ -302,31 +346,38 @@ int dgh_mesh_autotune_multi(dgh_mesh m, int k, uint32_t* tile_elems, float* us) 
     HIP_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
     uint32_t bt = 0;
-    for (uint32_t T : {128u, 256u, 512u, 1024u}) {
+    int bv = 0;
+    for (uint32_t T : {256u, 512u, 1024u, 2048u}) {
       try {
         plan_multi(m, T);
       } catch (const Error&) {
         continue;
       }
       if (m->mop.plan.tile_elems != T) continue;
-      auto run = [&] { launch_multi<float, OP_MATVEC>(m, MultiIO<float>{xb.p, yb.p, nullptr, nullptr, 0.f, k}, nullptr); };
-      for (int i = 0; i < 2; ++i) run();
-      HIP_CHECK(hipEventRecord(e0, nullptr));
-      for (int i = 0; i < 5; ++i) run();
-      HIP_CHECK(hipEventRecord(e1, nullptr));
-      HIP_CHECK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        bt = T;
+      for (int variant = 0; variant < NVARIANT; ++variant) {
+        m->mvariant = variant;
+        auto run = [&] { launch_multi<float, OP_MATVEC>(m, MultiIO<float>{xb.p, yb.p, nullptr, nullptr, 0.f, k}, nullptr); };
+        for (int i = 0; i < 2; ++i) run();
+        HIP_CHECK(hipEventRecord(e0, nullptr));
+        for (int i = 0; i < 5; ++i) run();
+        HIP_CHECK(hipEventRecord(e1, nullptr));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          bt = T;
+          bv = variant;
+        }
       }
     }
     HIP_CHECK(hipEventDestroy(e0));
     HIP_CHECK(hipEventDestroy(e1));
     if (!bt) fail(DGH_ERR_STATE, "no batched tiling candidate fits");
     plan_multi(m, bt);
+    m->mvariant = bv;
     if (tile_elems) *tile_elems = bt;
+    if (variant) *variant = bv;
     if (us) *us = best * 200.f;   // ms per 5 launches -> us per launch
   });
 }

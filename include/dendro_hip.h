@@ -196,13 +196,15 @@ int dgh_diagonal(dgh_mesh m, int prec, void* d_dev, void* stream);
  * per call; each column here equals that product.  FP32 runs the element products on the
  * matrix cores. */
 int dgh_matvec_multi(dgh_mesh m, int prec, int k, const void* x_dev, void* y_dev, void* stream);
-/* Tile size (elements per tile) of the batched operator; rounded down until its LDS fits.
+/* Tile size (elements per tile) of the batched operator, rounded down until its LDS fits,
+ * and the FP32 product variant (0: vector ALUs, 8 columns per workgroup slice, 512
+ * threads; 1: matrix cores; 2: as 0 with 1024 threads; 3: as 0 with 4 columns per slice).
  * No reference counterpart (tuning hook). */
-int dgh_mesh_set_tile_multi(dgh_mesh m, uint32_t tile_elems);
-/* Times the FP32 batched operator for k columns over the candidate tile sizes and keeps the
- * fastest; outputs (may be NULL) the tile size and its time per launch.  No reference
- * counterpart. */
-int dgh_mesh_autotune_multi(dgh_mesh m, int k, uint32_t* tile_elems, float* us);
+int dgh_mesh_set_tile_multi(dgh_mesh m, uint32_t tile_elems, int variant);
+/* Times the FP32 batched operator for k columns over the candidate tile sizes and all
+ * product variants and keeps the fastest; outputs (may be NULL) the tile size, the variant
+ * and its time per launch.  No reference counterpart. */
+int dgh_mesh_autotune_multi(dgh_mesh m, int k, uint32_t* tile_elems, int* variant, float* us);
 
 /* ================================================================ device: intergrid transfer */
 typedef struct dgh_transfer_s* dgh_transfer;

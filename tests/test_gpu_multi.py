@@ -40,14 +40,15 @@ def _meshes(h, coefs):
     return ms, ts
 
 
-@pytest.mark.parametrize("k,tile,mass", [(8, 256, True), (16, 512, False), (32, 128, True)])
-def test_matvec_multi_matches_oracle(dev, hier_small, k, tile, mass):
+@pytest.mark.parametrize("k,tile,mass,variant", [(8, 256, True, 0), (16, 512, False, 0), (32, 1024, True, 0),
+                                                  (8, 512, True, 1), (32, 1024, False, 1)])
+def test_matvec_multi_matches_oracle(dev, hier_small, k, tile, mass, variant):
     h = hier_small
     F = h.fine
     c = var_coefficients(h, mass_base=1.0 if mass else None)
     m = da.Mesh(F)
     m.set_coefficients(c[-1][0], c[-1][1])
-    m.set_tile_multi(tile)
+    m.set_tile_multi(tile, variant)
     X = np.random.default_rng(100 + k).uniform(-1, 1, (F.n_node, k))
     Xt = torch.tensor(X, dtype=torch.float64, device=dev)
     Yt = torch.empty_like(Xt)
@@ -62,7 +63,7 @@ def test_matvec_multi_matches_oracle(dev, hier_small, k, tile, mass):
         ref = oracle.matvec(F, X[:, j], kappa=c[-1][0], mass=c[-1][1])
         assert rel_err(Y[:, j], ref) <= 1e-12, j
         worst32 = max(worst32, rel_err(Y32[:, j], ref))
-    print(f"k={k} tile={tile}: FP32 batched MatVec relative error vs FP64 oracle {worst32:.3e}")
+    print(f"k={k} tile={tile} variant={variant}: FP32 batched MatVec relative error vs FP64 oracle {worst32:.3e}")
     assert worst32 <= 1e-5
 
 
@@ -79,8 +80,8 @@ def test_solve_multi_matches_single(dev, hier_small, graph):
     h = hier_small
     coefs = var_coefficients(h, mass_base=None)
     ms, ts = _meshes(h, coefs)
-    for m in ms[1:]:
-        m.set_tile_multi(256)
+    for i, m in enumerate(ms[1:]):
+        m.set_tile_multi(512, i % 2)
     p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=200, rtol=1e-8,
                           norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=da.F32,
                           use_graph=graph)
