@@ -151,8 +151,8 @@ constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 // K0: [18][72] child-0 Laplacian then mass upper triangles of precision T.
 // kappa/mass: per record (nullptr: uniform lap_fac / mass_fac).  MASS: mass term present.
 // Row sums accumulate in FP64 LDS for both precisions (TA = double).
-template <class T, class TC, bool MASS, int MODE, int BLOCK, int RCH = RCHUNK, int XCH = XCHUNK>
-__global__ void __launch_bounds__(BLOCK)
+template <class T, class TC, bool MASS, int MODE, int BLOCK, int RCH = RCHUNK, int XCH = XCHUNK, int MINW = 1>
+__global__ void __launch_bounds__(BLOCK, MINW)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
                const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
                const TC* __restrict__ mass, double lap_fac, double mass_fac, uint32_t maxD,
