@@ -422,7 +422,8 @@ int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* bloc
     HIP_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
     uint32_t bt = 0, bb = 0;
-    for (uint32_t T : {1024u, 2048u, 4096u}) {
+    // small levels need small tiles: enough workgroups to fill the device
+    for (uint32_t T : {128u, 256u, 512u, 1024u, 2048u, 4096u}) {
       for (int B : {256, 512, 1024}) {
         m->block = B;
         try {

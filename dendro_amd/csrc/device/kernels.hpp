@@ -226,7 +226,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
       io.y[n] = io.b[n] - v;
     } else {
       const T r = io.b[n] - v;
-      io.y[n] = x[n] + io.omega * (r * io.invd[n]);
+      io.y[n] = xs[i] + io.omega * (r * io.invd[n]);   // owned nodes lead the staged x
     }
   }
 }
