@@ -91,20 +91,33 @@ __device__ __forceinline__ void eval_rec_with(const Rec& R, uint32_t n_own, uint
     T xv[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) xv[j] = xs[R.l[j]];
+    // one pass over the 36 triangle entries: each stencil value is read once and feeds
+    // both rows it belongs to; the element factors scale the row sums at the end
+    T a[8], bm[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      if (R.l[k] < n_own) {
-        T acc = T(0);
+      a[k] = T(0);
+      bm[k] = T(0);
+    }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          if (MASS)
-            acc += (fac1 * Ls[tri(k, j)] + fac2 * Ms[tri(k, j)]) * xv[j];
-          else
-            acc += (fac1 * Ls[tri(k, j)]) * xv[j];
+    for (int k = 0; k < 8; ++k) {
+      a[k] += Ls[tri(k, k)] * xv[k];
+      if (MASS) bm[k] += Ms[tri(k, k)] * xv[k];
+#pragma unroll
+      for (int j = k + 1; j < 8; ++j) {
+        const T l = Ls[tri(k, j)];
+        a[k] += l * xv[j];
+        a[j] += l * xv[k];
+        if (MASS) {
+          const T m = Ms[tri(k, j)];
+          bm[k] += m * xv[j];
+          bm[j] += m * xv[k];
         }
-        atomic_add(&ys[R.l[k]], TA(acc));
       }
     }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (R.l[k] < n_own) atomic_add(&ys[R.l[k]], TA(MASS ? fac1 * a[k] + fac2 * bm[k] : fac1 * a[k]));
   } else {
 #pragma unroll
     for (int k = 0; k < 8; ++k)
@@ -145,7 +158,7 @@ __device__ __forceinline__ void eval_rec(const Rec& R, uint32_t n_own, uint32_t 
 // Keep a copy of the 18 stencil triangles in LDS for wavefronts with mixed stencil types
 // (false: those few wavefronts read the global table through L1).
 constexpr bool LDS_STENCIL = false;
-constexpr int RCHUNK = 2;   // records per thread whose loads are in flight together
+constexpr int RCHUNK = 1;   // records per thread whose loads are in flight together
 constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 
 // K0: [18][72] child-0 Laplacian then mass upper triangles of precision T.
