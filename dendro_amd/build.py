@@ -16,6 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libdendrohip.so")
+BINDIR = os.path.join(HERE, "bin")
+EXAMPLES = ["examples/tst_mg.cpp"]
 
 HOST_SOURCES = ["host/stencils.cpp", "host/octree.cpp", "host/da.cpp", "host/tiles.cpp",
                 "host/host_api.cpp"]
@@ -53,6 +55,13 @@ def build(verbose=True):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs +
          ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-pthread", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
+    # C++ drivers over the header-only facade (include/dendro_hip.hpp), linked to the library
+    os.makedirs(BINDIR, exist_ok=True)
+    for src in EXAMPLES:
+        exe = os.path.join(BINDIR, os.path.splitext(os.path.basename(src))[0])
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+              os.path.join(CSRC, src), "-o", exe, "-L" + LIBDIR, "-ldendrohip",
+              "-Wl,-rpath,$ORIGIN/../lib"])
     return LIB
 
 

@@ -26,7 +26,8 @@ EXPORTS = [
     "dgh_damg_build", "dgh_damg_num_levels", "dgh_damg_level_info", "dgh_damg_level_arrays",
     "dgh_damg_level_octants", "dgh_damg_transfer_arrays", "dgh_damg_coarsen_coefficients",
     "dgh_damg_free",
-    "dgh_init", "dgh_finalize",
+    "dgh_init", "dgh_finalize", "dgh_alloc", "dgh_free", "dgh_copy_to_device", "dgh_copy_to_host",
+    "dgh_synchronize",
     "dgh_mesh_create", "dgh_mesh_set_coefficients", "dgh_mesh_info", "dgh_mesh_destroy",
     "dgh_mesh_set_tile", "dgh_mesh_tile_info", "dgh_mesh_autotune",
     "dgh_matvec", "dgh_diagonal", "dgh_matvec_multi", "dgh_mesh_set_tile_multi",

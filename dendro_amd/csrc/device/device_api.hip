@@ -343,6 +343,48 @@ int dgh_init(int device) {
   });
 }
 
+int dgh_alloc(size_t bytes, void** ptr) {
+  return guarded([&] {
+    require_init();
+    if (!ptr) fail(DGH_ERR_ARG, "NULL argument");
+    *ptr = nullptr;
+    if (bytes && hipMalloc(ptr, bytes) != hipSuccess) {
+      *ptr = nullptr;
+      fail(DGH_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+  });
+}
+
+int dgh_free(void* ptr) {
+  return guarded([&] {
+    require_init();
+    if (ptr) HIP_CHECK(hipFree(ptr));
+  });
+}
+
+int dgh_copy_to_device(void* dst_dev, const void* src_host, size_t bytes) {
+  return guarded([&] {
+    require_init();
+    if (bytes && (!dst_dev || !src_host)) fail(DGH_ERR_ARG, "NULL argument");
+    if (bytes) HIP_CHECK(hipMemcpy(dst_dev, src_host, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int dgh_copy_to_host(void* dst_host, const void* src_dev, size_t bytes) {
+  return guarded([&] {
+    require_init();
+    if (bytes && (!dst_host || !src_dev)) fail(DGH_ERR_ARG, "NULL argument");
+    if (bytes) HIP_CHECK(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int dgh_synchronize(void) {
+  return guarded([&] {
+    require_init();
+    HIP_CHECK(hipDeviceSynchronize());
+  });
+}
+
 int dgh_finalize(void) {
   return guarded([&] {
     g.K64.release();

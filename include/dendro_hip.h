@@ -150,6 +150,14 @@ int dgh_damg_free(dgh_damg h);
  * Plays the role of DAMG_Initialize / DA_Initialize (src/omg/omg.C:66, src/oda/odaUtils.C:1742). */
 int dgh_init(int device);
 int dgh_finalize(void);
+/* Device memory and copies for callers without a HIP runtime of their own (the reference
+ * keeps its vectors in PETSc Vecs on the host; this is where they cross to the device).
+ * No reference counterpart.  Copies are synchronous; dgh_synchronize waits for all work. */
+int dgh_alloc(size_t bytes, void** ptr);
+int dgh_free(void* ptr);
+int dgh_copy_to_device(void* dst_dev, const void* src_host, size_t bytes);
+int dgh_copy_to_host(void* dst_host, const void* src_dev, size_t bytes);
+int dgh_synchronize(void);
 
 /* ================================================================ device: one mesh level */
 typedef struct dgh_mesh_s* dgh_mesh;
