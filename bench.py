@@ -302,7 +302,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg_bytes,
-                     "kernel": f"dgh::tile_op_kernel<double, double, false, 0, {tuned['block']}, 1, 4, 1>",
+                     "kernel": f"dgh::tile_op_kernel<double, double, false, 0, {tuned['block']}>",
                      "tile_elems": mf.tile_info()["tile_elems"], "block": tuned["block"]},
         "matvec_fp32_ms": round(ms32, 5),
         "matvec_fp32_gdofs": round(N / (ms32 * 1e-3) / 1e9, 4),

@@ -21,8 +21,7 @@ namespace dgh {
 
 template <class T>
 static size_t op_lds_bytes(const dgh_mesh m) {
-  return sizeof(double) * m->op.ys_len +
-         sizeof(T) * ((LDS_STENCIL ? size_t(NSTENCIL) * KSTRIDE : 0) + m->op.plan.max_touch);
+  return sizeof(T) == 8 ? m->op.lds64 : m->op.lds32;
 }
 
 template <class T> const T* rk(dgh_mesh m);
@@ -41,12 +40,12 @@ static void launch_op_b(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
   const T* mas = rm<T>(m);
   if (kap && !mas)
     hipLaunchKernelGGL((tile_op_kernel<T, T, false, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->op.tiles.p,
-                       nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
-                       m->op.node_tab.p, m->op.ys_len, io);
+                       nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, KFtab<T>(),
+                       m->op.node_tab.p, m->op.texp.p, io);
   else
     hipLaunchKernelGGL((tile_op_kernel<T, T, true, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->op.tiles.p,
-                       nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, Ktab<T>(),
-                       m->op.node_tab.p, m->op.ys_len, io);
+                       nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, KFtab<T>(),
+                       m->op.node_tab.p, m->op.texp.p, io);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -100,14 +99,38 @@ static void launch_prolong(dgh_transfer t, const T* c, const T* f_in, T* f_out, 
   HIP_CHECK(hipGetLastError());
 }
 
-// Per-record coefficient copies of a tiling (they follow its record order).
+// Per-record coefficient copies of a tiling (they follow its record order), and the per-tile
+// exponent bounds of the fixed-point sums: texp[t] = e with
+//   |fac1| max_k sum_j |L_t[k][j]| + |fac2| max_k sum_j |M_t[k][j]| < 2^e
+// over the tile's records, factors formed as the kernel forms them (elem_factors).
 void upload_coefs(dgh_mesh m, DevTiles& d) {
   d.rk64.release();
   d.rm64.release();
   d.rk32.release();
   d.rm32.release();
   const size_t R = d.plan.rec_elem.size();
-  if (m->h_kappa.empty() || R == 0) return;
+  if (R == 0) return;
+  const bool per_elem = !m->h_kappa.empty();
+  const bool mass = per_elem ? !m->h_mass.empty() : true;
+  std::vector<int> te(d.plan.tiles.size());
+  for (size_t t = 0; t < d.plan.tiles.size(); ++t) {
+    const TileRec& tr = d.plan.tiles[t];
+    double mx = 0.0;
+    for (uint32_t r = tr.elem_off; r < tr.elem_off + tr.n_elem; ++r) {
+      const uint32_t e = d.plan.rec_elem[r];
+      const double h = m->hFac * double(1u << (m->maxD - m->h_lev[e]));
+      const double k1 = per_elem ? m->h_kappa[e] : m->lap_fac;
+      const double k2 = per_elem ? (mass ? m->h_mass[e] : 0.0) : m->mass_fac;
+      const int tp = d.plan.meta[r] & 31;
+      mx = std::max(mx, std::fabs(k1 * h / 2.0) * g.lrow[tp] + (mass ? std::fabs(k2 * h * h * h / 8.0) * g.mrow[tp] : 0.0));
+    }
+    if (!std::isfinite(mx)) fail(DGH_ERR_ARG, "non-finite element coefficients");
+    int e = -1022;
+    if (mx > 0.0) std::frexp(mx, &e);   // mx < 2^e
+    te[t] = std::max(e, -1022);
+  }
+  d.texp.upload(te.data(), te.size());
+  if (!per_elem) return;
   std::vector<double> k(R);
   std::vector<float> kf(R);
   for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = m->h_kappa[d.plan.rec_elem[r]]);
@@ -120,11 +143,17 @@ void upload_coefs(dgh_mesh m, DevTiles& d) {
   }
 }
 
-// (Re)builds a tiling with the largest tile size <= T whose LDS need (lds(d)) fits.
-void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&)) {
+// (Re)builds a tiling with the largest tile size <= T whose LDS need (lds(d)) fits; with
+// budget > 0 tiles also end before their owned plus touched nodes exceed the budget.
+void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&), uint32_t budget) {
   for (;;) {
-    build_tiles(m->E, m->N, m->h_nl.data(), m->h_child.data(), m->h_hn.data(), m->h_lev.data(), T, d.plan);
+    build_tiles(m->E, m->N, m->h_nl.data(), m->h_child.data(), m->h_hn.data(), m->h_lev.data(), T, d.plan, budget);
     d.ys_len = (d.plan.max_own + 1) & ~1u;
+    d.lds64 = d.lds32 = 0;
+    for (const TileRec& tr : d.plan.tiles) {
+      d.lds64 = std::max(d.lds64, 8 * (tr.n_own + tr.n_touch));
+      d.lds32 = std::max(d.lds32, 8 * tr.n_own + 4 * tr.n_touch);
+    }
     if (lds(d) <= size_t(g.max_lds) || T <= 32) break;
     T /= 2;
   }
@@ -137,12 +166,19 @@ void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const Dev
   upload_coefs(m, d);
 }
 
-static size_t op_lds_f64(const DevTiles& d) {
-  return sizeof(double) * (d.ys_len + (LDS_STENCIL ? size_t(NSTENCIL) * KSTRIDE : 0) + d.plan.max_touch);
+static size_t op_lds_f64(const DevTiles& d) { return d.lds64; }
+
+// Node budget of a single right-hand-side tile for a block size: the LDS share that lets the
+// wave limit (32 wavefronts per CU) rather than LDS bound the workgroups per CU, less a
+// 512-byte allowance for static LDS and allocation granularity.
+static uint32_t op_budget(int block) {
+  const int per_cu = std::max(1, 2048 / block);
+  return uint32_t((OP_LDS_PER_CU / per_cu - 512) / 8);
 }
 
-// (Re)builds the single right-hand-side tiling with the largest tile size <= T that fits.
-static void plan_mesh(dgh_mesh m, uint32_t T) { build_devtiles(m, m->op, T, op_lds_f64); }
+// (Re)builds the single right-hand-side tiling with tiles of at most T elements within the
+// block size's node budget.
+static void plan_mesh(dgh_mesh m, uint32_t T) { build_devtiles(m, m->op, T, op_lds_f64, op_budget(m->block)); }
 
 }  // namespace dgh
 
@@ -339,6 +375,24 @@ int dgh_init(int device) {
     std::vector<float> Kf(K.begin(), K.end()), Wf(W.begin(), W.end());
     g.K32.upload(Kf.data(), Kf.size());
     g.W32.upload(Wf.data(), Wf.size());
+    std::vector<double> KF(size_t(NSTENCIL) * KFSTRIDE);
+    for (int t = 0; t < NSTENCIL; ++t) {   // child-0 full rows: [t][L rows, M rows]
+      g.lrow[t] = g.mrow[t] = 0.0;
+      for (int k = 0; k < 8; ++k) {
+        double ls = 0.0, ms = 0.0;
+        for (int j = 0; j < 8; ++j) {
+          KF[size_t(t) * KFSTRIDE + k * 8 + j] = L[size_t(t) * 64 + k * 8 + j];
+          KF[size_t(t) * KFSTRIDE + 64 + k * 8 + j] = M[size_t(t) * 64 + k * 8 + j];
+          ls += std::fabs(L[size_t(t) * 64 + k * 8 + j]);
+          ms += std::fabs(M[size_t(t) * 64 + k * 8 + j]);
+        }
+        g.lrow[t] = std::max(g.lrow[t], ls);
+        g.mrow[t] = std::max(g.mrow[t], ms);
+      }
+    }
+    g.KF64.upload(KF.data(), KF.size());
+    std::vector<float> KFf(KF.begin(), KF.end());
+    g.KF32.upload(KFf.data(), KFf.size());
     g.ready = true;
   });
 }
@@ -391,6 +445,8 @@ int dgh_finalize(void) {
     g.W64.release();
     g.K32.release();
     g.W32.release();
+    g.KF64.release();
+    g.KF32.release();
     if (g.blas) rocblas_destroy_handle(g.blas);
     g.blas = nullptr;
     g.ready = false;

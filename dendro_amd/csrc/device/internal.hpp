@@ -62,12 +62,17 @@ struct DevBuf {
 static inline unsigned blocks_for(size_t n, int bs = VBLOCK) { return unsigned((n + bs - 1) / bs); }
 
 // ---------------------------------------------------------------- library state
+constexpr int OP_LDS_PER_CU = 160 * 1024;   // LDS of one CU (gfx950)
+
 struct State {
   bool ready = false;
   int device = 0;
   int max_lds = 65536;            // dynamic LDS bytes a workgroup may use
   DevBuf<double> K64, W64;        // child-0 stencil triangles [18][72]; transfer weights [144][27][8]
   DevBuf<float> K32, W32;
+  DevBuf<double> KF64;            // child-0 full stencil rows [18][128] (Laplacian, mass)
+  DevBuf<float> KF32;
+  double lrow[NSTENCIL] = {}, mrow[NSTENCIL] = {};   // max row sums of |L|, |M| per type
   rocblas_handle blas = nullptr;
 };
 extern State g;
@@ -82,6 +87,9 @@ template <> inline const double* Ktab<double>() { return g.K64.p; }
 template <> inline const double* Wtab<double>() { return g.W64.p; }
 template <> inline const float* Ktab<float>() { return g.K32.p; }
 template <> inline const float* Wtab<float>() { return g.W32.p; }
+template <class T> const T* KFtab();
+template <> inline const double* KFtab<double>() { return g.KF64.p; }
+template <> inline const float* KFtab<float>() { return g.KF32.p; }
 
 }  // namespace dgh
 
@@ -90,12 +98,14 @@ namespace dgh {
 // One owner-computes tiling of a mesh (tiles.cpp) and its device copies.
 struct DevTiles {
   TilePlan plan;
-  uint32_t ys_len = 0;          // owned-node slots per tile (>= max_own, even)
+  uint32_t ys_len = 0;          // owned-node slots per tile (>= max_own, even; batched kernels)
+  uint32_t lds64 = 0, lds32 = 0;  // single right-hand-side kernel: max per-tile LDS bytes
   DevBuf<TileRec> tiles;
   DevBuf<uint16_t> lid, meta;
   DevBuf<uint32_t> node_tab;
   DevBuf<double> rk64, rm64;    // per-record coefficients
   DevBuf<float> rk32, rm32;
+  DevBuf<int> texp;             // per-tile exponent bound of the contributions (fixed point)
 };
 
 }  // namespace dgh
@@ -212,7 +222,7 @@ struct dgh_solver_s {
 
 namespace dgh {
 void upload_coefs(dgh_mesh m, DevTiles& d);
-void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&));
+void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&), uint32_t budget = 0);
 }  // namespace dgh
 
 namespace dgh {

@@ -1,11 +1,11 @@
 // CDNA4 (gfx950) operator and transfer kernels of libdendrohip.
 //
 // The operator kernel is owner-computes over tiles of Morton-ordered elements (the host
-// plan of tiles.cpp): one workgroup per tile stages the tile's x values and the 18 child-0
-// stencils in LDS, evaluates every record (own and halo elements), accumulates the rows of
-// the nodes the tile owns with LDS atomics, and stores each owned node exactly once.  There
-// are no global atomics and y needs no zeroing; the result is deterministic up to the order
-// of the LDS additions.
+// plan of tiles.cpp): one workgroup per tile stages the tile's x values in LDS, evaluates
+// every record (own and halo elements) with the 18 child-0 stencils read through the scalar
+// cache, accumulates the rows of the nodes the tile owns with 64-bit fixed-point LDS
+// atomics, and stores each owned node exactly once.  There are no global atomics, y needs no
+// zeroing, and the integer sums make the result independent of the order of the additions.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -125,114 +125,162 @@ __device__ __forceinline__ void eval_rec_with(const Rec& R, uint32_t n_own, uint
   }
 }
 
-// Every stencil read goes through the scalar cache: the wavefront walks over the distinct
-// stencil types among its active lanes (one in class-pure wavefronts, two at the class
-// boundaries of a tile) and evaluates the lanes of each type with a wave-uniform pointer.
-template <class T, class TA, bool MASS, int MODE>
-__device__ __forceinline__ void eval_rec(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac,
-                                         const T* __restrict__ sK, const T* __restrict__ K0,
-                                         const T* __restrict__ xs, TA* __restrict__ ys) {
-  (void)sK;
+// Full stencil rows for the single right-hand-side kernel: per child-0 type, the Laplacian
+// rows [8][8] then the mass rows [8][8].
+constexpr int KFSTRIDE = 128;
+
+// Owned-node rows accumulate in 64-bit fixed point.  A contribution c enters as the integer
+// round(c 2^es), formed by one FMA with 1.5 2^52: for |c 2^es| < 2^51 the bit pattern of
+// fma(c, 2^es, 1.5 2^52) is that of 1.5 2^52 plus round(c 2^es), so subtracting the former
+// leaves the integer in two's complement.  es is chosen per tile so that every contribution
+// stays below 2^FX_BITS: the host bounds the element factors times the stencil row sums
+// (texp[t]: bound < 2^texp), the kernel bounds max |x| over the tile's staged values.  Each
+// term is rounded to 2^-FX_BITS of the tile's bound (relative error of the output ~1e-14);
+// integer LDS additions (ds_add_u64) run about twice as fast as FP64 ones on scattered
+// addresses, and integer sums do not depend on the order of the additions, so the operator
+// is bit-reproducible.
+constexpr int FX_BITS = 50;
+constexpr double FX_MAGIC = 6755399441055744.0;   // 1.5 * 2^52
+constexpr unsigned long long FX_MAGIC_BITS = 0x4338000000000000ull;
+constexpr int FX_XMIN = -900;                      // |x| below 2^-900 counts as 2^-900
+
+__device__ __forceinline__ unsigned long long fx_bits(double v) {
+  return static_cast<unsigned long long>(__double_as_longlong(v)) - FX_MAGIC_BITS;
+}
+
+// Evaluates the rows of the owned vertices of one record and adds them, scaled by 2^es, to
+// the fixed-point sums.  Stencil rows are read through the scalar cache, one row at a time
+// (16 or 32 SGPRs live): the wavefront walks over the distinct stencil types among its
+// lanes (one in class-pure wavefronts, two at the class boundaries of a tile) with a
+// wave-uniform row pointer.  The compiler computes a row only when some lane owns it.
+template <class T, bool MASS, int MODE>
+__device__ __forceinline__ void eval_rows(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac, int es,
+                                          const T* __restrict__ KF, const T* __restrict__ xs,
+                                          unsigned long long* __restrict__ acc) {
+  T fac1, fac2;
+  elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
+  const double f1s = ldexp(double(fac1), es);
+  const double f2s = MASS ? ldexp(double(fac2), es) : 0.0;
+  T xv[8];
+  if (MODE != OP_DIAG) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xv[j] = xs[R.l[j]];
+  }
   const uint32_t tp = R.mt & 31u;
   bool done = false;
   for (;;) {
     const uint64_t pend = __ballot(!done);
     if (pend == 0) break;
-    // stencil type of the first pending lane: wave-uniform, so the stencil is read with
-    // scalar loads
     const uint32_t tu = __builtin_amdgcn_readlane(tp, __builtin_ctzll(pend));
-    const T* Ls = K0 + tu * KSTRIDE;
-    T Lr[36], Mr[36];
+    const bool mine = !done && tp == tu;
+    const T* Kt = KF + tu * KFSTRIDE;
 #pragma unroll
-    for (int q = 0; q < 36; ++q) {
-      Lr[q] = Ls[q];
-      if (MASS) Mr[q] = Ls[36 + q];
+    for (int k = 0; k < 8; ++k) {
+      T lr[8], mr[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        lr[j] = Kt[k * 8 + j];
+        if (MASS) mr[j] = Kt[64 + k * 8 + j];
+      }
+      if (mine && R.l[k] < n_own) {
+        double v;
+        if (MODE == OP_DIAG) {
+          v = MASS ? __builtin_fma(double(lr[k]), f1s, __builtin_fma(double(mr[k]), f2s, FX_MAGIC))
+                   : __builtin_fma(double(lr[k]), f1s, FX_MAGIC);
+        } else {
+          T a = lr[0] * xv[0];
+#pragma unroll
+          for (int j = 1; j < 8; ++j) a += lr[j] * xv[j];
+          if (MASS) {
+            T b = mr[0] * xv[0];
+#pragma unroll
+            for (int j = 1; j < 8; ++j) b += mr[j] * xv[j];
+            v = __builtin_fma(double(a), f1s, __builtin_fma(double(b), f2s, FX_MAGIC));
+          } else {
+            v = __builtin_fma(double(a), f1s, FX_MAGIC);
+          }
+        }
+        atomicAdd(&acc[R.l[k]], fx_bits(v));
+      }
     }
-    if (!done && tp == tu) {
-      eval_rec_with<T, TA, MASS, MODE>(R, n_own, maxD, hFac, Lr, Mr, xs, ys);
-      done = true;
-    }
+    done = done || mine;
   }
 }
 
-// Keep a copy of the 18 stencil triangles in LDS for wavefronts with mixed stencil types
-// (false: those few wavefronts read the global table through L1).
-constexpr bool LDS_STENCIL = false;
-constexpr int RCHUNK = 1;   // records per thread whose loads are in flight together
 constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 
-// K0: [18][72] child-0 Laplacian then mass upper triangles of precision T.
+// KF: [18][128] child-0 full stencil rows of precision T (Laplacian, then mass).
 // kappa/mass: per record (nullptr: uniform lap_fac / mass_fac).  MASS: mass term present.
-// Row sums accumulate in FP64 LDS for both precisions (TA = double).
-template <class T, class TC, bool MASS, int MODE, int BLOCK, int RCH = RCHUNK, int XCH = XCHUNK, int MINW = 1>
-__global__ void __launch_bounds__(BLOCK, MINW)
+// texp[t]: exponent bound of tile t's contributions per unit |x| (see FX_BITS).
+// LDS per tile: fixed-point sums [n_own] (u64), then the staged x of the touched nodes
+// [n_touch] (T), owned nodes first.
+template <class T, class TC, bool MASS, int MODE, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
                const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
                const TC* __restrict__ mass, double lap_fac, double mass_fac, uint32_t maxD,
-               double hFac, const T* __restrict__ K0, const uint32_t* __restrict__ node_tab,
-               uint32_t ys_len, OpIO<T> io) {
-  using TA = double;
+               double hFac, const T* __restrict__ KF, const uint32_t* __restrict__ node_tab,
+               const int* __restrict__ texp, OpIO<T> io) {
+  constexpr int NW = BLOCK / 64;
   const T* __restrict__ x = io.x;
-  // LDS: owned-node sums [ys_len >= n_own] (TA), stencils [18][72], x of touched nodes
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  TA* ys = reinterpret_cast<TA*>(smem_raw);
-  T* sK = LDS_STENCIL ? reinterpret_cast<T*>(ys + ys_len) : nullptr;
-  T* xs = LDS_STENCIL ? sK + NSTENCIL * KSTRIDE : reinterpret_cast<T*>(ys + ys_len);
+  __shared__ uint32_t s_xhi[NW];
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem_raw);
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t >= ntiles) return;
   const TileRec rec = tiles[t];
   const uint32_t* nodes = node_tab + rec.node_off;
   const uint32_t n_own = rec.n_own;
-  // first chunk of records: loads issued before the x gather so both chains overlap
-  Rec R[RCH];
-#pragma unroll
-  for (int q = 0; q < RCH; ++q) {
-    const uint32_t r = threadIdx.x + q * BLOCK;
-    if (r < rec.n_elem) load_rec<TC, MASS>(R[q], rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
-  }
-  if (LDS_STENCIL)
-    for (int i = threadIdx.x; i < NSTENCIL * KSTRIDE; i += BLOCK) sK[i] = K0[i];
+  T* xs = reinterpret_cast<T*>(acc + n_own);
+  // first record: its loads are issued before the x gather so both chains overlap
+  Rec R;
+  if (threadIdx.x < rec.n_elem) load_rec<TC, MASS>(R, rec.elem_off + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac);
+  uint32_t xhi = 0;   // high word of max |x| over the values this thread staged
   if (MODE != OP_DIAG) {
-    for (uint32_t base = 0; base < rec.n_touch; base += XCH * BLOCK) {
-      uint32_t nid[XCH];
-      T v[XCH];
+    for (uint32_t base = 0; base < rec.n_touch; base += XCHUNK * BLOCK) {
+      uint32_t nid[XCHUNK];
+      T v[XCHUNK];
 #pragma unroll
-      for (int q = 0; q < XCH; ++q) {
+      for (int q = 0; q < XCHUNK; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         nid[q] = (i < rec.n_touch) ? nodes[i] : 0u;
       }
 #pragma unroll
-      for (int q = 0; q < XCH; ++q) {
+      for (int q = 0; q < XCHUNK; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         v[q] = (i < rec.n_touch) ? x[nid[q]] : T(0);
       }
 #pragma unroll
-      for (int q = 0; q < XCH; ++q) {
+      for (int q = 0; q < XCHUNK; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         if (i < rec.n_touch) xs[i] = v[q];
+        xhi = max(xhi, uint32_t(uint64_t(__double_as_longlong(double(v[q]))) >> 32) & 0x7fffffffu);
       }
     }
   }
-  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) ys[i] = TA(0);
+  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) acc[i] = 0ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) xhi = max(xhi, uint32_t(__shfl_xor(int(xhi), off)));
+  if ((threadIdx.x & 63) == 0) s_xhi[threadIdx.x >> 6] = xhi;
   __syncthreads();
+#pragma unroll
+  for (int w = 0; w < NW; ++w) xhi = max(xhi, s_xhi[w]);
+  // max |x| < 2^ex; a non-finite x makes every owned output of the tile NaN
+  const bool finite = (xhi >> 20) < 0x7ffu;
+  const int ex = MODE == OP_DIAG ? 1 : max(int(xhi >> 20) - 1022, FX_XMIN);
+  const int es = FX_BITS - texp[t] - ex;
   for (uint32_t base = 0;;) {
-#pragma unroll
-    for (int q = 0; q < RCH; ++q) {
-      const uint32_t r = base + threadIdx.x + q * BLOCK;
-      if (r < rec.n_elem) eval_rec<T, TA, MASS, MODE>(R[q], n_own, maxD, hFac, sK, K0, xs, ys);
-    }
-    base += RCH * BLOCK;
+    if (base + threadIdx.x < rec.n_elem) eval_rows<T, MASS, MODE>(R, n_own, maxD, hFac, es, KF, xs, acc);
+    base += BLOCK;
     if (base >= rec.n_elem) break;
-#pragma unroll
-    for (int q = 0; q < RCH; ++q) {
-      const uint32_t r = base + threadIdx.x + q * BLOCK;
-      if (r < rec.n_elem) load_rec<TC, MASS>(R[q], rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
-    }
+    if (base + threadIdx.x < rec.n_elem)
+      load_rec<TC, MASS>(R, rec.elem_off + base + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) {
     const uint32_t n = nodes[i];
-    const T v = T(ys[i]);
+    const double s = ldexp(double(static_cast<long long>(acc[i])), -es);
+    const T v = finite ? T(s) : T(__builtin_nan(""));
     if (MODE == OP_MATVEC || MODE == OP_DIAG) {
       io.y[n] = v;
     } else if (MODE == OP_RESID) {

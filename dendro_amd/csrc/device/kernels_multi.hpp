@@ -40,7 +40,7 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 // Vector-ALU batched operator.  Lane = (element of the round, right-hand side c of the slice):
 // the KSL lanes of an element share its record, read the element's 8 node values of their
 // column from LDS and accumulate its 8 rows; the stencil comes through the scalar cache
-// (per-type wavefront loop of eval_rec), so LDS carries only the minimum per element and
+// (per-type wavefront loop of eval_rows), so LDS carries only the minimum per element and
 // column: 8 reads and at most 8 additions.  Sums accumulate in FP64 LDS for both precisions
 // (FP32 LDS additions are several times slower on this device).  LDS rows are padded to
 // KSL + 1 values so the lanes of different elements spread over the banks.

@@ -52,16 +52,23 @@ struct TileRec {
   uint32_t pad[3];
 };
 struct TilePlan {
-  uint32_t tile_elems = 0;
+  uint32_t tile_elems = 0;          // element cap per tile
+  uint32_t budget = 0;              // cap on n_own + n_touch per tile (0: none)
   std::vector<TileRec> tiles;
   std::vector<uint16_t> lid;        // [R][8] tile-local node ids, vertex order k^child
   std::vector<uint16_t> meta;       // [R] child-0 stencil type | level << 5
   std::vector<uint32_t> rec_elem;   // [R] element of each record
   std::vector<uint32_t> node_tab;   // global node ids, per tile: owned then others
   uint32_t max_touch = 0, max_own = 0, max_elem = 0;
+  uint32_t max_slots = 0;           // max over tiles of n_own + n_touch
 };
+// Tiles of at most T consecutive elements; with budget > 0 a tile also ends before its
+// owned plus touched node count would exceed the budget (tiles then vary in length).
 void build_tiles(uint32_t E, uint32_t N, const uint32_t* nl, const uint8_t* child, const uint8_t* hn,
-                 const uint8_t* lev, uint32_t T, TilePlan& plan);
+                 const uint8_t* lev, uint32_t T, TilePlan& plan, uint32_t budget = 0);
+// Tile boundaries alone: bounds[t] = first element of tile t, bounds.back() = E.
+void tile_bounds(uint32_t E, uint32_t N, const uint32_t* nl, uint32_t T, uint32_t budget,
+                 std::vector<uint32_t>& bounds);
 
 // Matching of coarse elements to fine elements (same element or 8 children), in order:
 // first fine element of each coarse element and whether it is the same element.

@@ -27,25 +27,96 @@ static uint8_t mirror_mask(uint8_t mask, int c) {
   return m;
 }
 
+// Greedy cut: a tile grows one element at a time.  Adding element b makes it an own
+// element (its nodes become touched) and makes it the owner of the nodes whose first
+// touching element it is; every later element touching such a node joins the tile's halo
+// (its nodes become touched too).  Owned and touched counts only grow, so the tile ends
+// before the first element whose addition would exceed the budget or the element cap.
+void tile_bounds(uint32_t E, uint32_t N, const uint32_t* nl, uint32_t T, uint32_t budget,
+                 std::vector<uint32_t>& bounds) {
+  bounds.clear();
+  if (budget == 0) {
+    for (uint32_t e = 0; e < E; e += T) bounds.push_back(e);
+    bounds.push_back(E);
+    return;
+  }
+  std::vector<uint32_t> first(N, 0xffffffffu);
+  for (uint32_t e = 0; e < E; ++e)
+    for (int k = 0; k < 8; ++k) {
+      uint32_t& f = first[nl[size_t(e) * 8 + k]];
+      if (f == 0xffffffffu) f = e;
+    }
+  // elements touching each node (CSR), in ascending element order
+  std::vector<uint32_t> ptr(size_t(N) + 1, 0), adj(size_t(E) * 8);
+  for (size_t i = 0; i < size_t(E) * 8; ++i) ptr[nl[i] + 1]++;
+  for (uint32_t n = 0; n < N; ++n) ptr[n + 1] += ptr[n];
+  {
+    std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (uint32_t e = 0; e < E; ++e)
+      for (int k = 0; k < 8; ++k) adj[fill[nl[size_t(e) * 8 + k]]++] = e;
+  }
+  std::vector<uint32_t> nmark(N, 0), emark(E, 0);
+  uint32_t stamp = 0;
+  for (uint32_t a = 0; a < E;) {
+    ++stamp;
+    bounds.push_back(a);
+    uint32_t own = 0, touch = 0, b = a;
+    auto touch_elem = [&](uint32_t e) {
+      if (emark[e] == stamp) return;
+      emark[e] = stamp;
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t n = nl[size_t(e) * 8 + k];
+        if (nmark[n] != stamp) {
+          nmark[n] = stamp;
+          ++touch;
+        }
+      }
+    };
+    for (; b < E && b - a < T; ++b) {
+      touch_elem(b);
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t n = nl[size_t(b) * 8 + k];
+        if (first[n] != b) continue;
+        bool dup = false;   // a node listed twice in one element is owned once
+        for (int q = 0; q < k; ++q) dup |= (nl[size_t(b) * 8 + q] == n);
+        if (dup) continue;
+        ++own;
+        for (uint32_t p = ptr[n]; p < ptr[n + 1]; ++p)
+          if (adj[p] > b) touch_elem(adj[p]);
+      }
+      if (own + touch > budget || touch > 65535u) break;
+    }
+    if (b == a) fail(DGH_ERR_ARG, "tile budget %u is smaller than one element's footprint", budget);
+    a = b;
+  }
+  bounds.push_back(E);
+}
+
 void build_tiles(uint32_t E, uint32_t N, const uint32_t* nl, const uint8_t* child, const uint8_t* hn,
-                 const uint8_t* lev, uint32_t T, TilePlan& plan) {
+                 const uint8_t* lev, uint32_t T, TilePlan& plan, uint32_t budget) {
   if (T < 64 || T > 16384) fail(DGH_ERR_ARG, "tile size %u out of range [64,16384]", T);
   plan = TilePlan();
   plan.tile_elems = T;
-  const uint32_t ntiles = (E + T - 1) / T;
+  plan.budget = budget;
+  std::vector<uint32_t> bounds;
+  tile_bounds(E, N, nl, T, budget, bounds);
+  const uint32_t ntiles = uint32_t(bounds.size() - 1);
+  std::vector<uint32_t> tile_of(E);
+  for (uint32_t t = 0; t < ntiles; ++t)
+    for (uint32_t e = bounds[t]; e < bounds[t + 1]; ++e) tile_of[e] = t;
   // owner tile of every node: first touching element's tile
   std::vector<uint32_t> owner(N, 0xffffffffu);
   for (uint32_t e = 0; e < E; ++e)
     for (int k = 0; k < 8; ++k) {
       uint32_t& o = owner[nl[size_t(e) * 8 + k]];
-      if (o == 0xffffffffu) o = e / T;
+      if (o == 0xffffffffu) o = tile_of[e];
     }
   for (uint32_t n = 0; n < N; ++n)
     if (owner[n] == 0xffffffffu) fail(DGH_ERR_MESH, "node %u is not a vertex of any element", n);
   // halo elements per tile
   std::vector<std::vector<uint32_t>> halo(ntiles);
   for (uint32_t e = 0; e < E; ++e) {
-    const uint32_t te = e / T;
+    const uint32_t te = tile_of[e];
     uint32_t seen[8];
     int ns = 0;
     for (int k = 0; k < 8; ++k) {
@@ -88,7 +159,7 @@ void build_tiles(uint32_t E, uint32_t N, const uint32_t* nl, const uint8_t* chil
       std::vector<int32_t> lid_of(N, -1);
       for (uint32_t t = w; t < ntiles; t += nt) {
         auto& recs = t_rec[t];
-        const uint32_t e0 = t * T, e1 = std::min(E, e0 + T);
+        const uint32_t e0 = bounds[t], e1 = bounds[t + 1];
         recs.resize(e1 - e0);
         std::iota(recs.begin(), recs.end(), e0);
         recs.insert(recs.end(), halo[t].begin(), halo[t].end());
@@ -156,6 +227,7 @@ void build_tiles(uint32_t E, uint32_t N, const uint32_t* nl, const uint8_t* chil
     plan.max_touch = std::max(plan.max_touch, plan.tiles[t].n_touch);
     plan.max_own = std::max(plan.max_own, plan.tiles[t].n_own);
     plan.max_elem = std::max(plan.max_elem, plan.tiles[t].n_elem);
+    plan.max_slots = std::max(plan.max_slots, plan.tiles[t].n_own + plan.tiles[t].n_touch);
   }
   if (R > 0xffffffffu || NT > 0xffffffffu) fail(DGH_ERR_ARG, "mesh too large for 32-bit tile offsets");
   plan.rec_elem.resize(R);
