@@ -303,7 +303,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "kernel": f"dgh::tile_op_kernel<double, double, false, 0, {tuned['block']}>",
-                     "tile_elems": mf.tile_info()["tile_elems"], "block": tuned["block"]},
+                     "tile_elems": mf.tile_info()["tile_elems"], "block": tuned["block"],
+                     "n_tiles": mf.tile_info()["n_tiles"], "n_records": mf.tile_info()["n_records"]},
         "matvec_fp32_ms": round(ms32, 5),
         "matvec_fp32_gdofs": round(N / (ms32 * 1e-3) / 1e9, 4),
     }

@@ -71,6 +71,37 @@ def test_matvec_fp64(dev, hier_small, mode):
     assert rel_err(_np(yt), y_ref) <= TOL64
 
 
+@pytest.mark.parametrize("prec", [da.F64, da.F32])
+def test_matvec_reproducible_and_scale_exact(dev, hier_small, prec):
+    """The operator sums owned-node rows in 64-bit fixed point with a per-tile scale taken
+    from max |x|: repeated launches are bit-identical, scaling x by a power of two scales
+    y by exactly the same power (the integers are unchanged), and a non-finite x turns the
+    rows it touches into NaN instead of wrapping around."""
+    h = hier_small
+    F = h.fine
+    c = var_coefficients(h, mass_base=1.0)
+    m = da.Mesh(F)
+    m.set_coefficients(c[-1][0], c[-1][1])
+    dt = torch.float64 if prec == da.F64 else torch.float32
+    s = 300 if prec == da.F64 else 60
+    x = _t(np.random.default_rng(7).uniform(-1, 1, F.n_node), dev, dt)
+    ys = []
+    for xv in (x, x, x * 2.0 ** s, x * 2.0 ** -s):
+        y = torch.empty_like(xv)
+        m.matvec(xv, y, prec=prec)
+        torch.cuda.synchronize()
+        ys.append(y)
+    assert torch.equal(ys[0], ys[1])
+    assert torch.equal(ys[2], ys[0] * 2.0 ** s)
+    assert torch.equal(ys[3], ys[0] * 2.0 ** -s)
+    xn = x.clone()
+    xn[F.n_node // 2] = float("nan")
+    yn = torch.empty_like(xn)
+    m.matvec(xn, yn, prec=prec)
+    torch.cuda.synchronize()
+    assert torch.isnan(yn[F.n_node // 2])
+
+
 def test_diagonal_fp64(dev, hier_small):
     h = hier_small
     c = var_coefficients(h, mass_base=1.0)
