@@ -100,10 +100,12 @@ static void launch_prolong(dgh_transfer t, const T* c, const T* f_in, T* f_out, 
 }
 
 // Per-record coefficient copies of a tiling (they follow its record order), and the per-tile
-// exponent bounds of the fixed-point sums: texp[t] = e with
+// bounds of the fixed-point sums: texp[t].x = e with
 //   |fac1| max_k sum_j |L_t[k][j]| + |fac2| max_k sum_j |M_t[k][j]| < 2^e
-// over the tile's records, factors formed as the kernel forms them (elem_factors).
+// over the tile's records, factors formed as the kernel forms them (elem_factors), and
+// texp[t].y = v with every owned node of the tile receiving at most 2^v terms.
 void upload_coefs(dgh_mesh m, DevTiles& d) {
+  ++m->gen;
   d.rk64.release();
   d.rm64.release();
   d.rk32.release();
@@ -112,9 +114,17 @@ void upload_coefs(dgh_mesh m, DevTiles& d) {
   if (R == 0) return;
   const bool per_elem = !m->h_kappa.empty();
   const bool mass = per_elem ? !m->h_mass.empty() : true;
-  std::vector<int> te(d.plan.tiles.size());
+  std::vector<int2> te(d.plan.tiles.size());
+  std::vector<uint32_t> terms;
   for (size_t t = 0; t < d.plan.tiles.size(); ++t) {
     const TileRec& tr = d.plan.tiles[t];
+    terms.assign(tr.n_own, 0u);
+    for (size_t q = size_t(tr.elem_off) * 8; q < size_t(tr.elem_off + tr.n_elem) * 8; ++q)
+      if (d.plan.lid[q] < tr.n_own) terms[d.plan.lid[q]]++;
+    uint32_t vmax = 1;
+    for (uint32_t c : terms) vmax = std::max(vmax, c);
+    int v = 0;
+    while ((1u << v) < vmax) ++v;
     double mx = 0.0;
     for (uint32_t r = tr.elem_off; r < tr.elem_off + tr.n_elem; ++r) {
       const uint32_t e = d.plan.rec_elem[r];
@@ -127,7 +137,7 @@ void upload_coefs(dgh_mesh m, DevTiles& d) {
     if (!std::isfinite(mx)) fail(DGH_ERR_ARG, "non-finite element coefficients");
     int e = -1022;
     if (mx > 0.0) std::frexp(mx, &e);   // mx < 2^e
-    te[t] = std::max(e, -1022);
+    te[t] = make_int2(std::max(e, -1022), v);
   }
   d.texp.upload(te.data(), te.size());
   if (!per_elem) return;
@@ -146,13 +156,14 @@ void upload_coefs(dgh_mesh m, DevTiles& d) {
 // (Re)builds a tiling with the largest tile size <= T whose LDS need (lds(d)) fits; with
 // budget > 0 tiles also end before their owned plus touched nodes exceed the budget.
 void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&), uint32_t budget) {
+  ++m->gen;
   for (;;) {
     build_tiles(m->E, m->N, m->h_nl.data(), m->h_child.data(), m->h_hn.data(), m->h_lev.data(), T, d.plan, budget);
     d.ys_len = (d.plan.max_own + 1) & ~1u;
     d.lds64 = d.lds32 = 0;
     for (const TileRec& tr : d.plan.tiles) {
       d.lds64 = std::max(d.lds64, 8 * (tr.n_own + tr.n_touch));
-      d.lds32 = std::max(d.lds32, 8 * tr.n_own + 4 * tr.n_touch);
+      d.lds32 = std::max(d.lds32, 4 * (tr.n_own + tr.n_touch));
     }
     if (lds(d) <= size_t(g.max_lds) || T <= 32) break;
     T /= 2;
@@ -204,9 +215,18 @@ static void project_mean(dgh_solver s, uint32_t n, T* v, hipStream_t st) {
 template <class T> std::vector<LevelVecs<T>>& vecs(dgh_solver s);
 template <> std::vector<LevelVecs<double>>& vecs<double>(dgh_solver s) { return s->v64; }
 template <> std::vector<LevelVecs<float>>& vecs<float>(dgh_solver s) { return s->v32; }
-template <class T> const T* ainv(dgh_solver s);
-template <> const double* ainv<double>(dgh_solver s) { return s->Ainv64.p; }
-template <> const float* ainv<float>(dgh_solver s) { return s->Ainv32.p; }
+template <class T> const T* ainvp(dgh_solver s);
+template <> const double* ainvp<double>(dgh_solver s) { return s->Ap64.p; }
+template <> const float* ainvp<float>(dgh_solver s) { return s->Ap32.p; }
+
+// x = Ainv b on the coarsest level (packed symmetric blocks, two passes)
+template <class T>
+static void coarse_apply(dgh_solver s, const T* b, T* x, hipStream_t st) {
+  const int nblk = s->nbc * (s->nbc + 1) / 2;
+  hipLaunchKernelGGL(sym_block_kernel<T>, dim3(nblk), dim3(128), 0, st, s->nc, s->npc, ainvp<T>(s), b, s->Qc.p);
+  hipLaunchKernelGGL(sym_sum_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->nc, s->nbc, s->npc,
+                     (const double*)s->Qc.p, b, (const double*)(s->prm.null_space ? s->c1.p : nullptr), x);
+}
 
 // PCMG multiplicative V-cycle on level l for right-hand side b (PCMGMCycle_Private with the
 // options of bin/options:182-206): pre-smoothing from a zero guess, residual, restriction,
@@ -217,15 +237,7 @@ static T* vcycle(dgh_solver s, int l, const T* b, hipStream_t st) {
   auto& VV = vecs<T>(s);
   auto& V = VV[l];
   if (l == 0) {
-    const int n = s->nc;
-    const T* bb = b;
-    if (s->prm.null_space) {
-      hipLaunchKernelGGL(copy_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, uint32_t(n), b, V.r.p);
-      project_mean<T>(s, uint32_t(n), V.r.p, st);
-      bb = V.r.p;
-    }
-    if (reinterpret_cast<uintptr_t>(bb) % 16) fail(DGH_ERR_ARG, "coarse right-hand side is not 16-byte aligned");
-    hipLaunchKernelGGL(dense_apply_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, s->ldc, ainv<T>(s), bb, V.xa.p);
+    coarse_apply<T>(s, b, V.xa.p, st);   // projects b first with a constant null space
     return V.xa.p;
   }
   dgh_mesh m = s->meshes[l];
@@ -267,31 +279,74 @@ static void precondition(dgh_solver s, const double* r, double* z, hipStream_t s
   if (s->prm.null_space) project_mean<double>(s, n, z, st);
 }
 
-// One CG iteration (everything between two stopping tests), on device scalars only.
+// z = B r on the finest level, unprojected, in the V-cycle's precision (the buffer is the
+// same on every call).  Mixed: the FP32 copy of r must already sit in the level's b vector.
+static const void* precondition_raw(dgh_solver s, const double* r, hipStream_t st) {
+  const int top = s->nlevels - 1;
+  if (s->prm.vcycle_prec == DGH_F32) return vcycle<float>(s, top, s->v32[top].b.p, st);
+  return vcycle<double>(s, top, r, st);
+}
+
+template <int BLOCK>
+static void launch_cg_b(dgh_mesh m, const double* p, double* w, double* part, hipStream_t st) {
+  const int nt = int(m->op.plan.tiles.size());
+  const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
+  const size_t lds = m->op.lds64;
+  const OpIO<double> io{p, w, nullptr, nullptr, 0.0, part};
+  const double* kap = m->op.rk64.p;
+  const double* mas = m->op.rm64.p;
+  if (kap && !mas)
+    hipLaunchKernelGGL((tile_op_kernel<double, double, false, OP_CG, BLOCK>), dim3(grid), dim3(BLOCK), lds, st,
+                       m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
+                       m->hFac, KFtab<double>(), m->op.node_tab.p, m->op.texp.p, io);
+  else
+    hipLaunchKernelGGL((tile_op_kernel<double, double, true, OP_CG, BLOCK>), dim3(grid), dim3(BLOCK), lds, st,
+                       m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
+                       m->hFac, KFtab<double>(), m->op.node_tab.p, m->op.texp.p, io);
+  HIP_CHECK(hipGetLastError());
+}
+
+// w = A p and the tile partials of p.w -> part
+static void launch_cg(dgh_mesh m, const double* p, double* w, double* part, hipStream_t st) {
+  switch (m->block) {
+    case 256: launch_cg_b<256>(m, p, w, part, st); break;
+    case 512: launch_cg_b<512>(m, p, w, part, st); break;
+    default: launch_cg_b<1024>(m, p, w, part, st); break;
+  }
+}
+
+// One CG iteration (everything between two stopping tests), on device scalars only:
+// direction update (projected z), the operator with the p.w tile partials, alpha, x and r
+// updates (and the FP32 copy of r), the V-cycle, one pass for all dot products (z kept
+// unprojected, see cg_dots_kernel), the scalars.
 static void cg_iteration(dgh_solver s, hipStream_t st) {
   const int top = s->nlevels - 1;
   dgh_mesh m = s->meshes[top];
   const uint32_t n = m->N;
-  const bool flexible = (s->prm.vcycle_prec == DGH_F32);
+  const bool mixed = (s->prm.vcycle_prec == DGH_F32);
   const bool pnorm = (s->prm.norm_type == DGH_NORM_PRECONDITIONED);
   double* S = s->S.p;
-  hipLaunchKernelGGL(cg_p_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
-                     (const double*)s->z.p, s->p.p);
-  launch_matvec<double>(m, s->p.p, s->w.p, st);
-  DotArgs<double> d{};
-  d.a[0] = s->p.p;
-  d.b[0] = s->w.p;
-  reduce<double>(n, d, 1, s->part.p, S + S_PW, st);
-  hipLaunchKernelGGL(cg_xr_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, S, (const double*)s->p.p,
-                     (const double*)s->w.p, s->x.p, s->r.p);
-  precondition(s, s->r.p, s->z.p, st);
-  DotArgs<double> e{};
-  e.a[0] = s->z.p; e.b[0] = s->z.p;    // ZZ
-  e.a[1] = s->z.p; e.b[1] = s->r.p;    // ZR
-  e.a[2] = flexible ? s->z.p : nullptr; e.b[2] = s->w.p;   // ZW
-  e.a[3] = pnorm ? nullptr : s->r.p; e.b[3] = s->r.p;      // RR
-  reduce<double>(n, e, 4, s->part.p, S + S_ZZ, st);
-  hipLaunchKernelGGL(cg_scalar_kernel, dim3(1), dim3(64), 0, st, S, int(pnorm), int(flexible));
+  if (mixed)
+    hipLaunchKernelGGL(cg_p_fused_kernel<float>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
+                       static_cast<const float*>(s->zres), s->p.p);
+  else
+    hipLaunchKernelGGL(cg_p_fused_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
+                       static_cast<const double*>(s->zres), s->p.p);
+  launch_cg(m, s->p.p, s->w.p, s->ptile.p, st);
+  hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED2_THREADS), 0, st, int(m->op.plan.tiles.size()),
+                     (const double*)s->ptile.p, S);
+  hipLaunchKernelGGL(cg_xr_fused_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
+                     (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p, mixed ? s->v32[top].b.p : nullptr);
+  const void* z = precondition_raw(s, s->r.p, st);
+  if (z != s->zres) fail(DGH_ERR_STATE, "V-cycle result buffer changed");
+  if (mixed)
+    hipLaunchKernelGGL(cg_dots_kernel<float>, dim3(DOT_BLOCKS), dim3(VBLOCK), 0, st, n, static_cast<const float*>(z),
+                       (const double*)s->r.p, (const double*)s->w.p, s->part.p);
+  else
+    hipLaunchKernelGGL(cg_dots_kernel<double>, dim3(DOT_BLOCKS), dim3(VBLOCK), 0, st, n, static_cast<const double*>(z),
+                       (const double*)s->r.p, (const double*)s->w.p, s->part.p);
+  hipLaunchKernelGGL(cg_scalar_fused_kernel, dim3(1), dim3(RED2_THREADS), 0, st, (const double*)s->part.p, n,
+                     int(s->prm.null_space), S, int(pnorm), int(mixed));
 }
 
 // Assembles the coarsest operator from its element blocks and inverts it on the device
@@ -341,10 +396,26 @@ static void setup_coarse(dgh_solver s) {
   if (hinfo != 0) fail(DGH_ERR_DIVERGED, "coarsest inverse failed (%d)", int(hinfo));
   hipLaunchKernelGGL(symmetrize_kernel<double>, dim3((n + 255) / 256, n), dim3(256), 0, 0, n, ld, s->Ainv64.p);
   HIP_CHECK(hipGetLastError());
+  s->nbc = (n + SB - 1) / SB;
+  s->npc = s->nbc * SB;
+  const int nblk = s->nbc * (s->nbc + 1) / 2;
+  s->Ap64.alloc(size_t(nblk) * SB * SB);
+  s->Qc.alloc(size_t(s->nbc) * s->npc);
+  hipLaunchKernelGGL(sym_pack_kernel<double>, dim3(nblk), dim3(256), 0, 0, n, ld, (const double*)s->Ainv64.p, s->Ap64.p);
+  HIP_CHECK(hipGetLastError());
+  if (s->prm.null_space) {   // Ainv 1, for the projection folded into the coarse solve
+    s->c1.alloc(n);
+    hipLaunchKernelGGL(row_sum_kernel<double>, dim3((n + 3) / 4), dim3(256), 0, 0, n, ld, (const double*)s->Ainv64.p,
+                       s->c1.p);
+    HIP_CHECK(hipGetLastError());
+  }
   if (s->prm.vcycle_prec == DGH_F32) {
     s->Ainv32.alloc(size_t(n) * ld);
     hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(size_t(n) * ld)), dim3(VBLOCK), 0, 0,
                        uint32_t(size_t(n) * ld), (const double*)s->Ainv64.p, s->Ainv32.p);
+    s->Ap32.alloc(size_t(nblk) * SB * SB);
+    hipLaunchKernelGGL(sym_pack_kernel<float>, dim3(nblk), dim3(256), 0, 0, n, ld, (const float*)s->Ainv32.p, s->Ap32.p);
+    HIP_CHECK(hipGetLastError());
   }
   HIP_CHECK(hipDeviceSynchronize());
 }
@@ -743,7 +814,7 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
     HIP_CHECK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s->hS), sizeof(double) * S_NSCAL, hipHostMallocDefault));
     s->S.alloc(S_NSCAL);
-    s->part.alloc(MAXQ * RED_BLOCKS);
+    s->part.alloc(size_t(std::max(MAXQ, NDOT)) * RED_BLOCKS);
     s->part_p.alloc(MAXQ * RED_BLOCKS);
     // per-level vectors and Jacobi diagonals
     s->v64.resize(nlevels);
@@ -818,20 +889,31 @@ int dgh_solver_solve(dgh_solver s, const double* b, double* x, dgh_solve_stats* 
     HIP_CHECK(hipEventRecord(eu, ust));          // b is ready on the caller's stream
     HIP_CHECK(hipStreamWaitEvent(st, eu, 0));
     HIP_CHECK(hipEventRecord(e0, st));
-    // r = b (projected), x = 0, p = 0
+    const bool mixed = (s->prm.vcycle_prec == DGH_F32);
+    const size_t ntiles = m->op.plan.tiles.size();
+    if (s->ptile.n < ntiles) s->ptile.alloc(ntiles);
+    const uint64_t gen = mesh_generation(s);
+    if (gen != s->graph_gen) {   // re-tiled or new coefficients since the graphs were captured
+      drop_graphs(s);
+      s->graph_gen = gen;
+    }
+    // r = b (projected), x = 0, p = 0, z = B r, dots
     hipLaunchKernelGGL(copy_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, b, s->r.p);
     if (s->prm.null_space) project_mean<double>(s, n, s->r.p, st);
     hipLaunchKernelGGL(fill_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, s->x.p, 0.0, n);
     hipLaunchKernelGGL(fill_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, s->p.p, 0.0, n);
-    precondition(s, s->r.p, s->z.p, st);
-    DotArgs<double> d{};
-    d.a[0] = s->z.p; d.b[0] = s->z.p;   // ZZ
-    d.a[1] = s->z.p; d.b[1] = s->r.p;   // ZR
-    reduce<double>(n, d, 2, s->part.p, S + S_ZZ, st);
-    DotArgs<double> bb{};
-    bb.a[0] = s->r.p; bb.b[0] = s->r.p;
-    reduce<double>(n, bb, 1, s->part_p.p, S + S_BB, st);
-    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(64), 0, st, S, int(pnorm));
+    if (mixed)
+      hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n,
+                         (const double*)s->r.p, s->v32[top].b.p);
+    s->zres = precondition_raw(s, s->r.p, st);
+    if (mixed)
+      hipLaunchKernelGGL(cg_dots_kernel<float>, dim3(DOT_BLOCKS), dim3(VBLOCK), 0, st, n,
+                         static_cast<const float*>(s->zres), (const double*)s->r.p, (const double*)nullptr, s->part.p);
+    else
+      hipLaunchKernelGGL(cg_dots_kernel<double>, dim3(DOT_BLOCKS), dim3(VBLOCK), 0, st, n,
+                         static_cast<const double*>(s->zres), (const double*)s->r.p, (const double*)nullptr, s->part.p);
+    hipLaunchKernelGGL(cg_init_fused_kernel, dim3(1), dim3(RED2_THREADS), 0, st, (const double*)s->part.p, n,
+                       int(s->prm.null_space), S, int(pnorm));
     HIP_CHECK(hipMemcpyAsync(s->hS, S, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     const double dp0 = s->hS[S_DP];

@@ -105,7 +105,7 @@ struct DevTiles {
   DevBuf<uint32_t> node_tab;
   DevBuf<double> rk64, rm64;    // per-record coefficients
   DevBuf<float> rk32, rm32;
-  DevBuf<int> texp;             // per-tile exponent bound of the contributions (fixed point)
+  DevBuf<int2> texp;            // per tile: exponent bound of the contributions, log2 bound of the terms per node
 };
 
 }  // namespace dgh
@@ -129,6 +129,7 @@ struct dgh_mesh_s {
   DevTiles mop;
   uint32_t mtile = 0;           // batched tile size (0: default)
   int mvariant = 0;             // batched FP32 products: 0 vector ALUs, 1 matrix cores
+  uint64_t gen = 0;             // bumped whenever device tilings or coefficients are rebuilt
 };
 
 struct dgh_transfer_s {
@@ -186,16 +187,24 @@ struct dgh_solver_s {
   std::vector<LevelVecs<float>> v32;
   int nc = 0;                     // coarsest size
   int ldc = 0;                    // row pitch of the dense inverse
-  DevBuf<double> Ainv64;          // dense coarsest inverse
+  DevBuf<double> Ainv64;          // dense coarsest inverse (full, row pitch ldc)
   DevBuf<float> Ainv32;
+  DevBuf<double> Ap64;            // the same in packed lower SB x SB blocks
+  DevBuf<float> Ap32;
+  DevBuf<double> Qc;              // partial sums of the packed coarse apply [nbc][npc]
+  DevBuf<double> c1;              // Ainv 1 (constant null space)
+  int nbc = 0, npc = 0;           // coarse block rows, padded coarse size
   DevBuf<double> S;               // CG scalars (CgScalar)
   DevBuf<double> part;            // reduction partials [MAXQ][RED_BLOCKS]
   DevBuf<double> part_p;          // partials of the null-space projections
   double* hS = nullptr;           // pinned host mirror of S
   DevBuf<double> x, r, z, p, w;   // outer CG vectors (finest, FP64)
+  DevBuf<double> ptile;           // per-tile partials of p.w from the operator kernel
+  const void* zres = nullptr;     // V-cycle result buffer on the finest level (z, unprojected)
   hipStream_t st = nullptr;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;     // one CG iteration
   hipGraphExec_t gexec = nullptr;
+  uint64_t graph_gen = 0;         // mesh_generation() when the graphs were captured
   // batched right-hand sides (sized for mk columns on first use)
   int mk = 0;
   std::vector<LevelVecs<float>> m32;   // per level [N_l][mk]: b, xa, xb, r
@@ -222,6 +231,12 @@ struct dgh_solver_s {
 
 namespace dgh {
 void upload_coefs(dgh_mesh m, DevTiles& d);
+// Sum of the generations of a solver's meshes: captured graphs hold their buffers.
+inline uint64_t mesh_generation(dgh_solver s) {
+  uint64_t g = 0;
+  for (dgh_mesh m : s->meshes) g += m->gen;
+  return g;
+}
 void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&), uint32_t budget = 0);
 }  // namespace dgh
 

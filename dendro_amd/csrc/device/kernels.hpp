@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "host/octree.hpp"
 
 namespace dgh {
@@ -41,16 +43,18 @@ __device__ __forceinline__ void elem_factors(uint32_t lev, uint32_t maxD, double
 }
 
 // MATVEC: y = A x;  DIAG: y = diag(A);  RESID: y = b - A x;
-// SMOOTH: y = x + omega (invd (b - A x))  (one Richardson + Jacobi sweep, y must not alias x)
-enum OpMode { OP_MATVEC = 0, OP_DIAG = 1, OP_RESID = 2, OP_SMOOTH = 3 };
+// SMOOTH: y = x + omega (invd (b - A x))  (one Richardson + Jacobi sweep, y must not alias x);
+// CG: y = A x and the tile's partial sum of x.y (the CG product p.Ap).
+enum OpMode { OP_MATVEC = 0, OP_DIAG = 1, OP_RESID = 2, OP_SMOOTH = 3, OP_CG = 4 };
 
 template <class T>
 struct OpIO {
-  const T* x;      // operator input (MATVEC, RESID, SMOOTH)
+  const T* x;      // operator input (MATVEC, RESID, SMOOTH, CG)
   T* y;            // output
   const T* b;      // right-hand side (RESID, SMOOTH)
   const T* invd;   // inverse diagonal (SMOOTH)
   T omega;         // Richardson scale (SMOOTH)
+  double* part;    // CG: per-tile partial sums of x.y
 };
 
 // Symmetric 8x8 stencils are kept as their upper triangles (36 entries, row-major):
@@ -129,23 +133,49 @@ __device__ __forceinline__ void eval_rec_with(const Rec& R, uint32_t n_own, uint
 // rows [8][8] then the mass rows [8][8].
 constexpr int KFSTRIDE = 128;
 
-// Owned-node rows accumulate in 64-bit fixed point.  A contribution c enters as the integer
-// round(c 2^es), formed by one FMA with 1.5 2^52: for |c 2^es| < 2^51 the bit pattern of
-// fma(c, 2^es, 1.5 2^52) is that of 1.5 2^52 plus round(c 2^es), so subtracting the former
-// leaves the integer in two's complement.  es is chosen per tile so that every contribution
-// stays below 2^FX_BITS: the host bounds the element factors times the stencil row sums
-// (texp[t]: bound < 2^texp), the kernel bounds max |x| over the tile's staged values.  Each
-// term is rounded to 2^-FX_BITS of the tile's bound (relative error of the output ~1e-14);
-// integer LDS additions (ds_add_u64) run about twice as fast as FP64 ones on scattered
-// addresses, and integer sums do not depend on the order of the additions, so the operator
-// is bit-reproducible.
+// Owned-node rows accumulate in fixed point: 64-bit integers for FP64 operators, 32-bit for
+// FP32.  FP64: a contribution c enters as the integer round(c 2^es), formed by one FMA with
+// 1.5 2^52: for |c 2^es| < 2^51 the bit pattern of fma(c, 2^es, 1.5 2^52) is that of
+// 1.5 2^52 plus round(c 2^es), so subtracting the former leaves the integer in two's
+// complement; es keeps every term below 2^FX_BITS.  FP32: round-to-nearest conversion of
+// c 2^es (float), es keeping every term below 2^(FX32_BITS - v) where 2^v bounds the number of
+// terms a node of the tile receives, so the 32-bit sums cannot overflow.  es is chosen per
+// tile: the host bounds the element factors times the stencil row sums (texp[t].x: bound
+// < 2^texp) and the terms per node (texp[t].y = v), the kernel bounds max |x| over the tile's
+// staged values.  Each FP64 term is rounded to 2^-FX_BITS of the tile's bound (output ~1e-14
+// relative), each FP32 term to about 2^-24 of it.  Integer LDS additions run faster than
+// floating-point ones on scattered addresses (ds_add_u64 about twice ds_add_f64, ds_add_u32
+// faster again) and do not depend on the order of the additions, so the operator is
+// bit-reproducible.
 constexpr int FX_BITS = 50;
+constexpr int FX32_BITS = 30;
 constexpr double FX_MAGIC = 6755399441055744.0;   // 1.5 * 2^52
 constexpr unsigned long long FX_MAGIC_BITS = 0x4338000000000000ull;
 constexpr int FX_XMIN = -900;                      // |x| below 2^-900 counts as 2^-900
 
-__device__ __forceinline__ unsigned long long fx_bits(double v) {
+template <class T>
+using fx_acc_t = typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned int>::type;
+
+// Scale exponent of a tile: every contribution times 2^es stays below the fixed-point bound.
+template <class T>
+__device__ __forceinline__ int fx_scale(int2 te, int ex) {
+  return sizeof(T) == 8 ? FX_BITS - te.x - ex : FX32_BITS - te.y - te.x - ex;
+}
+
+// fixed-point term of a + b (a = row sum times the scaled Laplacian factor, b likewise mass)
+__device__ __forceinline__ unsigned long long fx_term(double a, double f1s, double b, double f2s, bool mass) {
+  const double v = mass ? __builtin_fma(a, f1s, __builtin_fma(b, f2s, FX_MAGIC)) : __builtin_fma(a, f1s, FX_MAGIC);
   return static_cast<unsigned long long>(__double_as_longlong(v)) - FX_MAGIC_BITS;
+}
+__device__ __forceinline__ unsigned int fx_term(float a, float f1s, float b, float f2s, bool mass) {
+  const float v = mass ? __builtin_fmaf(b, f2s, a * f1s) : a * f1s;
+  return static_cast<unsigned int>(__float2int_rn(v));
+}
+
+template <class T>
+__device__ __forceinline__ double fx_value(fx_acc_t<T> s, int es) {
+  return sizeof(T) == 8 ? ldexp(double(static_cast<long long>(s)), -es)
+                        : ldexp(double(static_cast<int>(static_cast<unsigned int>(s))), -es);
 }
 
 // Evaluates the rows of the owned vertices of one record and adds them, scaled by 2^es, to
@@ -156,11 +186,11 @@ __device__ __forceinline__ unsigned long long fx_bits(double v) {
 template <class T, bool MASS, int MODE>
 __device__ __forceinline__ void eval_rows(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac, int es,
                                           const T* __restrict__ KF, const T* __restrict__ xs,
-                                          unsigned long long* __restrict__ acc) {
+                                          fx_acc_t<T>* __restrict__ acc) {
   T fac1, fac2;
   elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
-  const double f1s = ldexp(double(fac1), es);
-  const double f2s = MASS ? ldexp(double(fac2), es) : 0.0;
+  const T f1s = T(ldexp(double(fac1), es));
+  const T f2s = MASS ? T(ldexp(double(fac2), es)) : T(0);
   T xv[8];
   if (MODE != OP_DIAG) {
 #pragma unroll
@@ -183,24 +213,21 @@ __device__ __forceinline__ void eval_rows(const Rec& R, uint32_t n_own, uint32_t
         if (MASS) mr[j] = Kt[64 + k * 8 + j];
       }
       if (mine && R.l[k] < n_own) {
-        double v;
+        T a, b = T(0);
         if (MODE == OP_DIAG) {
-          v = MASS ? __builtin_fma(double(lr[k]), f1s, __builtin_fma(double(mr[k]), f2s, FX_MAGIC))
-                   : __builtin_fma(double(lr[k]), f1s, FX_MAGIC);
+          a = lr[k];
+          if (MASS) b = mr[k];
         } else {
-          T a = lr[0] * xv[0];
+          a = lr[0] * xv[0];
 #pragma unroll
           for (int j = 1; j < 8; ++j) a += lr[j] * xv[j];
           if (MASS) {
-            T b = mr[0] * xv[0];
+            b = mr[0] * xv[0];
 #pragma unroll
             for (int j = 1; j < 8; ++j) b += mr[j] * xv[j];
-            v = __builtin_fma(double(a), f1s, __builtin_fma(double(b), f2s, FX_MAGIC));
-          } else {
-            v = __builtin_fma(double(a), f1s, FX_MAGIC);
           }
         }
-        atomicAdd(&acc[R.l[k]], fx_bits(v));
+        atomicAdd(&acc[R.l[k]], fx_term(a, f1s, b, f2s, MASS));
       }
     }
     done = done || mine;
@@ -211,21 +238,22 @@ constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 
 // KF: [18][128] child-0 full stencil rows of precision T (Laplacian, then mass).
 // kappa/mass: per record (nullptr: uniform lap_fac / mass_fac).  MASS: mass term present.
-// texp[t]: exponent bound of tile t's contributions per unit |x| (see FX_BITS).
-// LDS per tile: fixed-point sums [n_own] (u64), then the staged x of the touched nodes
-// [n_touch] (T), owned nodes first.
+// texp[t]: exponent bound of tile t's contributions per unit |x| and log2 bound of the terms
+// per node (see FX_BITS).
+// LDS per tile: fixed-point sums [n_own] (u64 for FP64, u32 for FP32), then the staged x of
+// the touched nodes [n_touch] (T), owned nodes first.
 template <class T, class TC, bool MASS, int MODE, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
                const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
                const TC* __restrict__ mass, double lap_fac, double mass_fac, uint32_t maxD,
                double hFac, const T* __restrict__ KF, const uint32_t* __restrict__ node_tab,
-               const int* __restrict__ texp, OpIO<T> io) {
+               const int2* __restrict__ texp, OpIO<T> io) {
   constexpr int NW = BLOCK / 64;
   const T* __restrict__ x = io.x;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ uint32_t s_xhi[NW];
-  unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem_raw);
+  fx_acc_t<T>* acc = reinterpret_cast<fx_acc_t<T>*>(smem_raw);
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t >= ntiles) return;
   const TileRec rec = tiles[t];
@@ -236,6 +264,14 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   Rec R;
   if (threadIdx.x < rec.n_elem) load_rec<TC, MASS>(R, rec.elem_off + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac);
   uint32_t xhi = 0;   // high word of max |x| over the values this thread staged
+  // FP32 residual and smoothing: the owned nodes' ids, b and invd are loaded with the first
+  // gather batch, so the store phase waits on no global load (tiles of at most
+  // XCHUNK * BLOCK owned nodes; larger ones reload them).  In FP64 the held values would
+  // take the kernel past 64 VGPRs and cost a workgroup per CU.
+  constexpr bool PREF = sizeof(T) == 4 && (MODE == OP_RESID || MODE == OP_SMOOTH);
+  const bool pref = PREF && n_own <= uint32_t(XCHUNK * BLOCK);
+  uint32_t own_n[XCHUNK];
+  T own_b[XCHUNK], own_d[XCHUNK];
   if (MODE != OP_DIAG) {
     for (uint32_t base = 0; base < rec.n_touch; base += XCHUNK * BLOCK) {
       uint32_t nid[XCHUNK];
@@ -244,6 +280,15 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
       for (int q = 0; q < XCHUNK; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         nid[q] = (i < rec.n_touch) ? nodes[i] : 0u;
+      }
+      if (PREF && base == 0) {
+#pragma unroll
+        for (int q = 0; q < XCHUNK; ++q) {
+          const uint32_t i = threadIdx.x + q * BLOCK;
+          own_n[q] = nid[q];
+          own_b[q] = (pref && i < n_own) ? io.b[nid[q]] : T(0);
+          own_d[q] = (MODE == OP_SMOOTH && pref && i < n_own) ? io.invd[nid[q]] : T(0);
+        }
       }
 #pragma unroll
       for (int q = 0; q < XCHUNK; ++q) {
@@ -258,7 +303,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
       }
     }
   }
-  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) acc[i] = 0ull;
+  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) acc[i] = 0;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) xhi = max(xhi, uint32_t(__shfl_xor(int(xhi), off)));
   if ((threadIdx.x & 63) == 0) s_xhi[threadIdx.x >> 6] = xhi;
@@ -268,7 +313,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   // max |x| < 2^ex; a non-finite x makes every owned output of the tile NaN
   const bool finite = (xhi >> 20) < 0x7ffu;
   const int ex = MODE == OP_DIAG ? 1 : max(int(xhi >> 20) - 1022, FX_XMIN);
-  const int es = FX_BITS - texp[t] - ex;
+  const int es = fx_scale<T>(texp[t], ex);
   for (uint32_t base = 0;;) {
     if (base + threadIdx.x < rec.n_elem) eval_rows<T, MASS, MODE>(R, n_own, maxD, hFac, es, KF, xs, acc);
     base += BLOCK;
@@ -277,17 +322,48 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
       load_rec<TC, MASS>(R, rec.elem_off + base + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac);
   }
   __syncthreads();
+  if (PREF && pref) {
+#pragma unroll
+    for (int q = 0; q < XCHUNK; ++q) {
+      const uint32_t i = threadIdx.x + q * BLOCK;
+      if (i < n_own) {
+        const T v = finite ? T(fx_value<T>(acc[i], es)) : T(__builtin_nan(""));
+        if (MODE == OP_RESID)
+          io.y[own_n[q]] = own_b[q] - v;
+        else
+          io.y[own_n[q]] = xs[i] + io.omega * ((own_b[q] - v) * own_d[q]);
+      }
+    }
+    return;
+  }
+  double dot = 0.0;
   for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) {
     const uint32_t n = nodes[i];
-    const double s = ldexp(double(static_cast<long long>(acc[i])), -es);
+    const double s = fx_value<T>(acc[i], es);
     const T v = finite ? T(s) : T(__builtin_nan(""));
     if (MODE == OP_MATVEC || MODE == OP_DIAG) {
       io.y[n] = v;
     } else if (MODE == OP_RESID) {
       io.y[n] = io.b[n] - v;
-    } else {
+    } else if (MODE == OP_SMOOTH) {
       const T r = io.b[n] - v;
       io.y[n] = xs[i] + io.omega * (r * io.invd[n]);   // owned nodes lead the staged x
+    } else {
+      io.y[n] = v;
+      dot += double(xs[i]) * double(v);
+    }
+  }
+  if (MODE == OP_CG) {   // tile partial of p'.y, summed in a fixed order
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(smem_raw);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+      for (int w = 0; w < NW; ++w) tot += red[w];
+      io.part[t] = tot;
     }
   }
 }

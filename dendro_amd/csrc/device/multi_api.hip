@@ -389,6 +389,11 @@ int dgh_solver_solve_multi(dgh_solver s, int k, const double* B, double* X, dgh_
     if (!s || !B || !X) fail(DGH_ERR_ARG, "NULL argument");
     check_k(k);
     ensure_multi_buffers(s, k);
+    const uint64_t gen = mesh_generation(s);
+    if (gen != s->graph_gen) {   // re-tiled or new coefficients since the graphs were captured
+      drop_graphs(s);
+      s->graph_gen = gen;
+    }
     hipStream_t ust = static_cast<hipStream_t>(stream);
     hipStream_t st = s->st;
     const int top = s->nlevels - 1;

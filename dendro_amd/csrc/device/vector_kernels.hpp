@@ -104,7 +104,7 @@ __global__ void subtract_mean_kernel(uint32_t n, const double* __restrict__ sum,
 // ---------------------------------------------------------------- CG with device scalars
 // S: RZ = z.r, PW = p.w, DP = stopping-test norm, RATIO = p-update factor, ZZ, ZR, ZW, RR,
 // BB = b.b, ALPHA.
-enum CgScalar { S_RZ = 0, S_PW, S_DP, S_RATIO, S_ZZ, S_ZR, S_ZW, S_RR, S_BB, S_ALPHA, S_PROJ, S_NSCAL = 16 };
+enum CgScalar { S_RZ = 0, S_PW, S_DP, S_RATIO, S_ZZ, S_ZR, S_ZW, S_RR, S_BB, S_ALPHA, S_PROJ, S_MU, S_NSCAL = 16 };
 
 // p = z + ratio p  (VecAYPX)
 static __global__ void cg_p_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ z,
@@ -123,6 +123,146 @@ static __global__ void cg_xr_kernel(uint32_t n, double* __restrict__ S, const do
     x[i] = x[i] + a * p[i];
     r[i] = r[i] + (-a) * w[i];
   }
+}
+
+// ---------------------------------------------------------------- fused CG iteration
+// The preconditioned residual z stays unprojected (FP32 in the mixed solver); its constant
+// null-space component mu = mean(z) is applied where z is used: in the direction update
+// (p' = (z - mu) + beta p, inside the operator kernel) and in the dot products, which come
+// from one pass over z, r, w as
+//   z'.z' = z.z - n mu^2,  z'.r = z.r - mu sum(r),  z'.w = z.w - mu sum(w).
+constexpr int NDOT = 7;   // sum z, z.z, z.r, z.w, r.r, sum r, sum w
+enum CgDot { D_SZ = 0, D_ZZ, D_ZR, D_ZW, D_RR, D_SR, D_SW };
+constexpr int DOT_BLOCKS = 256;   // workgroups of the dot pass (one partial each)
+
+// Stage 1 of the dots: block partials part[q * DOT_BLOCKS + block] (w == nullptr: z.w and
+// sum w are zero).
+template <class TZ>
+__global__ void __launch_bounds__(VBLOCK)
+cg_dots_kernel(uint32_t n, const TZ* __restrict__ z, const double* __restrict__ r, const double* __restrict__ w,
+               double* __restrict__ part) {
+  double s[NDOT] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (uint32_t i = blockIdx.x * VBLOCK + threadIdx.x; i < n; i += DOT_BLOCKS * VBLOCK) {
+    const double zi = double(z[i]), ri = r[i], wi = w ? w[i] : 0.0;
+    s[D_SZ] += zi;
+    s[D_ZZ] += zi * zi;
+    s[D_ZR] += zi * ri;
+    s[D_ZW] += zi * wi;
+    s[D_RR] += ri * ri;
+    s[D_SR] += ri;
+    s[D_SW] += wi;
+  }
+  __shared__ double red[NDOT][VBLOCK / 64];
+#pragma unroll
+  for (int q = 0; q < NDOT; ++q) {
+    const double v = wave_sum(s[q]);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NDOT) {
+    double t = 0.0;
+    for (int k = 0; k < VBLOCK / 64; ++k) t += red[threadIdx.x][k];
+    part[threadIdx.x * DOT_BLOCKS + blockIdx.x] = t;
+  }
+}
+
+// Sums the NDOT x DOT_BLOCKS partials in a fixed order (one RED2_THREADS-thread workgroup:
+// one partial of every quantity per thread, all loads in flight together).
+static_assert(DOT_BLOCKS == RED2_THREADS, "one partial per thread");
+__device__ __forceinline__ void cg_sum_dots(const double* __restrict__ part, double* out) {
+  __shared__ double red[NDOT][RED2_THREADS / 64];
+  double v[NDOT];
+#pragma unroll
+  for (int q = 0; q < NDOT; ++q) v[q] = part[q * DOT_BLOCKS + threadIdx.x];
+#pragma unroll
+  for (int q = 0; q < NDOT; ++q) {
+    v[q] = wave_sum(v[q]);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v[q];
+  }
+  __syncthreads();
+  for (int q = 0; q < NDOT; ++q) {
+    double t = 0.0;
+    for (int k = 0; k < RED2_THREADS / 64; ++k) t += red[q][k];
+    out[q] = t;
+  }
+}
+
+// Projected dots into S (mu = 0 without a null space).
+__device__ __forceinline__ void cg_projected(const double* d, uint32_t n, int null_space, double* __restrict__ S) {
+  const double mu = null_space ? d[D_SZ] / double(n) : 0.0;
+  S[S_MU] = mu;
+  S[S_ZZ] = d[D_ZZ] - double(n) * mu * mu;
+  S[S_ZR] = d[D_ZR] - mu * d[D_SR];
+  S[S_ZW] = d[D_ZW] - mu * d[D_SW];
+  S[S_RR] = d[D_RR];
+}
+
+// after z = B r: projected dots, dp, the next p-update factor, rz (grid: one workgroup of
+// RED2_THREADS)
+static __global__ void __launch_bounds__(RED2_THREADS)
+cg_scalar_fused_kernel(const double* __restrict__ part, uint32_t n, int null_space, double* __restrict__ S,
+                       int precond_norm, int flexible) {
+  double d[NDOT];
+  cg_sum_dots(part, d);
+  if (threadIdx.x != 0) return;
+  cg_projected(d, n, null_space, S);
+  const double dp = sqrt(precond_norm ? S[S_ZZ] : S[S_RR]);
+  const double ratio = flexible ? (-S[S_ALPHA] * S[S_ZW]) / S[S_RZ] : S[S_ZR] / S[S_RZ];
+  S[S_DP] = dp;
+  S[S_RATIO] = ratio;
+  S[S_RZ] = S[S_ZR];
+}
+
+// before the first iteration (r = b projected, p = 0): rz, dp0, bb, ratio = 0
+static __global__ void __launch_bounds__(RED2_THREADS)
+cg_init_fused_kernel(const double* __restrict__ part, uint32_t n, int null_space, double* __restrict__ S,
+                     int precond_norm) {
+  double d[NDOT];
+  cg_sum_dots(part, d);
+  if (threadIdx.x != 0) return;
+  cg_projected(d, n, null_space, S);
+  S[S_BB] = d[D_RR];
+  S[S_RZ] = S[S_ZR];
+  S[S_DP] = sqrt(precond_norm ? S[S_ZZ] : S[S_BB]);
+  S[S_RATIO] = 0.0;
+}
+
+// alpha = rz / pw with pw the sum of the operator kernel's tile partials (one workgroup)
+static __global__ void __launch_bounds__(RED2_THREADS)
+cg_alpha_kernel(int ntiles, const double* __restrict__ part, double* __restrict__ S) {
+  __shared__ double red[RED2_THREADS / 64];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < ntiles; k += RED2_THREADS) v += part[k];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < RED2_THREADS / 64; ++k) t += red[k];
+    S[S_PW] = t;
+    S[S_ALPHA] = S[S_RZ] / t;
+  }
+}
+
+// p = (z - mu) + beta p  (the projected z with VecAYPX)
+template <class TZ>
+__global__ void cg_p_fused_kernel(uint32_t n, const double* __restrict__ S, const TZ* __restrict__ z,
+                                  double* __restrict__ p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (double(z[i]) - S[S_MU]) + S[S_RATIO] * p[i];
+}
+
+// x = x + alpha p;  r = r + (-alpha) w;  r32 = float(r) when given (the mixed V-cycle input)
+static __global__ void cg_xr_fused_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ p,
+                                          const double* __restrict__ w, double* __restrict__ x,
+                                          double* __restrict__ r, float* __restrict__ r32) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = S[S_ALPHA];
+  x[i] = x[i] + a * p[i];
+  const double ri = r[i] + (-a) * w[i];
+  r[i] = ri;
+  if (r32) r32[i] = float(ri);
 }
 
 // after z = B r and the dots: dp, the next p-update factor, rz
@@ -149,44 +289,120 @@ __global__ void copy_kernel(uint32_t n, const T* __restrict__ a, T* __restrict__
   if (i < n) b[i] = a[i];
 }
 
-// x = Ainv b for a dense symmetric n x n inverse stored row-major with row pitch ld (a
-// multiple of DENSE_PAD elements, 16-byte aligned rows), one wavefront per row.  Each lane
-// streams 16-byte vectors, four in flight; sums in FP64 in a fixed order.
+// Row pitch of the dense coarse inverse: a multiple of DENSE_PAD elements (16-byte aligned
+// rows for the batched coarse solve's vector loads).
 constexpr int DENSE_PAD = 8;
+
+// Symmetric coarse inverse in packed lower SB x SB blocks: block (I, J), J <= I, stored
+// row-major at sym_block_off(I, J), zero beyond n (diagonal blocks hold both triangles).
+// y = A b reads each stored block once: block (I, J) contributes A_IJ b_J to the rows of
+// block row I and, off the diagonal, A_IJ^T b_I to the rows of block row J.  The
+// contributions go to partial sums Q[K][i] (K: the block index of the other factor), each
+// (K, i) written exactly once, and y_i = sum_K Q[K][i] in a fixed order.
+constexpr int SB = 64;
+__host__ __device__ inline size_t sym_block_off(int I, int J) { return (size_t(I) * (I + 1) / 2 + J) * SB * SB; }
+
+// (I, J) of packed block number blk = I (I + 1) / 2 + J
+__device__ __forceinline__ void sym_block_ij(int blk, int& I, int& J) {
+  I = int((sqrt(8.0 * blk + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= blk) ++I;
+  while (I * (I + 1) / 2 > blk) --I;
+  J = blk - I * (I + 1) / 2;
+}
+
+// P <- packed lower blocks of the full symmetric A (row pitch ld); grid: one workgroup per
+// packed block
+template <class T>
+__global__ void __launch_bounds__(256) sym_pack_kernel(int n, int ld, const T* __restrict__ A, T* __restrict__ P) {
+  int I, J;
+  sym_block_ij(blockIdx.x, I, J);
+  T* dst = P + sym_block_off(I, J);
+  for (int idx = threadIdx.x; idx < SB * SB; idx += 256) {
+    const int r = I * SB + idx / SB, c = J * SB + idx % SB;
+    dst[idx] = (r < n && c < n) ? A[size_t(r) * ld + c] : T(0);
+  }
+}
+
+// Q[K][npad] <- block contributions; grid: one 128-thread workgroup per packed block.
+// Wavefront 0 forms the row sums A_IJ b_J (lane = row), wavefront 1 the column sums
+// A_IJ^T b_I (lane = column), both from the block staged in LDS (rows padded to SB + 1).
+template <class T>
+__global__ void __launch_bounds__(128)
+sym_block_kernel(int n, int npad, const T* __restrict__ P, const T* __restrict__ b, double* __restrict__ Q) {
+  __shared__ T s[SB][SB + 1];
+  __shared__ T bi[SB], bj[SB];
+  int I, J;
+  sym_block_ij(blockIdx.x, I, J);
+  const T* src = P + sym_block_off(I, J);
+#pragma unroll 8
+  for (int idx = threadIdx.x; idx < SB * SB; idx += 128) s[idx / SB][idx % SB] = src[idx];
+  if (threadIdx.x < SB) {
+    const int r = I * SB + threadIdx.x;
+    bi[threadIdx.x] = r < n ? b[r] : T(0);
+  } else {
+    const int c = J * SB + threadIdx.x - SB;
+    bj[threadIdx.x - SB] = c < n ? b[c] : T(0);
+  }
+  __syncthreads();
+  if (threadIdx.x < SB) {
+    const int r = threadIdx.x;
+    double u = 0.0;
+#pragma unroll 16
+    for (int c = 0; c < SB; ++c) u += double(s[r][c]) * double(bj[c]);
+    Q[size_t(J) * npad + size_t(I) * SB + r] = u;
+  } else if (I != J) {
+    const int c = threadIdx.x - SB;
+    double v = 0.0;
+#pragma unroll 16
+    for (int r = 0; r < SB; ++r) v += double(s[r][c]) * double(bi[r]);
+    Q[size_t(I) * npad + size_t(J) * SB + c] = v;
+  }
+}
+
+// y_i = sum_{K < nb} Q[K][i] - mu c1_i: one 256-thread workgroup per 64 rows, the four
+// wavefronts take every fourth K and combine in LDS (fixed order).  With a constant null
+// space (c1 != nullptr) the right-hand side is projected inside the solve:
+// Ainv (b - mu 1) = Ainv b - mu Ainv 1 with mu = mean(b) and c1 = Ainv 1 (setup); every
+// workgroup forms mu from b in the same fixed order.
 template <class T>
 __global__ void __launch_bounds__(256)
-dense_apply_kernel(int n, int ld, const T* __restrict__ Ainv, const T* __restrict__ b, T* __restrict__ x) {
-  constexpr int VEC = 16 / sizeof(T);
-  using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;
-  const V* a = reinterpret_cast<const V*>(Ainv + size_t(row) * ld);
-  const V* bv = reinterpret_cast<const V*>(b);
-  const int nv = n / VEC;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int v0 = lane; v0 < nv; v0 += 256) {
-    V av[4], xv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int v = v0 + 64 * u;
-      if (v < nv) { av[u] = a[v]; xv[u] = bv[v]; }
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (v0 + 64 * u < nv) {
-        const T* pa = reinterpret_cast<const T*>(&av[u]);
-        const T* px = reinterpret_cast<const T*>(&xv[u]);
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) s[u] += double(pa[c]) * double(px[c]);
-      }
-    }
+sym_sum_kernel(int n, int nb, int npad, const double* __restrict__ Q, const T* __restrict__ b,
+               const double* __restrict__ c1, T* __restrict__ y) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double mu = 0.0;
+  if (c1) {
+    double sb = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) sb += double(b[j]);
+    sb = wave_sum(sb);
+    if (lane == 0) red[wv][0] = sb;
+    __syncthreads();
+    mu = ((red[0][0] + red[1][0]) + (red[2][0] + red[3][0])) / double(n);
+    __syncthreads();
   }
-  double t = (s[0] + s[1]) + (s[2] + s[3]);
-  const int j = nv * VEC + lane;    // tail
-  if (j < n) t += double(Ainv[size_t(row) * ld + j]) * double(b[j]);
+  const int i = blockIdx.x * 64 + lane;
+  double acc = 0.0;
+  if (i < npad) {
+#pragma unroll 4
+    for (int K = wv; K < nb; K += 4) acc += Q[size_t(K) * npad + i];
+  }
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && i < n) {
+    const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    y[i] = T(c1 ? v - mu * c1[i] : v);
+  }
+}
+
+// c1 = A 1 (row sums of the full symmetric A), one wavefront per row
+template <class T>
+__global__ void __launch_bounds__(256) row_sum_kernel(int n, int ld, const T* __restrict__ A, double* __restrict__ c1) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double t = 0.0;
+  for (int j = lane; j < n; j += 64) t += double(A[size_t(row) * ld + j]);
   t = wave_sum(t);
-  if (lane == 0) x[row] = T(t);
+  if (lane == 0) c1[row] = t;
 }
 
 // copy the lower triangle (column-major lower == row-major upper) to a full symmetric matrix
