@@ -82,53 +82,6 @@ __device__ __forceinline__ void load_rec(Rec& R, uint32_t g, const uint16_t* __r
   R.k2 = kappa ? (MASS ? double(mass[g]) : 0.0) : mass_fac;
 }
 
-// Evaluates one record with stencil triangles Ls/Ms (LDS or, when the stencil type is
-// uniform across the wavefront, global memory read through the scalar cache); row sums of
-// owned nodes are added into ys (accumulator type TA).
-template <class T, class TA, bool MASS, int MODE>
-__device__ __forceinline__ void eval_rec_with(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac,
-                                              const T* Ls, const T* Ms, const T* __restrict__ xs,
-                                              TA* __restrict__ ys) {
-  T fac1, fac2;
-  elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
-  if (MODE != OP_DIAG) {
-    T xv[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) xv[j] = xs[R.l[j]];
-    // one pass over the 36 triangle entries: each stencil value is read once and feeds
-    // both rows it belongs to; the element factors scale the row sums at the end
-    T a[8], bm[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      a[k] = T(0);
-      bm[k] = T(0);
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      a[k] += Ls[tri(k, k)] * xv[k];
-      if (MASS) bm[k] += Ms[tri(k, k)] * xv[k];
-#pragma unroll
-      for (int j = k + 1; j < 8; ++j) {
-        const T l = Ls[tri(k, j)];
-        a[k] += l * xv[j];
-        a[j] += l * xv[k];
-        if (MASS) {
-          const T m = Ms[tri(k, j)];
-          bm[k] += m * xv[j];
-          bm[j] += m * xv[k];
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (R.l[k] < n_own) atomic_add(&ys[R.l[k]], TA(MASS ? fac1 * a[k] + fac2 * bm[k] : fac1 * a[k]));
-  } else {
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (R.l[k] < n_own) atomic_add(&ys[R.l[k]], TA(fac1 * Ls[tri(k, k)] + (MASS ? fac2 * Ms[tri(k, k)] : T(0))));
-  }
-}
-
 // Full stencil rows for the single right-hand-side kernel: per child-0 type, the Laplacian
 // rows [8][8] then the mass rows [8][8].
 constexpr int KFSTRIDE = 128;
@@ -176,6 +129,47 @@ template <class T>
 __device__ __forceinline__ double fx_value(fx_acc_t<T> s, int es) {
   return sizeof(T) == 8 ? ldexp(double(static_cast<long long>(s)), -es)
                         : ldexp(double(static_cast<int>(static_cast<unsigned int>(s))), -es);
+}
+
+// Batched kernels: evaluates one record with stencil triangles Ls/Ms held in scalar
+// registers (one pass over the 36 triangle entries: each stencil value feeds both rows it
+// belongs to) and adds the rows of owned nodes, scaled by 2^es, to the fixed-point sums.
+template <class T, bool MASS, int MODE>
+__device__ __forceinline__ void eval_rec_tri(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac, int es,
+                                             const T* Ls, const T* Ms, const T* __restrict__ xs,
+                                             fx_acc_t<T>* __restrict__ acc) {
+  T fac1, fac2;
+  elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
+  const T f1s = T(ldexp(double(fac1), es));
+  const T f2s = MASS ? T(ldexp(double(fac2), es)) : T(0);
+  T xv[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) xv[j] = xs[R.l[j]];
+  T a[8], bm[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = T(0);
+    bm[k] = T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] += Ls[tri(k, k)] * xv[k];
+    if (MASS) bm[k] += Ms[tri(k, k)] * xv[k];
+#pragma unroll
+    for (int j = k + 1; j < 8; ++j) {
+      const T l = Ls[tri(k, j)];
+      a[k] += l * xv[j];
+      a[j] += l * xv[k];
+      if (MASS) {
+        const T m = Ms[tri(k, j)];
+        bm[k] += m * xv[j];
+        bm[j] += m * xv[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (R.l[k] < n_own) atomicAdd(&acc[R.l[k]], fx_term(a[k], f1s, bm[k], f2s, MASS));
 }
 
 // Evaluates the rows of the owned vertices of one record and adds them, scaled by 2^es, to

@@ -41,10 +41,11 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 // the KSL lanes of an element share its record, read the element's 8 node values of their
 // column from LDS and accumulate its 8 rows; the stencil comes through the scalar cache
 // (per-type wavefront loop of eval_rows), so LDS carries only the minimum per element and
-// column: 8 reads and at most 8 additions.  Sums accumulate in FP64 LDS for both precisions
-// (FP32 LDS additions are several times slower on this device).  LDS rows are padded to
-// KSL + 1 values so the lanes of different elements spread over the banks.
-// LDS: owned sums [ys_len][KSL + 1] (double), touched x [max_touch][KSL + 1] (T).
+// column: 8 reads and at most 8 additions.  Sums accumulate in fixed point as in the single
+// right-hand-side kernel (64-bit integers for FP64, 32-bit for FP32), with one scale per
+// column of the slice (a thread works on one column throughout: gather, products, store).
+// LDS rows are padded to KSL + 1 values so the lanes of different elements spread over the
+// banks.  LDS: owned sums [ys_len][KSL + 1] (fixed point), touched x [max_touch][KSL + 1] (T).
 template <class T>
 __host__ __device__ constexpr int multi_row(int ksl) { return ksl + 1; }
 
@@ -54,12 +55,15 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
                   const uint16_t* __restrict__ lid, const uint16_t* __restrict__ meta,
                   const T* __restrict__ kappa, const T* __restrict__ mass, double lap_fac,
                   double mass_fac, uint32_t maxD, double hFac, const T* __restrict__ K0,
-                  const uint32_t* __restrict__ node_tab, uint32_t ys_len, MultiIO<T> io) {
+                  const uint32_t* __restrict__ node_tab, const int2* __restrict__ texp, uint32_t ys_len,
+                  MultiIO<T> io) {
   static_assert(MODE != OP_DIAG, "the diagonal does not depend on the right-hand side");
-  static_assert(64 % KSL == 0, "a wavefront holds whole elements");
-  using TA = double;
+  static_assert(64 % KSL == 0 && MB % KSL == 0, "a wavefront holds whole elements");
+  using TA = fx_acc_t<T>;
   constexpr int RS = multi_row<T>(KSL);      // padded LDS row
+  constexpr int NW = MB / 64;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ uint32_t s_xhi[NW][KSL];
   TA* ys = reinterpret_cast<TA*>(smem_raw);
   T* xs = reinterpret_cast<T*>(ys + size_t(ys_len) * RS);
   const int np = ntiles * nslices;
@@ -70,16 +74,27 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   const uint32_t* nodes = node_tab + rec.node_off;
   const uint32_t n_own = rec.n_own;
   const size_t k = size_t(io.k);
+  uint32_t xhi = 0;   // high word of max |x| of this thread's column (threadIdx.x % KSL)
   for (uint32_t i = threadIdx.x; i < rec.n_touch * KSL; i += MB) {
     const uint32_t nd = i / KSL, c = i % KSL;
-    xs[nd * RS + c] = io.x[size_t(nodes[nd]) * k + size_t(sl) * KSL + c];
+    const T v = io.x[size_t(nodes[nd]) * k + size_t(sl) * KSL + c];
+    xs[nd * RS + c] = v;
+    xhi = max(xhi, uint32_t(uint64_t(__double_as_longlong(double(v))) >> 32) & 0x7fffffffu);
   }
   for (uint32_t i = threadIdx.x; i < n_own * RS; i += MB) ys[i] = TA(0);
-  __syncthreads();
-  constexpr int EPW = 64 / KSL;               // elements per wavefront round
-  constexpr int STRIDE = (MB / 64) * EPW;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int es = lane / KSL, c = lane % KSL;
+#pragma unroll
+  for (int off = 32; off >= KSL; off >>= 1) xhi = max(xhi, uint32_t(__shfl_xor(int(xhi), off)));
+  if (lane < KSL) s_xhi[wv][lane] = xhi;
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < NW; ++w) xhi = max(xhi, s_xhi[w][c]);
+  // per column: max |x| < 2^ex, fixed-point scale 2^esc; a non-finite column gives NaN
+  const bool finite = (xhi >> 20) < 0x7ffu;
+  const int esc = fx_scale<T>(texp[t], max(int(xhi >> 20) - 1022, FX_XMIN));
+  constexpr int EPW = 64 / KSL;               // elements per wavefront round
+  constexpr int STRIDE = (MB / 64) * EPW;
   // the records of the next round are loaded while the current round is evaluated; the
   // stencil of the wavefront's current type stays in scalar registers across rounds
   Rec Rn;
@@ -113,7 +128,7 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
         Rec Q = R;
 #pragma unroll
         for (int q = 0; q < 8; ++q) Q.l[q] *= RS;
-        eval_rec_with<T, TA, MASS, MODE>(Q, n_own * RS, maxD, hFac, Lr, Mr, xs + c, ys + c);
+        eval_rec_tri<T, MASS, MODE>(Q, n_own * RS, maxD, hFac, esc, Lr, Mr, xs + c, ys + c);
         done = true;
       }
     }
@@ -122,7 +137,7 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   for (uint32_t i = threadIdx.x; i < n_own * KSL; i += MB) {
     const uint32_t nd = i / KSL, cc = i % KSL;
     const size_t gi = size_t(nodes[nd]) * k + size_t(sl) * KSL + cc;
-    const T v = T(ys[nd * RS + cc]);
+    const T v = finite ? T(fx_value<T>(ys[nd * RS + cc], esc)) : T(__builtin_nan(""));   // cc == c
     if (MODE == OP_MATVEC) {
       io.y[gi] = v;
     } else if (MODE == OP_RESID) {

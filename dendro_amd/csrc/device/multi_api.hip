@@ -20,12 +20,13 @@ template <> const double* dm<double>(const DevTiles& d) { return d.rm64.p; }
 template <> const float* dk<float>(const DevTiles& d) { return d.rk32.p; }
 template <> const float* dm<float>(const DevTiles& d) { return d.rm32.p; }
 
-// LDS bytes of one batched workgroup: FP64 sums of the owned nodes and the staged x of the
-// touched nodes, rows of a slice padded by one value (vector-ALU kernel: FP32 slices of 8
-// columns, FP64 slices of 4); the matrix-core variant adds the full stencil table.
+// LDS bytes of one batched workgroup: fixed-point sums of the owned nodes (vector-ALU kernel;
+// FP64 sums in the matrix-core variant) and the staged x of the touched nodes, rows of a
+// slice padded by one value (vector-ALU kernel: FP32 slices of 8 columns, FP64 slices of 4);
+// the matrix-core variant adds the full stencil table.
 template <class T, int KSL>
 static size_t multi_lds_valu(const DevTiles& d) {
-  return size_t(multi_row<T>(KSL)) * (sizeof(double) * d.ys_len + sizeof(T) * d.plan.max_touch);
+  return size_t(multi_row<T>(KSL)) * (sizeof(fx_acc_t<T>) * d.ys_len + sizeof(T) * d.plan.max_touch);
 }
 static size_t multi_lds_mfma(const DevTiles& d, bool mass) {
   return sizeof(double) * KS * d.ys_len + sizeof(float) * KS * d.plan.max_touch +
@@ -61,7 +62,7 @@ static void launch_valu(dgh_mesh m, const DevTiles& d, const T* kap, const T* ma
   const size_t lds = multi_lds_valu<T, KSL>(d);
   hipLaunchKernelGGL((tile_multi_kernel<T, MASS, MODE, KSL, MB>), dim3(grid), dim3(MB), lds, st,
                      d.tiles.p, nt, io.k / KSL, d.lid.p, d.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
-                     m->hFac, Ktab<T>(), d.node_tab.p, d.ys_len, io);
+                     m->hFac, Ktab<T>(), d.node_tab.p, (const int2*)d.texp.p, d.ys_len, io);
 }
 
 template <bool MASS, int MODE>

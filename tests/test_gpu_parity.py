@@ -223,3 +223,33 @@ def test_pcg_graph_replays_match_eager(dev, hier_small):
         assert it1 == it0
         assert rel_err(x1, x0) <= 1e-12
         assert np.allclose(h1, h0, rtol=1e-9, atol=0)
+
+
+def test_pcg_graph_recaptured_after_retiling(dev, hier_small):
+    """A captured CG iteration holds the meshes' device buffers: re-tiling a mesh or setting
+    new coefficients after a solve must not replay a stale graph (the solver re-captures)."""
+    h = hier_small
+    coefs = var_coefficients(h, mass_base=1.0)
+    ms, ts = _meshes(h, coefs)
+    b = _t(np.random.default_rng(32).uniform(-1, 1, h.fine.n_node), dev)
+    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=50, rtol=1e-10,
+                          norm_type=da.NORM_UNPRECONDITIONED, use_graph=1, vcycle_prec=da.F32)
+    s = da.Solver(ms, ts, p)
+    x0 = torch.zeros_like(b)
+    st0, _ = s.solve(b, x0)
+    ms[-1].set_tile(128, 256)            # new tiling of the finest operator
+    ms[-2].set_tile(256, 512)
+    x1 = torch.zeros_like(b)
+    st1, _ = s.solve(b, x1)
+    assert st1.converged and st1.iterations == st0.iterations
+    assert rel_err(_np(x1), _np(x0)) <= 1e-6
+    ms[-1].set_coefficients(coefs[-1][0] * 2.0, coefs[-1][1] * 2.0)   # finest operator doubled
+    x2 = torch.zeros_like(b)
+    st2, _ = s.solve(b, x2)
+    assert st2.converged
+    # the outer CG sees the doubled finest operator: its solution is close to half of x0
+    # (the V-cycle keeps the old coarse levels, so only the outer operator is exact)
+    r = torch.empty_like(b)
+    ms[-1].matvec(x2, r)
+    torch.cuda.synchronize()
+    assert rel_err(_np(r), _np(b)) <= 1e-8
