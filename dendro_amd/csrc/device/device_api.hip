@@ -81,12 +81,14 @@ static void launch_smooth(dgh_mesh m, const T* x, const T* b, const T* invd, T o
   launch_op<T, OP_SMOOTH>(m, OpIO<T>{x, x_out, b, invd, omega}, st);
 }
 
+// c = R f; with x1 given, also x1 = omega (c * invd) (the first sweep on the coarse level)
 template <class T>
-static void launch_restrict(dgh_transfer t, const T* f, T* c, hipStream_t st) {
+static void launch_restrict(dgh_transfer t, const T* f, T* c, hipStream_t st, T omega = T(0),
+                            const T* invd = nullptr, T* x1 = nullptr) {
   const uint32_t nc = t->coarse->N;
   if (!nc) return;
   hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(size_t(nc) * RLANES)), dim3(VBLOCK), 0, st, nc,
-                     t->rowptr.p, t->col.p, t->w.p, f, c);
+                     t->rowptr.p, t->col.p, t->w.p, f, c, omega, invd, x1);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -231,9 +233,12 @@ static void coarse_apply(dgh_solver s, const T* b, T* x, hipStream_t st) {
 // PCMG multiplicative V-cycle on level l for right-hand side b (PCMGMCycle_Private with the
 // options of bin/options:182-206): pre-smoothing from a zero guess, residual, restriction,
 // recursion (coarsest: direct solve), interpolate-add, post-smoothing.  Returns the buffer
-// holding the result (one of the level's two iterate buffers).
+// holding the result (one of the level's two iterate buffers).  The first sweep from zero,
+// x = omega (b * invd), needs no operator: with x1_ready the kernel that produced b has
+// already written it into the level's first iterate buffer (the restriction does so for
+// every level below the top).
 template <class T>
-static T* vcycle(dgh_solver s, int l, const T* b, hipStream_t st) {
+static T* vcycle(dgh_solver s, int l, const T* b, hipStream_t st, bool x1_ready = false) {
   auto& VV = vecs<T>(s);
   auto& V = VV[l];
   if (l == 0) {
@@ -245,16 +250,20 @@ static T* vcycle(dgh_solver s, int l, const T* b, hipStream_t st) {
   const T om = T(s->om_level[l]);
   T* xc = V.xa.p;
   T* xn = V.xb.p;
-  // pre-smoothing: the first sweep from x = 0 needs no operator application
-  hipLaunchKernelGGL(smooth_first_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, om, b,
-                     (const T*)V.invd.p, xc);
+  if (!x1_ready)
+    hipLaunchKernelGGL(smooth_first_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, om, b,
+                       (const T*)V.invd.p, xc);
   for (int it = 1; it < s->prm.nu_pre; ++it) {
     launch_smooth<T>(m, xc, b, V.invd.p, om, xn, st);
     std::swap(xc, xn);
   }
   launch_resid<T>(m, xc, b, V.r.p, st);
-  launch_restrict<T>(s->transfers[l - 1], V.r.p, VV[l - 1].b.p, st);
-  T* xcoarse = vcycle<T>(s, l - 1, VV[l - 1].b.p, st);
+  auto& C = VV[l - 1];
+  if (l - 1 > 0 && s->prm.nu_pre > 0)
+    launch_restrict<T>(s->transfers[l - 1], V.r.p, C.b.p, st, T(s->om_level[l - 1]), C.invd.p, C.xa.p);
+  else
+    launch_restrict<T>(s->transfers[l - 1], V.r.p, C.b.p, st);
+  T* xcoarse = vcycle<T>(s, l - 1, C.b.p, st, l - 1 > 0 && s->prm.nu_pre > 0);
   launch_prolong<T>(s->transfers[l - 1], xcoarse, xc, xc, st);
   for (int it = 0; it < s->prm.nu_post; ++it) {
     launch_smooth<T>(m, xc, b, V.invd.p, om, xn, st);
@@ -281,10 +290,10 @@ static void precondition(dgh_solver s, const double* r, double* z, hipStream_t s
 
 // z = B r on the finest level, unprojected, in the V-cycle's precision (the buffer is the
 // same on every call).  Mixed: the FP32 copy of r must already sit in the level's b vector.
-static const void* precondition_raw(dgh_solver s, const double* r, hipStream_t st) {
+static const void* precondition_raw(dgh_solver s, const double* r, hipStream_t st, bool x1_ready = false) {
   const int top = s->nlevels - 1;
-  if (s->prm.vcycle_prec == DGH_F32) return vcycle<float>(s, top, s->v32[top].b.p, st);
-  return vcycle<double>(s, top, r, st);
+  if (s->prm.vcycle_prec == DGH_F32) return vcycle<float>(s, top, s->v32[top].b.p, st, x1_ready);
+  return vcycle<double>(s, top, r, st, x1_ready);
 }
 
 template <int BLOCK>
@@ -335,9 +344,16 @@ static void cg_iteration(dgh_solver s, hipStream_t st) {
   launch_cg(m, s->p.p, s->w.p, s->ptile.p, st);
   hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED2_THREADS), 0, st, int(m->op.plan.tiles.size()),
                      (const double*)s->ptile.p, S);
-  hipLaunchKernelGGL(cg_xr_fused_kernel, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
-                     (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p, mixed ? s->v32[top].b.p : nullptr);
-  const void* z = precondition_raw(s, s->r.p, st);
+  const bool x1 = s->nlevels > 1 && s->prm.nu_pre > 0;   // first sweep of the V-cycle fused
+  if (mixed)
+    hipLaunchKernelGGL(cg_xr_fused_kernel<float>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
+                       (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p, s->v32[top].b.p,
+                       float(s->om_level[top]), (const float*)s->v32[top].invd.p, x1 ? s->v32[top].xa.p : s->v32[top].xb.p);
+  else
+    hipLaunchKernelGGL(cg_xr_fused_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
+                       (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p, (float*)nullptr,
+                       s->om_level[top], (const double*)s->v64[top].invd.p, x1 ? s->v64[top].xa.p : s->v64[top].xb.p);
+  const void* z = precondition_raw(s, s->r.p, st, x1);
   if (z != s->zres) fail(DGH_ERR_STATE, "V-cycle result buffer changed");
   if (mixed)
     hipLaunchKernelGGL(cg_dots_kernel<float>, dim3(DOT_BLOCKS), dim3(VBLOCK), 0, st, n, static_cast<const float*>(z),

@@ -390,10 +390,13 @@ prolong_kernel(uint32_t nf, const uint32_t* __restrict__ celem, const uint8_t* _
 // atomics).  RLANES lanes share a row (rows hold about 4-30 entries) and combine their
 // partial sums with shuffles; the sum order is fixed.
 constexpr int RLANES = 8;
+// With x1 given, also the coarse level's first smoothing sweep from a zero guess:
+// x1 = omega (c * invd), as smooth_first_kernel forms it.
 template <class T>
 __global__ void __launch_bounds__(256)
 restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
-                const float* __restrict__ w, const T* __restrict__ f, T* __restrict__ c) {
+                const float* __restrict__ w, const T* __restrict__ f, T* __restrict__ c, T omega = T(0),
+                const T* __restrict__ invd = nullptr, T* __restrict__ x1 = nullptr) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t i = gid / RLANES, lane = gid % RLANES;
   T acc = T(0);
@@ -403,7 +406,10 @@ restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t
   }
 #pragma unroll
   for (int off = RLANES / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, RLANES);
-  if (i < nc && lane == 0) c[i] = acc;
+  if (i < nc && lane == 0) {
+    c[i] = acc;
+    if (x1) x1[i] = T(0) + omega * (acc * invd[i]);
+  }
 }
 
 template <class T>

@@ -252,10 +252,13 @@ __global__ void cg_p_fused_kernel(uint32_t n, const double* __restrict__ S, cons
   if (i < n) p[i] = (double(z[i]) - S[S_MU]) + S[S_RATIO] * p[i];
 }
 
-// x = x + alpha p;  r = r + (-alpha) w;  r32 = float(r) when given (the mixed V-cycle input)
-static __global__ void cg_xr_fused_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ p,
-                                          const double* __restrict__ w, double* __restrict__ x,
-                                          double* __restrict__ r, float* __restrict__ r32) {
+// x = x + alpha p;  r = r + (-alpha) w;  then the V-cycle's first smoothing sweep from zero,
+// x1 = omega (r invd), in the V-cycle's precision TV (mixed: also the FP32 copy of r, r32)
+template <class TV>
+__global__ void cg_xr_fused_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ p,
+                                   const double* __restrict__ w, double* __restrict__ x, double* __restrict__ r,
+                                   float* __restrict__ r32, TV omega, const TV* __restrict__ invd,
+                                   TV* __restrict__ x1) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double a = S[S_ALPHA];
@@ -263,6 +266,7 @@ static __global__ void cg_xr_fused_kernel(uint32_t n, const double* __restrict__
   const double ri = r[i] + (-a) * w[i];
   r[i] = ri;
   if (r32) r32[i] = float(ri);
+  x1[i] = TV(0) + omega * (TV(ri) * invd[i]);
 }
 
 // after z = B r and the dots: dp, the next p-update factor, rz

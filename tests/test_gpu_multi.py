@@ -67,6 +67,39 @@ def test_matvec_multi_matches_oracle(dev, hier_small, k, tile, mass, variant):
     assert worst32 <= 1e-5
 
 
+@pytest.mark.parametrize("prec", [da.F64, da.F32])
+def test_matvec_multi_column_scales(dev, hier_small, prec):
+    """The vector-ALU batched operator scales its fixed-point sums per column: columns of
+    very different magnitude in one slice keep their own precision (each column equals the
+    single-column product bit for bit up to its power-of-two scale), and a NaN in one column
+    leaves the other columns finite."""
+    h = hier_small
+    F = h.fine
+    c = var_coefficients(h, mass_base=1.0)
+    m = da.Mesh(F)
+    m.set_coefficients(c[-1][0], c[-1][1])
+    m.set_tile_multi(512, 0)
+    k = 8
+    dt = torch.float64 if prec == da.F64 else torch.float32
+    base = np.random.default_rng(5).uniform(-1, 1, F.n_node)
+    s = 200 if prec == da.F64 else 40
+    scales = [2.0 ** (s * ((j % 3) - 1)) for j in range(k)]   # 2^-s, 1, 2^s, ...
+    X = torch.tensor(np.stack([base * sc for sc in scales], axis=1), dtype=dt, device=dev).contiguous()
+    Y = torch.empty_like(X)
+    m.matvec_multi(X, Y, k, prec=prec)
+    torch.cuda.synchronize()
+    for j in range(1, k):   # same column up to the power of two: bit-identical after rescaling
+        assert torch.equal(Y[:, j] / scales[j], Y[:, 0] / scales[0]), j
+    ref = oracle.matvec(F, base, kappa=c[-1][0], mass=c[-1][1])
+    assert rel_err(_np(Y[:, 1]), ref) <= (1e-12 if prec == da.F64 else 1e-5)
+    X[F.n_node // 3, 2] = float("nan")
+    m.matvec_multi(X, Y, k, prec=prec)
+    torch.cuda.synchronize()
+    assert torch.isnan(Y[F.n_node // 3, 2])
+    for j in (0, 1, 3, 7):
+        assert torch.isfinite(Y[:, j]).all(), j
+
+
 def test_matvec_multi_rejects_bad_k(dev, hier_tiny):
     m = da.Mesh(hier_tiny.fine)
     x = torch.zeros(hier_tiny.fine.n_node * 12, dtype=torch.float64, device=dev)

