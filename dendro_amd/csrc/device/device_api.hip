@@ -87,8 +87,12 @@ static void launch_restrict(dgh_transfer t, const T* f, T* c, hipStream_t st, T 
                             const T* invd = nullptr, T* x1 = nullptr) {
   const uint32_t nc = t->coarse->N;
   if (!nc) return;
-  hipLaunchKernelGGL(restrict_kernel<T>, dim3(blocks_for(size_t(nc) * RLANES)), dim3(VBLOCK), 0, st, nc,
-                     t->rowptr.p, t->col.p, t->w.p, f, c, omega, invd, x1);
+  if (t->ent.p)
+    hipLaunchKernelGGL((restrict_kernel<T, true>), dim3(blocks_for(size_t(nc) * RLANES)), dim3(VBLOCK), 0, st, nc,
+                       t->rowptr.p, t->ent.p, (const float*)nullptr, f, c, omega, invd, x1);
+  else
+    hipLaunchKernelGGL((restrict_kernel<T, false>), dim3(blocks_for(size_t(nc) * RLANES)), dim3(VBLOCK), 0, st, nc,
+                       t->rowptr.p, t->col.p, t->w.p, f, c, omega, invd, x1);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -750,6 +754,16 @@ int dgh_transfer_create(dgh_mesh coarse, dgh_mesh fine, const uint32_t* coarse_e
     t->rowptr.upload(rowptr.data(), rowptr.size());
     t->col.upload(col.data(), col.size());
     t->w.upload(wv.data(), wv.size());
+    // packed entries (half the index traffic of the restriction) when every weight is a
+    // multiple of 1/32 in (0, 1] and the fine nodes fit in 27 bits
+    bool packable = nf <= RP_COL + 1u;
+    std::vector<uint32_t> ent(col.size());
+    for (size_t q = 0; q < col.size() && packable; ++q) {
+      const double k = double(wv[q]) * 32.0;
+      packable = (k == std::floor(k) && k >= 1.0 && k <= 32.0);
+      ent[q] = (uint32_t(k - 1.0) << RP_SHIFT) | col[q];
+    }
+    if (packable) t->ent.upload(ent.data(), ent.size());
     *out = t.release();
   });
 }

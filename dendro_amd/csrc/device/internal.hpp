@@ -138,6 +138,7 @@ struct dgh_transfer_s {
   DevBuf<uint8_t> pos;          // [N_f] position code inside it
   DevBuf<uint32_t> rowptr, col; // R = P^T in CSR (rows: coarse nodes)
   DevBuf<float> w;              // exact dyadic weights
+  DevBuf<uint32_t> ent;         // packed (32 w - 1) << 27 | col, when every entry fits
 };
 
 namespace dgh {

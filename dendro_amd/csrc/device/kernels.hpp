@@ -392,7 +392,11 @@ prolong_kernel(uint32_t nf, const uint32_t* __restrict__ celem, const uint8_t* _
 constexpr int RLANES = 8;
 // With x1 given, also the coarse level's first smoothing sweep from a zero guess:
 // x1 = omega (c * invd), as smooth_first_kernel forms it.
-template <class T>
+// PACKED: one 32-bit word per entry, (32 w - 1) << 27 | fine node (every weight is a
+// multiple of 1/32 in (0, 1]); otherwise col and w arrays.
+constexpr int RP_SHIFT = 27;
+constexpr uint32_t RP_COL = (1u << RP_SHIFT) - 1u;
+template <class T, bool PACKED = false>
 __global__ void __launch_bounds__(256)
 restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
                 const float* __restrict__ w, const T* __restrict__ f, T* __restrict__ c, T omega = T(0),
@@ -402,7 +406,14 @@ restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t
   T acc = T(0);
   if (i < nc) {
     const uint32_t p1 = rowptr[i + 1];
-    for (uint32_t p = rowptr[i] + lane; p < p1; p += RLANES) acc += T(w[p]) * f[col[p]];
+    for (uint32_t p = rowptr[i] + lane; p < p1; p += RLANES) {
+      if (PACKED) {
+        const uint32_t e = col[p];
+        acc += T(float((e >> RP_SHIFT) + 1u) * 0.03125f) * f[e & RP_COL];
+      } else {
+        acc += T(w[p]) * f[col[p]];
+      }
+    }
   }
 #pragma unroll
   for (int off = RLANES / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, RLANES);
