@@ -222,12 +222,16 @@ def main():
     log(f"MatVec FP64: {ms_per*1e3:.1f} us/launch, {gdofs:.2f} GDoF/s, {achieved:.0f} GB/s algorithmic "
         f"({100*achieved/HBM_PEAK_GBS:.1f}% of HBM peak); wall {wall:.3f}s")
     # The operator's data (about 110 MB) fits in the 256 MB Infinity Cache, so back-to-back
-    # launches read much of it from there.  Cold: a 1 GiB fill between launches evicts it and
-    # each launch is timed alone.
-    flush = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+    # launches read much of it from there.  Cold: reading a 1 GiB buffer between launches
+    # evicts it and each launch is timed alone.  The flush reads rather than writes: a fill
+    # would leave 1 GiB of dirty lines whose write-back lands inside the timed launch (a plain
+    # 110 MiB read then takes 40 us instead of 21).
+    flush = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+    for _ in range(2):
+        flush.sum()
     cold = []
     for it in range(12):
-        flush.fill_(float(it))
+        flush.sum()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         mf.matvec(x, y, stream=stream)

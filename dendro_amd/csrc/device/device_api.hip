@@ -31,8 +31,17 @@ template <> const double* rm<double>(dgh_mesh m) { return m->op.rm64.p; }
 template <> const float* rk<float>(dgh_mesh m) { return m->op.rk32.p; }
 template <> const float* rm<float>(dgh_mesh m) { return m->op.rm32.p; }
 
+// The kernel's LDS layout holds op_own_cap(BLOCK) owned nodes: the plan must be the one built
+// for that block size.
+template <int BLOCK>
+static void check_op_plan(dgh_mesh m) {
+  if (m->op.block != BLOCK || m->op.plan.own_cap != op_own_cap(BLOCK))
+    fail(DGH_ERR_STATE, "operator tiling built for block %d, launched with %d", m->op.block, BLOCK);
+}
+
 template <class T, int MODE, int BLOCK>
 static void launch_op_b(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
+  check_op_plan<BLOCK>(m);
   const int nt = int(m->op.plan.tiles.size());
   const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
   const size_t lds = op_lds_bytes<T>(m);
@@ -160,17 +169,20 @@ void upload_coefs(dgh_mesh m, DevTiles& d) {
 }
 
 // (Re)builds a tiling with the largest tile size <= T whose LDS need (lds(d)) fits; with
-// budget > 0 tiles also end before their owned plus touched nodes exceed the budget.
-void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&), uint32_t budget) {
+// budget > 0 tiles also end before their owned plus touched nodes exceed the budget, and with
+// block > 0 (single right-hand-side operator) the owned nodes get the fixed capacity
+// op_own_cap(block) of the kernel's LDS layout.
+void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&), uint32_t budget,
+                    int block) {
   ++m->gen;
+  const uint32_t own_cap = block > 0 ? op_own_cap(block) : 0u;
   for (;;) {
-    build_tiles(m->E, m->N, m->h_nl.data(), m->h_child.data(), m->h_hn.data(), m->h_lev.data(), T, d.plan, budget);
+    build_tiles(m->E, m->N, m->h_nl.data(), m->h_child.data(), m->h_hn.data(), m->h_lev.data(), T, d.plan, budget,
+                own_cap);
     d.ys_len = (d.plan.max_own + 1) & ~1u;
-    d.lds64 = d.lds32 = 0;
-    for (const TileRec& tr : d.plan.tiles) {
-      d.lds64 = std::max(d.lds64, 8 * (tr.n_own + tr.n_touch));
-      d.lds32 = std::max(d.lds32, 4 * (tr.n_own + tr.n_touch));
-    }
+    d.block = block;
+    d.lds64 = block > 0 ? uint32_t(tile_lds_bytes<double>(block, d.plan.max_touch)) : 0u;
+    d.lds32 = block > 0 ? uint32_t(tile_lds_bytes<float>(block, d.plan.max_touch)) : 0u;
     if (lds(d) <= size_t(g.max_lds) || T <= 32) break;
     T /= 2;
   }
@@ -195,7 +207,9 @@ static uint32_t op_budget(int block) {
 
 // (Re)builds the single right-hand-side tiling with tiles of at most T elements within the
 // block size's node budget.
-static void plan_mesh(dgh_mesh m, uint32_t T) { build_devtiles(m, m->op, T, op_lds_f64, op_budget(m->block)); }
+static void plan_mesh(dgh_mesh m, uint32_t T) {
+  build_devtiles(m, m->op, T, op_lds_f64, op_budget(m->block), m->block);
+}
 
 }  // namespace dgh
 
@@ -302,6 +316,7 @@ static const void* precondition_raw(dgh_solver s, const double* r, hipStream_t s
 
 template <int BLOCK>
 static void launch_cg_b(dgh_mesh m, const double* p, double* w, double* part, hipStream_t st) {
+  check_op_plan<BLOCK>(m);
   const int nt = int(m->op.plan.tiles.size());
   const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
   const size_t lds = m->op.lds64;

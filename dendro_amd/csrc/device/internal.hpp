@@ -99,7 +99,8 @@ namespace dgh {
 struct DevTiles {
   TilePlan plan;
   uint32_t ys_len = 0;          // owned-node slots per tile (>= max_own, even; batched kernels)
-  uint32_t lds64 = 0, lds32 = 0;  // single right-hand-side kernel: max per-tile LDS bytes
+  uint32_t lds64 = 0, lds32 = 0;  // single right-hand-side kernel: LDS bytes of a launch
+  int block = 0;                  // single right-hand-side kernel: block size of the plan
   DevBuf<TileRec> tiles;
   DevBuf<uint16_t> lid, meta;
   DevBuf<uint32_t> node_tab;
@@ -238,7 +239,8 @@ inline uint64_t mesh_generation(dgh_solver s) {
   for (dgh_mesh m : s->meshes) g += m->gen;
   return g;
 }
-void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&), uint32_t budget = 0);
+void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const DevTiles&), uint32_t budget = 0,
+                    int block = 0);
 }  // namespace dgh
 
 namespace dgh {

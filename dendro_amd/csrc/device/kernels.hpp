@@ -131,6 +131,42 @@ __device__ __forceinline__ double fx_value(fx_acc_t<T> s, int es) {
                         : ldexp(double(static_cast<int>(static_cast<unsigned int>(s))), -es);
 }
 
+// Fixed LDS layout of one operator tile.  tile_op_kernel declares no static LDS, so its
+// dynamic LDS starts at address 0 and each region sits at a compile-time byte offset.  The
+// sums start at address 0 and the staged x's offset rides in the immediate field of the ds
+// instructions: a record corner costs one address computation (its id times the element
+// size) for both its x read and its sum.
+//   [0, tile_xhi_off)             fixed-point sums of the owned nodes, op_own_cap(BLOCK) entries
+//   [tile_xhi_off, + 64)          high word of max |x|, one per wavefront (at most 16)
+//   [tile_xs_off, ...)            staged x of the touched nodes, owned first
+// The host plan keeps n_own <= op_own_cap(BLOCK) and op_own_cap(BLOCK) + n_touch within the
+// block size's node budget.
+__host__ __device__ constexpr uint32_t op_own_cap(int block) {
+  return block >= 1024 ? 4096u : block >= 512 ? 2048u : 1024u;
+}
+template <class T, int BLOCK>
+__host__ __device__ constexpr uint32_t tile_xhi_off() {
+  return uint32_t(sizeof(fx_acc_t<T>)) * op_own_cap(BLOCK);
+}
+template <class T, int BLOCK>
+__host__ __device__ constexpr uint32_t tile_xs_off() {
+  return tile_xhi_off<T, BLOCK>() + 64u;
+}
+template <class T>
+__host__ constexpr size_t tile_lds_bytes(int block, uint32_t max_touch) {
+  return sizeof(fx_acc_t<T>) * op_own_cap(block) + 64 + sizeof(T) * size_t(max_touch);
+}
+template <class U>
+using lds_ptr = __attribute__((address_space(3))) U*;
+template <class U>
+__device__ __forceinline__ lds_ptr<U> lds_at(uint32_t byte) {
+  return (lds_ptr<U>)byte;
+}
+template <class U>
+__device__ __forceinline__ void lds_add(lds_ptr<U> p, U v) {
+  __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+}
+
 // Batched kernels: evaluates one record with stencil triangles Ls/Ms held in scalar
 // registers (one pass over the 36 triangle entries: each stencil value feeds both rows it
 // belongs to) and adds the rows of owned nodes, scaled by 2^es, to the fixed-point sums.
@@ -177,10 +213,9 @@ __device__ __forceinline__ void eval_rec_tri(const Rec& R, uint32_t n_own, uint3
 // (16 or 32 SGPRs live): the wavefront walks over the distinct stencil types among its
 // lanes (one in class-pure wavefronts, two at the class boundaries of a tile) with a
 // wave-uniform row pointer.  The compiler computes a row only when some lane owns it.
-template <class T, bool MASS, int MODE>
+template <class T, bool MASS, int MODE, uint32_t XS>
 __device__ __forceinline__ void eval_rows(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac, int es,
-                                          const T* __restrict__ KF, const T* __restrict__ xs,
-                                          fx_acc_t<T>* __restrict__ acc) {
+                                          const T* __restrict__ KF) {
   T fac1, fac2;
   elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
   const T f1s = T(ldexp(double(fac1), es));
@@ -188,7 +223,7 @@ __device__ __forceinline__ void eval_rows(const Rec& R, uint32_t n_own, uint32_t
   T xv[8];
   if (MODE != OP_DIAG) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) xv[j] = xs[R.l[j]];
+    for (int j = 0; j < 8; ++j) xv[j] = lds_at<T>(XS)[R.l[j]];
   }
   const uint32_t tp = R.mt & 31u;
   bool done = false;
@@ -221,7 +256,7 @@ __device__ __forceinline__ void eval_rows(const Rec& R, uint32_t n_own, uint32_t
             for (int j = 1; j < 8; ++j) b += mr[j] * xv[j];
           }
         }
-        atomicAdd(&acc[R.l[k]], fx_term(a, f1s, b, f2s, MASS));
+        lds_add(lds_at<fx_acc_t<T>>(0) + R.l[k], fx_term(a, f1s, b, f2s, MASS));
       }
     }
     done = done || mine;
@@ -234,8 +269,8 @@ constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 // kappa/mass: per record (nullptr: uniform lap_fac / mass_fac).  MASS: mass term present.
 // texp[t]: exponent bound of tile t's contributions per unit |x| and log2 bound of the terms
 // per node (see FX_BITS).
-// LDS per tile: fixed-point sums [n_own] (u64 for FP64, u32 for FP32), then the staged x of
-// the touched nodes [n_touch] (T), owned nodes first.
+// LDS per tile: the fixed layout above (tile_xs_off), fixed-point sums u64 for FP64, u32 for
+// FP32; the plan must have been built with op_own_cap(BLOCK).
 template <class T, class TC, bool MASS, int MODE, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
@@ -245,15 +280,17 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
                const int2* __restrict__ texp, OpIO<T> io) {
   constexpr int NW = BLOCK / 64;
   const T* __restrict__ x = io.x;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ uint32_t s_xhi[NW];
-  fx_acc_t<T>* acc = reinterpret_cast<fx_acc_t<T>*>(smem_raw);
+  static_assert(NW <= 16, "one max |x| word per wavefront in 64 LDS bytes");
+  constexpr uint32_t XS = tile_xs_off<T, BLOCK>();
+  static_assert(XS <= 65535u, "staged x must start within the ds immediate offset range");
+  const lds_ptr<fx_acc_t<T>> acc = lds_at<fx_acc_t<T>>(0);
+  const lds_ptr<T> xs = lds_at<T>(XS);
+  const lds_ptr<uint32_t> s_xhi = lds_at<uint32_t>(tile_xhi_off<T, BLOCK>());
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t >= ntiles) return;
   const TileRec rec = tiles[t];
   const uint32_t* nodes = node_tab + rec.node_off;
   const uint32_t n_own = rec.n_own;
-  T* xs = reinterpret_cast<T*>(acc + n_own);
   // first record: its loads are issued before the x gather so both chains overlap
   Rec R;
   if (threadIdx.x < rec.n_elem) load_rec<TC, MASS>(R, rec.elem_off + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac);
@@ -309,7 +346,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   const int ex = MODE == OP_DIAG ? 1 : max(int(xhi >> 20) - 1022, FX_XMIN);
   const int es = fx_scale<T>(texp[t], ex);
   for (uint32_t base = 0;;) {
-    if (base + threadIdx.x < rec.n_elem) eval_rows<T, MASS, MODE>(R, n_own, maxD, hFac, es, KF, xs, acc);
+    if (base + threadIdx.x < rec.n_elem) eval_rows<T, MASS, MODE, XS>(R, n_own, maxD, hFac, es, KF);
     base += BLOCK;
     if (base >= rec.n_elem) break;
     if (base + threadIdx.x < rec.n_elem)
@@ -349,7 +386,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   }
   if (MODE == OP_CG) {   // tile partial of p'.y, summed in a fixed order
     __syncthreads();
-    double* red = reinterpret_cast<double*>(smem_raw);
+    const lds_ptr<double> red = lds_at<double>(0);   // over the sums, dead past the barrier
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;

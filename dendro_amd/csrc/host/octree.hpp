@@ -54,6 +54,7 @@ struct TileRec {
 struct TilePlan {
   uint32_t tile_elems = 0;          // element cap per tile
   uint32_t budget = 0;              // cap on n_own + n_touch per tile (0: none)
+  uint32_t own_cap = 0;             // with budget: n_own <= own_cap, n_touch <= budget - own_cap
   std::vector<TileRec> tiles;
   std::vector<uint16_t> lid;        // [R][8] tile-local node ids, vertex order k^child
   std::vector<uint16_t> meta;       // [R] child-0 stencil type | level << 5
@@ -63,12 +64,14 @@ struct TilePlan {
   uint32_t max_slots = 0;           // max over tiles of n_own + n_touch
 };
 // Tiles of at most T consecutive elements; with budget > 0 a tile also ends before its
-// owned plus touched node count would exceed the budget (tiles then vary in length).
+// owned plus touched node count would exceed the budget (tiles then vary in length), and with
+// own_cap > 0 before it owns more than own_cap nodes or touches more than budget - own_cap
+// (the owned-node sums then sit in a fixed-size LDS region ahead of the staged x).
 void build_tiles(uint32_t E, uint32_t N, const uint32_t* nl, const uint8_t* child, const uint8_t* hn,
-                 const uint8_t* lev, uint32_t T, TilePlan& plan, uint32_t budget = 0);
+                 const uint8_t* lev, uint32_t T, TilePlan& plan, uint32_t budget = 0, uint32_t own_cap = 0);
 // Tile boundaries alone: bounds[t] = first element of tile t, bounds.back() = E.
 void tile_bounds(uint32_t E, uint32_t N, const uint32_t* nl, uint32_t T, uint32_t budget,
-                 std::vector<uint32_t>& bounds);
+                 std::vector<uint32_t>& bounds, uint32_t own_cap = 0);
 
 // Matching of coarse elements to fine elements (same element or 8 children), in order:
 // first fine element of each coarse element and whether it is the same element.

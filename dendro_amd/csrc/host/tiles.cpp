@@ -32,8 +32,10 @@ static uint8_t mirror_mask(uint8_t mask, int c) {
 // touching element it is; every later element touching such a node joins the tile's halo
 // (its nodes become touched too).  Owned and touched counts only grow, so the tile ends
 // before the first element whose addition would exceed the budget or the element cap.
+// With own_cap > 0 the owned nodes have a fixed capacity: a tile also ends before it owns
+// more than own_cap nodes or touches more than budget - own_cap.
 void tile_bounds(uint32_t E, uint32_t N, const uint32_t* nl, uint32_t T, uint32_t budget,
-                 std::vector<uint32_t>& bounds) {
+                 std::vector<uint32_t>& bounds, uint32_t own_cap) {
   bounds.clear();
   if (budget == 0) {
     for (uint32_t e = 0; e < E; e += T) bounds.push_back(e);
@@ -84,7 +86,8 @@ void tile_bounds(uint32_t E, uint32_t N, const uint32_t* nl, uint32_t T, uint32_
         for (uint32_t p = ptr[n]; p < ptr[n + 1]; ++p)
           if (adj[p] > b) touch_elem(adj[p]);
       }
-      if (own + touch > budget || touch > 65535u) break;
+      if (touch > 65535u) break;
+      if (own_cap ? (own > own_cap || own_cap + touch > budget) : (own + touch > budget)) break;
     }
     if (b == a) fail(DGH_ERR_ARG, "tile budget %u is smaller than one element's footprint", budget);
     a = b;
@@ -93,13 +96,15 @@ void tile_bounds(uint32_t E, uint32_t N, const uint32_t* nl, uint32_t T, uint32_
 }
 
 void build_tiles(uint32_t E, uint32_t N, const uint32_t* nl, const uint8_t* child, const uint8_t* hn,
-                 const uint8_t* lev, uint32_t T, TilePlan& plan, uint32_t budget) {
+                 const uint8_t* lev, uint32_t T, TilePlan& plan, uint32_t budget, uint32_t own_cap) {
   if (T < 64 || T > 16384) fail(DGH_ERR_ARG, "tile size %u out of range [64,16384]", T);
+  if (own_cap && own_cap >= budget) fail(DGH_ERR_ARG, "owned-node capacity %u not below the budget %u", own_cap, budget);
   plan = TilePlan();
   plan.tile_elems = T;
   plan.budget = budget;
+  plan.own_cap = own_cap;
   std::vector<uint32_t> bounds;
-  tile_bounds(E, N, nl, T, budget, bounds);
+  tile_bounds(E, N, nl, T, budget, bounds, own_cap);
   const uint32_t ntiles = uint32_t(bounds.size() - 1);
   std::vector<uint32_t> tile_of(E);
   for (uint32_t t = 0; t < ntiles; ++t)
