@@ -135,10 +135,10 @@ def bench_multi(args, meshes, trans, N, dev, stream, extra):
     return out
 
 
-def measured_traffic(tile_elems, block):
+def measured_traffic(kernel, tile_elems, block):
     """HBM bytes per FP64 MatVec launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     passes over this workload (already scaled by the copy-kernel calibration), when they were
-    taken with the same tiling; None otherwise."""
+    taken with the same kernel and tiling; None otherwise."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "tile_op_matvec_f64_lognormal1M_d12_traffic.json")
     try:
@@ -146,7 +146,7 @@ def measured_traffic(tile_elems, block):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("tile_elems") != tile_elems or t.get("block") != block:
+    if t.get("kernel") != kernel or t.get("tile_elems") != tile_elems or t.get("block") != block:
         return None
     return t.get("bytes_per_launch")
 
@@ -304,7 +304,8 @@ def main():
     if not args.no_multi:
         extra["multi_rhs"] = bench_multi(args, meshes, trans, N, dev, stream, extra)
 
-    traffic = measured_traffic(mf.tile_info()["tile_elems"], tuned["block"])
+    kernel = f"dgh::tile_op_kernel<double, double, false, 0, {tuned['block']}, false>"
+    traffic = measured_traffic(kernel, mf.tile_info()["tile_elems"], tuned["block"])
     out = {
         "metric": METRIC,
         "value": round(gdofs, 4),
@@ -324,7 +325,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg_bytes,
-                     "kernel": f"dgh::tile_op_kernel<double, double, false, 0, {tuned['block']}>",
+                     "kernel": kernel,
                      "tile_elems": mf.tile_info()["tile_elems"], "block": tuned["block"],
                      "n_tiles": mf.tile_info()["n_tiles"], "n_records": mf.tile_info()["n_records"],
                      "cold_cache_us": round(cold_ms * 1e3, 2),

@@ -47,14 +47,17 @@ static void launch_op_b(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
   const size_t lds = op_lds_bytes<T>(m);
   const T* kap = rk<T>(m);
   const T* mas = rm<T>(m);
-  if (kap && !mas)
-    hipLaunchKernelGGL((tile_op_kernel<T, T, false, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->op.tiles.p,
-                       nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, KFtab<T>(),
-                       m->op.node_tab.p, m->op.texp.p, io);
+#define DGH_OP_LAUNCH(MASS, UNI)                                                                                  \
+  hipLaunchKernelGGL((tile_op_kernel<T, T, MASS, MODE, BLOCK, UNI>), dim3(grid), dim3(BLOCK), lds, st, m->op.tiles.p, \
+                     nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, KFtab<T>(),  \
+                     m->op.node_tab.p, m->op.texp.p, io)
+  if (!kap)
+    DGH_OP_LAUNCH(true, true);
+  else if (!mas)
+    DGH_OP_LAUNCH(false, false);
   else
-    hipLaunchKernelGGL((tile_op_kernel<T, T, true, MODE, BLOCK>), dim3(grid), dim3(BLOCK), lds, st, m->op.tiles.p,
-                       nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, KFtab<T>(),
-                       m->op.node_tab.p, m->op.texp.p, io);
+    DGH_OP_LAUNCH(true, false);
+#undef DGH_OP_LAUNCH
   HIP_CHECK(hipGetLastError());
 }
 
@@ -114,7 +117,7 @@ static void launch_prolong(dgh_transfer t, const T* c, const T* f_in, T* f_out, 
   HIP_CHECK(hipGetLastError());
 }
 
-// Per-record coefficient copies of a tiling (they follow its record order), and the per-tile
+// Per-record element factors of a tiling (they follow its record order), and the per-tile
 // bounds of the fixed-point sums: texp[t].x = e with
 //   |fac1| max_k sum_j |L_t[k][j]| + |fac2| max_k sum_j |M_t[k][j]| < 2^e
 // over the tile's records, factors formed as the kernel forms them (elem_factors), and
@@ -158,11 +161,19 @@ void upload_coefs(dgh_mesh m, DevTiles& d) {
   if (!per_elem) return;
   std::vector<double> k(R);
   std::vector<float> kf(R);
-  for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = m->h_kappa[d.plan.rec_elem[r]]);
+  // per-record element factors, formed as elem_factors forms them
+  auto h_of = [&](size_t r) { return m->hFac * double(1u << (m->maxD - m->h_lev[d.plan.rec_elem[r]])); };
+  for (size_t r = 0; r < R; ++r) {
+    const double h = h_of(r);
+    kf[r] = float(k[r] = m->h_kappa[d.plan.rec_elem[r]] * h / 2.0);
+  }
   d.rk64.upload(k.data(), R);
   d.rk32.upload(kf.data(), R);
   if (!m->h_mass.empty()) {
-    for (size_t r = 0; r < R; ++r) kf[r] = float(k[r] = m->h_mass[d.plan.rec_elem[r]]);
+    for (size_t r = 0; r < R; ++r) {
+      const double h = h_of(r);
+      kf[r] = float(k[r] = m->h_mass[d.plan.rec_elem[r]] * h * h * h / 8.0);
+    }
     d.rm64.upload(k.data(), R);
     d.rm32.upload(kf.data(), R);
   }
@@ -323,14 +334,17 @@ static void launch_cg_b(dgh_mesh m, const double* p, double* w, double* part, hi
   const OpIO<double> io{p, w, nullptr, nullptr, 0.0, part};
   const double* kap = m->op.rk64.p;
   const double* mas = m->op.rm64.p;
-  if (kap && !mas)
-    hipLaunchKernelGGL((tile_op_kernel<double, double, false, OP_CG, BLOCK>), dim3(grid), dim3(BLOCK), lds, st,
-                       m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
-                       m->hFac, KFtab<double>(), m->op.node_tab.p, m->op.texp.p, io);
+#define DGH_CG_LAUNCH(MASS, UNI)                                                                               \
+  hipLaunchKernelGGL((tile_op_kernel<double, double, MASS, OP_CG, BLOCK, UNI>), dim3(grid), dim3(BLOCK), lds, st, \
+                     m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,     \
+                     m->hFac, KFtab<double>(), m->op.node_tab.p, m->op.texp.p, io)
+  if (!kap)
+    DGH_CG_LAUNCH(true, true);
+  else if (!mas)
+    DGH_CG_LAUNCH(false, false);
   else
-    hipLaunchKernelGGL((tile_op_kernel<double, double, true, OP_CG, BLOCK>), dim3(grid), dim3(BLOCK), lds, st,
-                       m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
-                       m->hFac, KFtab<double>(), m->op.node_tab.p, m->op.texp.p, io);
+    DGH_CG_LAUNCH(true, false);
+#undef DGH_CG_LAUNCH
   HIP_CHECK(hipGetLastError());
 }
 

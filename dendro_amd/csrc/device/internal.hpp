@@ -104,7 +104,7 @@ struct DevTiles {
   DevBuf<TileRec> tiles;
   DevBuf<uint16_t> lid, meta;
   DevBuf<uint32_t> node_tab;
-  DevBuf<double> rk64, rm64;    // per-record coefficients
+  DevBuf<double> rk64, rm64;    // per-record element factors (kappa h / 2, mass h^3 / 8)
   DevBuf<float> rk32, rm32;
   DevBuf<int2> texp;            // per tile: exponent bound of the contributions, log2 bound of the terms per node
 };

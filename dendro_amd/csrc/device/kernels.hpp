@@ -34,12 +34,13 @@ __device__ __forceinline__ void atomic_add(T* p, T v) {
 
 // Element factors exactly as the reference forms them (odaJac.C:271-276,
 // omgJac.C:1665-1672): h = hFac 2^(maxD - lev), fac1 = kappa h / 2, fac2 = mass h^3 / 8.
-template <class T>
+// Per-element coefficients arrive with the factors already formed per record by the host
+// (h is a power of two, so the products are exact either way); uniform ones are scaled here.
 __device__ __forceinline__ void elem_factors(uint32_t lev, uint32_t maxD, double hFac, double k1,
-                                             double k2, T& fac1, T& fac2) {
+                                             double k2, double& fac1, double& fac2) {
   const double h = hFac * double(1u << (maxD - lev));
-  fac1 = T(k1 * h / 2.0);
-  fac2 = T(k2 * h * h * h / 8.0);
+  fac1 = k1 * h / 2.0;
+  fac2 = k2 * h * h * h / 8.0;
 }
 
 // MATVEC: y = A x;  DIAG: y = diag(A);  RESID: y = b - A x;
@@ -64,22 +65,35 @@ __host__ __device__ constexpr int tri(int k, int j) {
 }
 constexpr int KSTRIDE = 72;   // per stencil type: Laplacian triangle then mass triangle
 
+// One record: tile-local ids of its 8 vertices, child-0 type | level << 5, and its element
+// factors (k1 = kappa h / 2, k2 = mass h^3 / 8).
 struct Rec {
   uint32_t l[8];
   uint32_t mt;
   double k1, k2;
 };
 
-template <class TC, bool MASS>
+// kappa/mass: per-record factors formed by the host, or nullptr for uniform coefficients
+// (lap_fac, mass_fac scaled by the element size here).  LSC scales the vertex ids (the
+// single right-hand-side kernel takes them as LDS byte offsets).  UNI: 1 uniform, 0 per
+// record, -1 decided by kappa at run time.
+template <class TC, bool MASS, uint32_t LSC = 1, int UNI = -1>
 __device__ __forceinline__ void load_rec(Rec& R, uint32_t g, const uint16_t* __restrict__ lid,
                                          const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
-                                         const TC* __restrict__ mass, double lap_fac, double mass_fac) {
+                                         const TC* __restrict__ mass, double lap_fac, double mass_fac,
+                                         uint32_t maxD, double hFac) {
   const uint4 lp = reinterpret_cast<const uint4*>(lid)[g];
-  R.l[0] = lp.x & 0xffffu; R.l[1] = lp.x >> 16; R.l[2] = lp.y & 0xffffu; R.l[3] = lp.y >> 16;
-  R.l[4] = lp.z & 0xffffu; R.l[5] = lp.z >> 16; R.l[6] = lp.w & 0xffffu; R.l[7] = lp.w >> 16;
+  R.l[0] = (lp.x & 0xffffu) * LSC; R.l[1] = (lp.x >> 16) * LSC;
+  R.l[2] = (lp.y & 0xffffu) * LSC; R.l[3] = (lp.y >> 16) * LSC;
+  R.l[4] = (lp.z & 0xffffu) * LSC; R.l[5] = (lp.z >> 16) * LSC;
+  R.l[6] = (lp.w & 0xffffu) * LSC; R.l[7] = (lp.w >> 16) * LSC;
   R.mt = meta[g];
-  R.k1 = kappa ? double(kappa[g]) : lap_fac;
-  R.k2 = kappa ? (MASS ? double(mass[g]) : 0.0) : mass_fac;
+  if (UNI < 0 ? kappa != nullptr : UNI == 0) {
+    R.k1 = double(kappa[g]);
+    R.k2 = MASS ? double(mass[g]) : 0.0;
+  } else {
+    elem_factors(R.mt >> 5, maxD, hFac, lap_fac, mass_fac, R.k1, R.k2);
+  }
 }
 
 // Full stencil rows for the single right-hand-side kernel: per child-0 type, the Laplacian
@@ -174,8 +188,7 @@ template <class T, bool MASS, int MODE>
 __device__ __forceinline__ void eval_rec_tri(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac, int es,
                                              const T* Ls, const T* Ms, const T* __restrict__ xs,
                                              fx_acc_t<T>* __restrict__ acc) {
-  T fac1, fac2;
-  elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
+  const T fac1 = T(R.k1), fac2 = T(R.k2);
   const T f1s = T(ldexp(double(fac1), es));
   const T f2s = MASS ? T(ldexp(double(fac2), es)) : T(0);
   T xv[8];
@@ -213,17 +226,16 @@ __device__ __forceinline__ void eval_rec_tri(const Rec& R, uint32_t n_own, uint3
 // (16 or 32 SGPRs live): the wavefront walks over the distinct stencil types among its
 // lanes (one in class-pure wavefronts, two at the class boundaries of a tile) with a
 // wave-uniform row pointer.  The compiler computes a row only when some lane owns it.
+// R.l holds LDS byte offsets (ids times sizeof(T)); own_bytes = n_own sizeof(T).
 template <class T, bool MASS, int MODE, uint32_t XS>
-__device__ __forceinline__ void eval_rows(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac, int es,
-                                          const T* __restrict__ KF) {
-  T fac1, fac2;
-  elem_factors<T>(R.mt >> 5, maxD, hFac, R.k1, R.k2, fac1, fac2);
+__device__ __forceinline__ void eval_rows(const Rec& R, uint32_t own_bytes, int es, const T* __restrict__ KF) {
+  const T fac1 = T(R.k1), fac2 = T(R.k2);
   const T f1s = T(ldexp(double(fac1), es));
   const T f2s = MASS ? T(ldexp(double(fac2), es)) : T(0);
   T xv[8];
   if (MODE != OP_DIAG) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) xv[j] = lds_at<T>(XS)[R.l[j]];
+    for (int j = 0; j < 8; ++j) xv[j] = *lds_at<T>(XS + R.l[j]);
   }
   const uint32_t tp = R.mt & 31u;
   bool done = false;
@@ -241,7 +253,7 @@ __device__ __forceinline__ void eval_rows(const Rec& R, uint32_t n_own, uint32_t
         lr[j] = Kt[k * 8 + j];
         if (MASS) mr[j] = Kt[64 + k * 8 + j];
       }
-      if (mine && R.l[k] < n_own) {
+      if (mine && R.l[k] < own_bytes) {
         T a, b = T(0);
         if (MODE == OP_DIAG) {
           a = lr[k];
@@ -256,7 +268,7 @@ __device__ __forceinline__ void eval_rows(const Rec& R, uint32_t n_own, uint32_t
             for (int j = 1; j < 8; ++j) b += mr[j] * xv[j];
           }
         }
-        lds_add(lds_at<fx_acc_t<T>>(0) + R.l[k], fx_term(a, f1s, b, f2s, MASS));
+        lds_add(lds_at<fx_acc_t<T>>(R.l[k]), fx_term(a, f1s, b, f2s, MASS));
       }
     }
     done = done || mine;
@@ -271,7 +283,10 @@ constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 // per node (see FX_BITS).
 // LDS per tile: the fixed layout above (tile_xs_off), fixed-point sums u64 for FP64, u32 for
 // FP32; the plan must have been built with op_own_cap(BLOCK).
-template <class T, class TC, bool MASS, int MODE, int BLOCK>
+// UNI: uniform coefficients (kappa and mass unused; lap_fac, mass_fac, maxD and hFac form the
+// factors), else per-record factors (lap_fac, mass_fac, maxD and hFac unused).  The two are
+// separate instantiations so that neither keeps the other's arguments in scalar registers.
+template <class T, class TC, bool MASS, int MODE, int BLOCK, bool UNI = false>
 __global__ void __launch_bounds__(BLOCK)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
                const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
@@ -281,6 +296,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   constexpr int NW = BLOCK / 64;
   const T* __restrict__ x = io.x;
   static_assert(NW <= 16, "one max |x| word per wavefront in 64 LDS bytes");
+  static_assert(sizeof(fx_acc_t<T>) == sizeof(T), "sums and staged x share the byte offsets of a record");
   constexpr uint32_t XS = tile_xs_off<T, BLOCK>();
   static_assert(XS <= 65535u, "staged x must start within the ds immediate offset range");
   const lds_ptr<fx_acc_t<T>> acc = lds_at<fx_acc_t<T>>(0);
@@ -293,7 +309,9 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   const uint32_t n_own = rec.n_own;
   // first record: its loads are issued before the x gather so both chains overlap
   Rec R;
-  if (threadIdx.x < rec.n_elem) load_rec<TC, MASS>(R, rec.elem_off + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac);
+  if (threadIdx.x < rec.n_elem)
+    load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0>(R, rec.elem_off + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac,
+                                               maxD, hFac);
   uint32_t xhi = 0;   // high word of max |x| over the values this thread staged
   // FP32 residual and smoothing: the owned nodes' ids, b and invd are loaded with the first
   // gather batch, so the store phase waits on no global load (tiles of at most
@@ -346,11 +364,12 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   const int ex = MODE == OP_DIAG ? 1 : max(int(xhi >> 20) - 1022, FX_XMIN);
   const int es = fx_scale<T>(texp[t], ex);
   for (uint32_t base = 0;;) {
-    if (base + threadIdx.x < rec.n_elem) eval_rows<T, MASS, MODE, XS>(R, n_own, maxD, hFac, es, KF);
+    if (base + threadIdx.x < rec.n_elem) eval_rows<T, MASS, MODE, XS>(R, n_own * uint32_t(sizeof(T)), es, KF);
     base += BLOCK;
     if (base >= rec.n_elem) break;
     if (base + threadIdx.x < rec.n_elem)
-      load_rec<TC, MASS>(R, rec.elem_off + base + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac);
+      load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0>(R, rec.elem_off + base + threadIdx.x, lid, meta, kappa, mass,
+                                                 lap_fac, mass_fac, maxD, hFac);
   }
   __syncthreads();
   if (PREF && pref) {

@@ -100,7 +100,7 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   Rec Rn;
   uint32_t r0 = uint32_t(wv) * EPW;
   bool hn = r0 + es < rec.n_elem;
-  if (hn) load_rec<T, MASS>(Rn, rec.elem_off + r0 + es, lid, meta, kappa, mass, lap_fac, mass_fac);
+  if (hn) load_rec<T, MASS>(Rn, rec.elem_off + r0 + es, lid, meta, kappa, mass, lap_fac, mass_fac, maxD, hFac);
   uint32_t cur = 0xffffffffu;
   T Lr[36], Mr[36];
   for (; r0 < rec.n_elem; r0 += STRIDE) {
@@ -108,7 +108,7 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
     const bool h = hn;
     const uint32_t rn = r0 + STRIDE + es;
     hn = rn < rec.n_elem;
-    if (hn) load_rec<T, MASS>(Rn, rec.elem_off + rn, lid, meta, kappa, mass, lap_fac, mass_fac);
+    if (hn) load_rec<T, MASS>(Rn, rec.elem_off + rn, lid, meta, kappa, mass, lap_fac, mass_fac, maxD, hFac);
     const uint32_t tp = R.mt & 31u;
     bool done = !h;
     for (;;) {
@@ -206,9 +206,8 @@ tile_multi_mfma_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslice
     float a[8], bv[8];
     if (act) {
       Rec R;
-      load_rec<float, MASS>(R, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac);
-      float f1, f2;
-      elem_factors<float>(R.mt >> 5, maxD, hFac, R.k1, R.k2, f1, f2);
+      load_rec<float, MASS>(R, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac, maxD, hFac);
+      const float f1 = float(R.k1), f2 = float(R.k2);
       const float* Lrow = sK + (R.mt & 31u) * (NM * 64) + (4 * ib + q) * 8;
 #pragma unroll
       for (int kk = 0; kk < 8; ++kk) {
