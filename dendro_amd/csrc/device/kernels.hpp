@@ -367,9 +367,12 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
     if (base + threadIdx.x < rec.n_elem) eval_rows<T, MASS, MODE, XS>(R, n_own * uint32_t(sizeof(T)), es, KF);
     base += BLOCK;
     if (base >= rec.n_elem) break;
-    if (base + threadIdx.x < rec.n_elem)
-      load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0>(R, rec.elem_off + base + threadIdx.x, lid, meta, kappa, mass,
-                                                 lap_fac, mass_fac, maxD, hFac);
+    if (base + threadIdx.x < rec.n_elem) {   // wave-uniform bases, lane offsets fixed
+      const uint32_t r0 = rec.elem_off + base;
+      load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0>(R, threadIdx.x, lid + size_t(r0) * 8, meta + r0,
+                                                 UNI ? kappa : kappa + r0, UNI || !MASS ? mass : mass + r0, lap_fac,
+                                                 mass_fac, maxD, hFac);
+    }
   }
   __syncthreads();
   if (PREF && pref) {
