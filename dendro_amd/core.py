@@ -29,7 +29,8 @@ EXPORTS = [
     "dgh_init", "dgh_finalize", "dgh_alloc", "dgh_free", "dgh_copy_to_device", "dgh_copy_to_host",
     "dgh_synchronize",
     "dgh_mesh_create", "dgh_mesh_set_coefficients", "dgh_mesh_info", "dgh_mesh_destroy",
-    "dgh_mesh_set_tile", "dgh_mesh_tile_info", "dgh_mesh_autotune",
+    "dgh_mesh_set_tile", "dgh_mesh_tile_info", "dgh_mesh_autotune", "dgh_mesh_set_lane_stencils",
+    "dgh_mesh_lane_stencils",
     "dgh_matvec", "dgh_diagonal", "dgh_matvec_multi", "dgh_mesh_set_tile_multi",
     "dgh_mesh_autotune_multi",
     "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add",
@@ -304,13 +305,22 @@ class Mesh:
         nr, nn = C.c_uint64(), C.c_uint64()
         check(lib().dgh_mesh_tile_info(self.h, C.byref(te), C.byref(nt), C.byref(nr), C.byref(nn),
                                        C.byref(lds)))
+        on = C.c_int()
+        check(lib().dgh_mesh_lane_stencils(self.h, C.byref(on)))
         return {"tile_elems": te.value, "n_tiles": nt.value, "n_records": nr.value,
-                "n_node_entries": nn.value, "lds_bytes_f64": lds.value}
+                "n_node_entries": nn.value, "lds_bytes_f64": lds.value, "lane_stencils": on.value}
+
+    def set_lane_stencils(self, on):
+        """1: operator kernels read stencil rows per lane from an LDS copy of the table;
+        0: per stencil type of a wavefront through the scalar cache (default)."""
+        check(lib().dgh_mesh_set_lane_stencils(self.h, int(on)))
 
     def autotune(self, prec=F64):
         te, blk, us = C.c_uint32(), C.c_uint32(), C.c_float()
         check(lib().dgh_mesh_autotune(self.h, int(prec), C.byref(te), C.byref(blk), C.byref(us)))
-        return {"tile_elems": te.value, "block": blk.value, "us": us.value}
+        on = C.c_int()
+        check(lib().dgh_mesh_lane_stencils(self.h, C.byref(on)))
+        return {"tile_elems": te.value, "block": blk.value, "us": us.value, "lane_stencils": on.value}
 
     def matvec(self, x, y, prec=F64, stream=None):
         check(lib().dgh_matvec(self.h, int(prec), ptr(x), ptr(y), _stream(stream)))

@@ -21,7 +21,8 @@ namespace dgh {
 
 template <class T>
 static size_t op_lds_bytes(const dgh_mesh m) {
-  return sizeof(T) == 8 ? m->op.lds64 : m->op.lds32;
+  return m->lane_rows ? tile_lds_bytes<T>(m->op.block, m->op.plan.max_touch, true)
+                      : (sizeof(T) == 8 ? m->op.lds64 : m->op.lds32);
 }
 
 template <class T> const T* rk(dgh_mesh m);
@@ -47,16 +48,25 @@ static void launch_op_b(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
   const size_t lds = op_lds_bytes<T>(m);
   const T* kap = rk<T>(m);
   const T* mas = rm<T>(m);
-#define DGH_OP_LAUNCH(MASS, UNI)                                                                                  \
-  hipLaunchKernelGGL((tile_op_kernel<T, T, MASS, MODE, BLOCK, UNI>), dim3(grid), dim3(BLOCK), lds, st, m->op.tiles.p, \
-                     nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, m->hFac, KFtab<T>(),  \
-                     m->op.node_tab.p, m->op.texp.p, io)
-  if (!kap)
-    DGH_OP_LAUNCH(true, true);
-  else if (!mas)
-    DGH_OP_LAUNCH(false, false);
-  else
-    DGH_OP_LAUNCH(true, false);
+#define DGH_OP_LAUNCH(MASS, UNI, LANE)                                                                        \
+  hipLaunchKernelGGL((tile_op_kernel<T, T, MASS, MODE, BLOCK, UNI, LANE>), dim3(grid), dim3(BLOCK), lds, st,      \
+                     m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,     \
+                     m->hFac, KFtab<T>(), m->op.node_tab.p, m->op.texp.p, io)
+  if (m->lane_rows) {
+    if (!kap)
+      DGH_OP_LAUNCH(true, true, true);
+    else if (!mas)
+      DGH_OP_LAUNCH(false, false, true);
+    else
+      DGH_OP_LAUNCH(true, false, true);
+  } else {
+    if (!kap)
+      DGH_OP_LAUNCH(true, true, false);
+    else if (!mas)
+      DGH_OP_LAUNCH(false, false, false);
+    else
+      DGH_OP_LAUNCH(true, false, false);
+  }
 #undef DGH_OP_LAUNCH
   HIP_CHECK(hipGetLastError());
 }
@@ -220,6 +230,8 @@ static uint32_t op_budget(int block) {
 // block size's node budget.
 static void plan_mesh(dgh_mesh m, uint32_t T) {
   build_devtiles(m, m->op, T, op_lds_f64, op_budget(m->block), m->block);
+  if (m->lane_rows && tile_lds_bytes<double>(m->block, m->op.plan.max_touch, true) > size_t(g.max_lds))
+    m->lane_rows = 0;   // no LDS room left for the stencil table
 }
 
 }  // namespace dgh
@@ -330,20 +342,29 @@ static void launch_cg_b(dgh_mesh m, const double* p, double* w, double* part, hi
   check_op_plan<BLOCK>(m);
   const int nt = int(m->op.plan.tiles.size());
   const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
-  const size_t lds = m->op.lds64;
+  const size_t lds = op_lds_bytes<double>(m);
   const OpIO<double> io{p, w, nullptr, nullptr, 0.0, part};
   const double* kap = m->op.rk64.p;
   const double* mas = m->op.rm64.p;
-#define DGH_CG_LAUNCH(MASS, UNI)                                                                               \
-  hipLaunchKernelGGL((tile_op_kernel<double, double, MASS, OP_CG, BLOCK, UNI>), dim3(grid), dim3(BLOCK), lds, st, \
-                     m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,     \
+#define DGH_CG_LAUNCH(MASS, UNI, LANE)                                                                         \
+  hipLaunchKernelGGL((tile_op_kernel<double, double, MASS, OP_CG, BLOCK, UNI, LANE>), dim3(grid), dim3(BLOCK), lds, \
+                     st, m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, \
                      m->hFac, KFtab<double>(), m->op.node_tab.p, m->op.texp.p, io)
-  if (!kap)
-    DGH_CG_LAUNCH(true, true);
-  else if (!mas)
-    DGH_CG_LAUNCH(false, false);
-  else
-    DGH_CG_LAUNCH(true, false);
+  if (m->lane_rows) {
+    if (!kap)
+      DGH_CG_LAUNCH(true, true, true);
+    else if (!mas)
+      DGH_CG_LAUNCH(false, false, true);
+    else
+      DGH_CG_LAUNCH(true, false, true);
+  } else {
+    if (!kap)
+      DGH_CG_LAUNCH(true, true, false);
+    else if (!mas)
+      DGH_CG_LAUNCH(false, false, false);
+    else
+      DGH_CG_LAUNCH(true, false, false);
+  }
 #undef DGH_CG_LAUNCH
   HIP_CHECK(hipGetLastError());
 }
@@ -615,6 +636,25 @@ int dgh_mesh_set_tile(dgh_mesh m, uint32_t tile_elems, uint32_t block) {
   });
 }
 
+int dgh_mesh_set_lane_stencils(dgh_mesh m, int on) {
+  return guarded([&] {
+    require_init();
+    if (!m) fail(DGH_ERR_ARG, "NULL mesh");
+    if (on != 0 && on != 1) fail(DGH_ERR_ARG, "lane stencils: 0 or 1");
+    if (on && tile_lds_bytes<double>(m->block, m->op.plan.max_touch, true) > size_t(g.max_lds))
+      fail(DGH_ERR_ARG, "the tiling leaves no LDS room for the stencil table");
+    ++m->gen;   // captured graphs hold the kernel choice
+    m->lane_rows = on;
+  });
+}
+
+int dgh_mesh_lane_stencils(dgh_mesh m, int* on) {
+  return guarded([&] {
+    if (!m || !on) fail(DGH_ERR_ARG, "NULL argument");
+    *on = m->lane_rows;
+  });
+}
+
 int dgh_mesh_tile_info(dgh_mesh m, uint32_t* tile_elems, uint32_t* n_tiles, uint64_t* n_records,
                        uint64_t* n_node_entries, uint32_t* lds_bytes_f64) {
   return guarded([&] {
@@ -640,6 +680,7 @@ int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* bloc
     HIP_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
     uint32_t bt = 0, bb = 0;
+    int bl = 0;
     // small levels need small tiles: enough workgroups to fill the device
     for (uint32_t T : {128u, 256u, 512u, 1024u, 2048u, 4096u}) {
       for (int B : {256, 512, 1024}) {
@@ -650,23 +691,28 @@ int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* bloc
           continue;
         }
         if (m->op.plan.tile_elems != T) continue;   // reduced to fit LDS: tried already
-        auto run = [&] {
-          if (prec == DGH_F64)
-            launch_matvec<double>(m, xb.p, yb.p, nullptr);
-          else
-            launch_matvec<float>(m, reinterpret_cast<const float*>(xb.p), reinterpret_cast<float*>(yb.p), nullptr);
-        };
-        for (int i = 0; i < 3; ++i) run();
-        HIP_CHECK(hipEventRecord(e0, nullptr));
-        for (int i = 0; i < 10; ++i) run();
-        HIP_CHECK(hipEventRecord(e1, nullptr));
-        HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-          best = ms;
-          bt = T;
-          bb = uint32_t(B);
+        for (int L : {0, 1}) {   // stencil rows per wavefront type, or per lane from LDS
+          m->lane_rows = L;
+          if (op_lds_bytes<double>(m) > size_t(g.max_lds)) continue;
+          auto run = [&] {
+            if (prec == DGH_F64)
+              launch_matvec<double>(m, xb.p, yb.p, nullptr);
+            else
+              launch_matvec<float>(m, reinterpret_cast<const float*>(xb.p), reinterpret_cast<float*>(yb.p), nullptr);
+          };
+          for (int i = 0; i < 3; ++i) run();
+          HIP_CHECK(hipEventRecord(e0, nullptr));
+          for (int i = 0; i < 10; ++i) run();
+          HIP_CHECK(hipEventRecord(e1, nullptr));
+          HIP_CHECK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            bt = T;
+            bb = uint32_t(B);
+            bl = L;
+          }
         }
       }
     }
@@ -674,6 +720,7 @@ int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* bloc
     HIP_CHECK(hipEventDestroy(e1));
     if (!bt) fail(DGH_ERR_STATE, "no tiling candidate fits");
     m->block = int(bb);
+    m->lane_rows = bl;
     plan_mesh(m, bt);
     if (tile_elems) *tile_elems = bt;
     if (block) *block = bb;

@@ -123,6 +123,7 @@ struct dgh_mesh_s {
   DevBuf<uint32_t> nl;          // [E][8]
   DevBuf<uint8_t> cs;           // [E] stencil index 18c+t
   double lap_fac = 1.0, mass_fac = 1.0;
+  int lane_rows = 0;            // operator kernels read stencil rows per lane from LDS
   // owner-computes tilings: single right-hand side, and batched right-hand sides (smaller
   // tiles: the batched kernel stages several values per node in LDS)
   DevTiles op;

@@ -166,9 +166,18 @@ template <class T, int BLOCK>
 __host__ __device__ constexpr uint32_t tile_xs_off() {
   return tile_xhi_off<T, BLOCK>() + 64u;
 }
+// Lane stencils (small levels): a copy of the stencil table follows the staged x of the
+// tile, 16-byte aligned.
+constexpr uint32_t KF_ELEMS = NSTENCIL * KFSTRIDE;
 template <class T>
-__host__ constexpr size_t tile_lds_bytes(int block, uint32_t max_touch) {
-  return sizeof(fx_acc_t<T>) * op_own_cap(block) + 64 + sizeof(T) * size_t(max_touch);
+__host__ __device__ constexpr uint32_t tile_kf_off(uint32_t xs_off, uint32_t n_touch) {
+  return (xs_off + uint32_t(sizeof(T)) * n_touch + 15u) & ~15u;
+}
+template <class T>
+__host__ constexpr size_t tile_lds_bytes(int block, uint32_t max_touch, bool lane_rows = false) {
+  const uint32_t xs = uint32_t(sizeof(fx_acc_t<T>)) * op_own_cap(block) + 64u;
+  return lane_rows ? size_t(tile_kf_off<T>(xs, max_touch)) + sizeof(T) * KF_ELEMS
+                   : size_t(xs) + sizeof(T) * size_t(max_touch);
 }
 template <class U>
 using lds_ptr = __attribute__((address_space(3))) U*;
@@ -275,6 +284,46 @@ __device__ __forceinline__ void eval_rows(const Rec& R, uint32_t own_bytes, int 
   }
 }
 
+// Lane-stencil variant for small levels: each lane reads its own type's rows from the LDS
+// copy of the stencil table at kf (byte offset; the row offsets ride in the ds immediate
+// field), so a wavefront whose lanes hold several stencil types makes one pass instead of
+// one per type (small tiles mix many types in a wavefront; each pass is a chain of eight
+// scalar row loads).
+template <class T, bool MASS, int MODE, uint32_t XS>
+__device__ __forceinline__ void eval_rows_lane(const Rec& R, uint32_t own_bytes, int es, uint32_t kf) {
+  const T fac1 = T(R.k1), fac2 = T(R.k2);
+  const T f1s = T(ldexp(double(fac1), es));
+  const T f2s = MASS ? T(ldexp(double(fac2), es)) : T(0);
+  T xv[8];
+  if (MODE != OP_DIAG) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xv[j] = *lds_at<T>(XS + R.l[j]);
+  }
+  const uint32_t rb = kf + (R.mt & 31u) * uint32_t(KFSTRIDE * sizeof(T));
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (R.l[k] < own_bytes) {
+      const lds_ptr<T> lr = lds_at<T>(rb + uint32_t(k * 8 * sizeof(T)));
+      const lds_ptr<T> mr = lds_at<T>(rb + uint32_t((64 + k * 8) * sizeof(T)));
+      T a, b = T(0);
+      if (MODE == OP_DIAG) {
+        a = lr[k];
+        if (MASS) b = mr[k];
+      } else {
+        a = lr[0] * xv[0];
+#pragma unroll
+        for (int j = 1; j < 8; ++j) a += lr[j] * xv[j];
+        if (MASS) {
+          b = mr[0] * xv[0];
+#pragma unroll
+          for (int j = 1; j < 8; ++j) b += mr[j] * xv[j];
+        }
+      }
+      lds_add(lds_at<fx_acc_t<T>>(R.l[k]), fx_term(a, f1s, b, f2s, MASS));
+    }
+  }
+}
+
 constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 
 // KF: [18][128] child-0 full stencil rows of precision T (Laplacian, then mass).
@@ -286,7 +335,9 @@ constexpr int XCHUNK = 4;   // x gathers per thread in flight together
 // UNI: uniform coefficients (kappa and mass unused; lap_fac, mass_fac, maxD and hFac form the
 // factors), else per-record factors (lap_fac, mass_fac, maxD and hFac unused).  The two are
 // separate instantiations so that neither keeps the other's arguments in scalar registers.
-template <class T, class TC, bool MASS, int MODE, int BLOCK, bool UNI = false>
+// LANE: per-lane stencil rows from an LDS copy of KF (eval_rows_lane) instead of per-type
+// passes through the scalar cache.
+template <class T, class TC, bool MASS, int MODE, int BLOCK, bool UNI = false, bool LANE = false>
 __global__ void __launch_bounds__(BLOCK)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
                const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
@@ -312,6 +363,10 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   if (threadIdx.x < rec.n_elem)
     load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0>(R, rec.elem_off + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac,
                                                maxD, hFac);
+  const uint32_t kf = tile_kf_off<T>(XS, rec.n_touch);
+  if (LANE) {   // the stencil table, copied while the gather runs (visible after the barrier)
+    for (uint32_t i = threadIdx.x; i < KF_ELEMS; i += BLOCK) lds_at<T>(kf)[i] = KF[i];
+  }
   uint32_t xhi = 0;   // high word of max |x| over the values this thread staged
   // FP32 residual and smoothing: the owned nodes' ids, b and invd are loaded with the first
   // gather batch, so the store phase waits on no global load (tiles of at most
@@ -364,7 +419,12 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   const int ex = MODE == OP_DIAG ? 1 : max(int(xhi >> 20) - 1022, FX_XMIN);
   const int es = fx_scale<T>(texp[t], ex);
   for (uint32_t base = 0;;) {
-    if (base + threadIdx.x < rec.n_elem) eval_rows<T, MASS, MODE, XS>(R, n_own * uint32_t(sizeof(T)), es, KF);
+    if (base + threadIdx.x < rec.n_elem) {
+      if (LANE)
+        eval_rows_lane<T, MASS, MODE, XS>(R, n_own * uint32_t(sizeof(T)), es, kf);
+      else
+        eval_rows<T, MASS, MODE, XS>(R, n_own * uint32_t(sizeof(T)), es, KF);
+    }
     base += BLOCK;
     if (base >= rec.n_elem) break;
     if (base + threadIdx.x < rec.n_elem) {   // wave-uniform bases, lane offsets fixed

@@ -185,10 +185,17 @@ int dgh_mesh_set_tile(dgh_mesh m, uint32_t tile_elems, uint32_t block);
  * LDS bytes per FP64 workgroup.  No reference counterpart. */
 int dgh_mesh_tile_info(dgh_mesh m, uint32_t* tile_elems, uint32_t* n_tiles, uint64_t* n_records,
                        uint64_t* n_node_entries, uint32_t* lds_bytes_f64);
+/* Stencil rows of the operator kernels: 0 = read through the scalar cache once per stencil
+ * type present in a wavefront (default), 1 = read per lane from a copy of the stencil table
+ * in LDS (one pass per wavefront however many types its lanes hold; pays on small levels,
+ * whose small tiles mix many types).  Fails if the tiling leaves no LDS room for the table.
+ * No reference counterpart: a kernel choice, also made by dgh_mesh_autotune. */
+int dgh_mesh_set_lane_stencils(dgh_mesh m, int on);
+int dgh_mesh_lane_stencils(dgh_mesh m, int* on);
 /* Autotuner: times the operator kernel of precision prec on this mesh for every candidate
- * (tile size, block) pair and keeps the fastest.  Synchronous; uses scratch vectors it
- * allocates itself.  Outputs (any may be NULL): the chosen pair and its time per launch.
- * No reference counterpart. */
+ * (tile size, block, stencil rows) choice and keeps the fastest.  Synchronous; uses scratch
+ * vectors it allocates itself.  Outputs (any may be NULL): the chosen tile size and block
+ * and the time per launch.  No reference counterpart. */
 int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* block, float* us);
 int dgh_mesh_destroy(dgh_mesh m);
 

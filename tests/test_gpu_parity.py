@@ -35,7 +35,7 @@ def _np(t):
     return t.detach().cpu().numpy().astype(np.float64)
 
 
-def _meshes(h, coefs=None, lap_fac=1.0, mass_fac=1.0):
+def _meshes(h, coefs=None, lap_fac=1.0, mass_fac=1.0, lane=0):
     ms = []
     for l in range(h.n):
         m = da.Mesh(h.levels[l])
@@ -43,13 +43,16 @@ def _meshes(h, coefs=None, lap_fac=1.0, mass_fac=1.0):
             m.set_coefficients(None, None, lap_fac, mass_fac)
         else:
             m.set_coefficients(coefs[l][0], coefs[l][1])
+        m.set_lane_stencils(lane)
         ms.append(m)
     ts = [da.Transfer(ms[l - 1], ms[l], *h.transfers[l - 1]) for l in range(1, h.n)]
     return ms, ts
 
 
+@pytest.mark.parametrize("lane", [0, 1])
 @pytest.mark.parametrize("mode", ["jac1", "jac2", "jac2_nomass"])
-def test_matvec_fp64(dev, hier_small, mode):
+def test_matvec_fp64(dev, hier_small, mode, lane):
+    """lane = 1: stencil rows read per lane from the LDS copy of the table."""
     h = hier_small
     F = h.fine
     rng = np.random.default_rng(11)
@@ -65,6 +68,8 @@ def test_matvec_fp64(dev, hier_small, mode):
         m.set_coefficients(None, None, 1.0, 1.0)
     else:
         m.set_coefficients(coef[0], coef[1])
+    m.set_lane_stencils(lane)
+    assert m.tile_info()["lane_stencils"] == lane
     xt = _t(x, dev)
     yt = torch.empty_like(xt)
     m.matvec(xt, yt)
@@ -102,11 +107,13 @@ def test_matvec_reproducible_and_scale_exact(dev, hier_small, prec):
     assert torch.isnan(yn[F.n_node // 2])
 
 
-def test_diagonal_fp64(dev, hier_small):
+@pytest.mark.parametrize("lane", [0, 1])
+def test_diagonal_fp64(dev, hier_small, lane):
     h = hier_small
     c = var_coefficients(h, mass_base=1.0)
     m = da.Mesh(h.fine)
     m.set_coefficients(c[-1][0], c[-1][1])
+    m.set_lane_stencils(lane)
     d = torch.empty(h.fine.n_node, dtype=torch.float64, device=dev)
     m.diagonal(d)
     assert rel_err(_np(d), oracle.diagonal(h.fine, c[-1][0], c[-1][1])) <= TOL64
@@ -129,12 +136,14 @@ def test_transfer_fp64(dev, hier_small):
         assert rel_err(_np(fout), f + oracle.prolong(C, F, c)) <= TOL64
 
 
-def test_fp32_operators_error_stated(dev, hier_small):
+@pytest.mark.parametrize("lane", [0, 1])
+def test_fp32_operators_error_stated(dev, hier_small, lane):
     h = hier_small
     F = h.fine
     x = np.random.default_rng(4).uniform(-1, 1, F.n_node)
     m = da.Mesh(F)
     m.set_coefficients(None, None, 1.0, 1.0)
+    m.set_lane_stencils(lane)
     xt = _t(x, dev, torch.float32)
     yt = torch.empty_like(xt)
     m.matvec(xt, yt, prec=da.F32)
@@ -143,11 +152,12 @@ def test_fp32_operators_error_stated(dev, hier_small):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("lane", [0, 1])
 @pytest.mark.parametrize("null_space", [False, True])
-def test_vcycle_fp64(dev, hier_tiny, null_space):
+def test_vcycle_fp64(dev, hier_tiny, null_space, lane):
     h = hier_tiny
     coefs = var_coefficients(h, mass_base=None if null_space else 1.0)
-    ms, ts = _meshes(h, coefs)
+    ms, ts = _meshes(h, coefs, lane=lane)
     p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=int(null_space))
     s = da.Solver(ms, ts, p)
     mg = oracle.MG([(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)],
@@ -160,10 +170,11 @@ def test_vcycle_fp64(dev, hier_tiny, null_space):
     assert rel_err(_np(z), mg.vcycle(r)) <= 1e-11
 
 
+@pytest.mark.parametrize("lane", [0, 1])
 @pytest.mark.parametrize("norm_type", [da.NORM_PRECONDITIONED, da.NORM_UNPRECONDITIONED])
-def test_pcg_iterates_fp64(dev, hier_tiny, norm_type):
+def test_pcg_iterates_fp64(dev, hier_tiny, norm_type, lane):
     h = hier_tiny
-    ms, ts = _meshes(h, None, 1.0, 1.0)
+    ms, ts = _meshes(h, None, 1.0, 1.0, lane=lane)
     b = np.random.default_rng(12).uniform(-1, 1, h.fine.n_node)
     mg = oracle.MG([(h.levels[l], None, None, 1.0, 1.0) for l in range(h.n)],
                    omega=2.0 / 3.0, nu_pre=2, nu_post=2)
@@ -179,10 +190,11 @@ def test_pcg_iterates_fp64(dev, hier_tiny, norm_type):
         assert np.allclose(hist, ho, rtol=1e-10, atol=0)
 
 
-def test_pcg_converges_fp64_and_mixed(dev, hier_small):
+@pytest.mark.parametrize("lane", [0, 1])
+def test_pcg_converges_fp64_and_mixed(dev, hier_small, lane):
     h = hier_small
     coefs = var_coefficients(h, mass_base=None)
-    ms, ts = _meshes(h, coefs)
+    ms, ts = _meshes(h, coefs, lane=lane)
     b = np.random.default_rng(22).uniform(-1, 1, h.fine.n_node)
     out = {}
     for prec in (da.F64, da.F32):
