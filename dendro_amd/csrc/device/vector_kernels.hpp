@@ -133,7 +133,9 @@ static __global__ void cg_xr_kernel(uint32_t n, double* __restrict__ S, const do
 //   z'.z' = z.z - n mu^2,  z'.r = z.r - mu sum(r),  z'.w = z.w - mu sum(w).
 constexpr int NDOT = 7;   // sum z, z.z, z.r, z.w, r.r, sum r, sum w
 enum CgDot { D_SZ = 0, D_ZZ, D_ZR, D_ZW, D_RR, D_SR, D_SW };
-constexpr int DOT_BLOCKS = 256;   // workgroups of the dot pass (one partial each)
+constexpr int DOT_BLOCKS = 1024;  // workgroups of the dot pass (one partial each)
+constexpr int DOT_UNROLL = 2;     // elements per thread and step of the dot pass
+static_assert(NDOT * DOT_BLOCKS <= (MAXQ > NDOT ? MAXQ : NDOT) * RED_BLOCKS, "partials fit the buffer");
 
 // Stage 1 of the dots: block partials part[q * DOT_BLOCKS + block] (w == nullptr: z.w and
 // sum w are zero).
@@ -142,15 +144,28 @@ __global__ void __launch_bounds__(VBLOCK)
 cg_dots_kernel(uint32_t n, const TZ* __restrict__ z, const double* __restrict__ r, const double* __restrict__ w,
                double* __restrict__ part) {
   double s[NDOT] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (uint32_t i = blockIdx.x * VBLOCK + threadIdx.x; i < n; i += DOT_BLOCKS * VBLOCK) {
-    const double zi = double(z[i]), ri = r[i], wi = w ? w[i] : 0.0;
-    s[D_SZ] += zi;
-    s[D_ZZ] += zi * zi;
-    s[D_ZR] += zi * ri;
-    s[D_ZW] += zi * wi;
-    s[D_RR] += ri * ri;
-    s[D_SR] += ri;
-    s[D_SW] += wi;
+  // DOT_UNROLL strided elements per step, loaded together
+  constexpr uint32_t STRIDE = DOT_BLOCKS * VBLOCK;
+  for (uint32_t i0 = blockIdx.x * VBLOCK + threadIdx.x; i0 < n; i0 += DOT_UNROLL * STRIDE) {
+    double zi[DOT_UNROLL], ri[DOT_UNROLL], wi[DOT_UNROLL];
+#pragma unroll
+    for (int u = 0; u < DOT_UNROLL; ++u) {
+      const uint32_t i = i0 + u * STRIDE;
+      const bool in = i < n;
+      zi[u] = in ? double(z[i]) : 0.0;
+      ri[u] = in ? r[i] : 0.0;
+      wi[u] = (in && w) ? w[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < DOT_UNROLL; ++u) {
+      s[D_SZ] += zi[u];
+      s[D_ZZ] += zi[u] * zi[u];
+      s[D_ZR] += zi[u] * ri[u];
+      s[D_ZW] += zi[u] * wi[u];
+      s[D_RR] += ri[u] * ri[u];
+      s[D_SR] += ri[u];
+      s[D_SW] += wi[u];
+    }
   }
   __shared__ double red[NDOT][VBLOCK / 64];
 #pragma unroll
@@ -167,13 +182,23 @@ cg_dots_kernel(uint32_t n, const TZ* __restrict__ z, const double* __restrict__ 
 }
 
 // Sums the NDOT x DOT_BLOCKS partials in a fixed order (one RED2_THREADS-thread workgroup:
-// one partial of every quantity per thread, all loads in flight together).
-static_assert(DOT_BLOCKS == RED2_THREADS, "one partial per thread");
+// DOT_BLOCKS / RED2_THREADS partials of every quantity per thread, all loads in flight).
+static_assert(DOT_BLOCKS % RED2_THREADS == 0, "whole partials per thread");
 __device__ __forceinline__ void cg_sum_dots(const double* __restrict__ part, double* out) {
+  constexpr int PPT = DOT_BLOCKS / RED2_THREADS;
   __shared__ double red[NDOT][RED2_THREADS / 64];
   double v[NDOT];
+  double pv[NDOT][PPT];
 #pragma unroll
-  for (int q = 0; q < NDOT; ++q) v[q] = part[q * DOT_BLOCKS + threadIdx.x];
+  for (int q = 0; q < NDOT; ++q)
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) pv[q][k] = part[q * DOT_BLOCKS + k * RED2_THREADS + threadIdx.x];
+#pragma unroll
+  for (int q = 0; q < NDOT; ++q) {
+    v[q] = pv[q][0];
+#pragma unroll
+    for (int k = 1; k < PPT; ++k) v[q] += pv[q][k];
+  }
 #pragma unroll
   for (int q = 0; q < NDOT; ++q) {
     v[q] = wave_sum(v[q]);
@@ -338,8 +363,20 @@ sym_block_kernel(int n, int npad, const T* __restrict__ P, const T* __restrict__
   int I, J;
   sym_block_ij(blockIdx.x, I, J);
   const T* src = P + sym_block_off(I, J);
-#pragma unroll 8
-  for (int idx = threadIdx.x; idx < SB * SB; idx += 128) s[idx / SB][idx % SB] = src[idx];
+  // 16-byte loads, all of a thread's in flight together (a block is 16 KB in FP32)
+  using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+  constexpr int PER = int(sizeof(V4) / sizeof(T));
+  constexpr int NV = SB * SB / PER / 128;
+  V4 v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = reinterpret_cast<const V4*>(src)[threadIdx.x + q * 128];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const int idx = (threadIdx.x + q * 128) * PER;
+    const T* e = reinterpret_cast<const T*>(&v[q]);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) s[(idx + u) / SB][(idx + u) % SB] = e[u];
+  }
   if (threadIdx.x < SB) {
     const int r = I * SB + threadIdx.x;
     bi[threadIdx.x] = r < n ? b[r] : T(0);
