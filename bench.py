@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--sweep-tiles", action="store_true", help="time the MatVec for several tile sizes")
     ap.add_argument("--no-multi", action="store_true", help="skip the 32 right-hand-side batch")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-damping PCG")
+    ap.add_argument("--pcg-trace", action="store_true",
+                    help="only three eager (no hipGraph) mixed-precision PCG solves, for a kernel trace")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -198,6 +200,18 @@ def main():
             tuned = m.autotune(da.F64)
         log(f"autotuned tilings in {time.time() - t0:.1f}s: finest {tuned} {mf.tile_info()}")
     log(f"tiling: {mf.tile_info()}")
+    if args.pcg_trace:
+        log("levels: " + ", ".join(f"{l}: E={L.n_elem} {meshes[l].tile_info()}" for l, L in enumerate(levels)))
+        b = torch.tensor(np.random.default_rng(22).uniform(-1, 1, N), dtype=torch.float64, device=dev)
+        p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
+                              norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=da.F32,
+                              use_graph=0)
+        s = da.Solver(meshes, trans, p)
+        xs = torch.zeros(N, dtype=torch.float64, device=dev)
+        for _ in range(3):
+            st, _ = s.solve(b, xs)
+            log(f"eager mixed PCG: {st.iterations} its, {st.seconds * 1e3:.2f} ms")
+        return
 
     # ---- MatVec: K steps, inputs resident in HBM, events on the launch stream
     stream = torch.cuda.current_stream()
