@@ -496,11 +496,16 @@ prolong_kernel(uint32_t nf, const uint32_t* __restrict__ celem, const uint8_t* _
   const uint4 a = reinterpret_cast<const uint4*>(cnl)[2 * ce];
   const uint4 b = reinterpret_cast<const uint4*>(cnl)[2 * ce + 1];
   const uint32_t idx[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  // the 8 coarse values are loaded whatever the weights (they sit in L2): the loads then wait
+  // on the vertex ids only, not on the weights as well; zero weights still add nothing
+  T cv[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) cv[k] = c[idx[k]];
   T acc = T(0);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const T wk = w[k];
-    if (wk != T(0)) acc += wk * c[idx[k]];
+    acc += (wk != T(0)) ? wk * cv[k] : T(0);
   }
   f_out[j] = (f_in ? f_in[j] : T(0)) + acc;
 }
@@ -525,12 +530,27 @@ restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t
   T acc = T(0);
   if (i < nc) {
     const uint32_t p1 = rowptr[i + 1];
-    for (uint32_t p = rowptr[i] + lane; p < p1; p += RLANES) {
-      if (PACKED) {
-        const uint32_t e = col[p];
-        acc += T(float((e >> RP_SHIFT) + 1u) * 0.03125f) * f[e & RP_COL];
-      } else {
-        acc += T(w[p]) * f[col[p]];
+    // RU entries of a lane per step: their column loads, then their f loads, go out together
+    // (the sum still runs over the lane's entries in order)
+    constexpr int RU = 4;
+    for (uint32_t p0 = rowptr[i] + lane; p0 < p1; p0 += RU * RLANES) {
+      uint32_t e[RU];
+      T fv[RU], wv[RU];
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const uint32_t p = p0 + u * RLANES;
+        e[u] = p < p1 ? col[p] : 0u;
+        wv[u] = (!PACKED && p < p1) ? T(w[p]) : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < RU; ++u) fv[u] = p0 + u * RLANES < p1 ? f[PACKED ? (e[u] & RP_COL) : e[u]] : T(0);
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        if (p0 + u * RLANES >= p1) break;
+        if (PACKED)
+          acc += T(float((e[u] >> RP_SHIFT) + 1u) * 0.03125f) * fv[u];
+        else
+          acc += wv[u] * fv[u];
       }
     }
   }
