@@ -513,7 +513,7 @@ prolong_kernel(uint32_t nf, const uint32_t* __restrict__ celem, const uint8_t* _
 // c[i] = sum_p w_p f[col_p] over row i of R = P^T (CSR built on the host; gather, no
 // atomics).  RLANES lanes share a row (rows hold about 4-30 entries) and combine their
 // partial sums with shuffles; the sum order is fixed.
-constexpr int RLANES = 8;
+constexpr int RLANES = 4;
 // With x1 given, also the coarse level's first smoothing sweep from a zero guess:
 // x1 = omega (c * invd), as smooth_first_kernel forms it.
 // PACKED: one 32-bit word per entry, (32 w - 1) << 27 | fine node (every weight is a
@@ -532,7 +532,7 @@ restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t
     const uint32_t p1 = rowptr[i + 1];
     // RU entries of a lane per step: their column loads, then their f loads, go out together
     // (the sum still runs over the lane's entries in order)
-    constexpr int RU = 4;
+    constexpr int RU = 8;
     for (uint32_t p0 = rowptr[i] + lane; p0 < p1; p0 += RU * RLANES) {
       uint32_t e[RU];
       T fv[RU], wv[RU];
