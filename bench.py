@@ -318,7 +318,8 @@ def main():
     if not args.no_multi:
         extra["multi_rhs"] = bench_multi(args, meshes, trans, N, dev, stream, extra)
 
-    kernel = f"dgh::tile_op_kernel<double, double, false, 0, {tuned['block']}, false>"
+    lane = "true" if mf.tile_info()["lane_stencils"] else "false"
+    kernel = f"dgh::tile_op_kernel<double, double, false, 0, {tuned['block']}, false, {lane}>"
     traffic = measured_traffic(kernel, mf.tile_info()["tile_elems"], tuned["block"])
     out = {
         "metric": METRIC,
