@@ -267,7 +267,24 @@ restrict_multi_kernel(uint32_t nc, int k, const uint32_t* __restrict__ rowptr, c
   if (i >= nc) return;
   T acc = T(0);
   const uint32_t p1 = rowptr[i + 1];
-  for (uint32_t p = rowptr[i]; p < p1; ++p) acc += T(w[p]) * f[size_t(col[p]) * k + j];
+  // four entries per step: their column and weight loads, then their f loads, go out
+  // together (the sum still runs over the row in order)
+  constexpr int RU = 4;
+  for (uint32_t p0 = rowptr[i]; p0 < p1; p0 += RU) {
+    uint32_t cc[RU];
+    T wv[RU], fv[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const bool in = p0 + u < p1;
+      cc[u] = in ? col[p0 + u] : 0u;
+      wv[u] = in ? T(w[p0 + u]) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < RU; ++u) fv[u] = p0 + u < p1 ? f[size_t(cc[u]) * k + j] : T(0);
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+      if (p0 + u < p1) acc += wv[u] * fv[u];
+  }
   c[i * k + j] = acc;
 }
 
@@ -284,11 +301,18 @@ prolong_multi_kernel(uint32_t nf, int k, const uint32_t* __restrict__ celem, con
   if (j >= nf) return;
   const uint32_t ce = celem[j];
   const T* wv = W + (size_t(cs[ce]) * 27 + pos[j]) * 8;
+  // the coarse values are loaded whatever the weights: they then wait on the vertex ids only
+  const uint4 a = reinterpret_cast<const uint4*>(cnl)[2 * size_t(ce)];
+  const uint4 b = reinterpret_cast<const uint4*>(cnl)[2 * size_t(ce) + 1];
+  const uint32_t idx[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  T cv[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) cv[m] = c[size_t(idx[m]) * k + q];
   T acc = T(0);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const T wk = wv[m];
-    if (wk != T(0)) acc += wk * c[size_t(cnl[size_t(ce) * 8 + m]) * k + q];
+    acc += (wk != T(0)) ? wk * cv[m] : T(0);
   }
   f_out[gid] = (f_in ? f_in[gid] : T(0)) + acc;
 }
