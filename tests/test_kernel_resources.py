@@ -1,0 +1,48 @@
+"""Build-level guards of the operator kernels (CPU: hipcc cross-compiles gfx950 assembly).
+
+tile_op_kernel addresses its LDS at fixed byte offsets from address 0 (sums, max |x| words,
+staged x, stencil copy), which holds only while the kernel declares no static LDS: the
+dynamic LDS of a launch then starts at 0.  Every instantiation must also run without
+scratch (no register spills)."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HIPCC = "/opt/rocm/bin/hipcc"
+
+
+@pytest.fixture(scope="module")
+def asm(tmp_path_factory):
+    if not os.path.exists(HIPCC) and not shutil.which("hipcc"):
+        pytest.skip("hipcc not available")
+    out = str(tmp_path_factory.mktemp("asm") / "device_api.s")
+    cmd = [HIPCC if os.path.exists(HIPCC) else "hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+           "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wno-unused-command-line-argument", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(ROOT, "dendro_amd", "csrc"), "--cuda-device-only", "-S",
+           os.path.join(ROOT, "dendro_amd", "csrc", "device", "device_api.hip"), "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return open(out).read()
+
+
+def _kernels(asm, pattern):
+    ks = re.findall(r"\.amdhsa_kernel (\S+)\n(.*?)\.end_amdhsa_kernel", asm, re.S)
+    return [(n, b) for n, b in ks if pattern in n]
+
+
+def test_operator_kernels_have_no_static_lds(asm):
+    ks = _kernels(asm, "tile_op_kernel")
+    assert len(ks) > 100
+    bad = [n for n, b in ks if int(re.search(r"\.amdhsa_group_segment_fixed_size (\d+)", b).group(1)) != 0]
+    assert not bad, bad[:3]
+
+
+def test_operator_kernels_do_not_spill(asm):
+    ks = _kernels(asm, "tile_op_kernel")
+    bad = [n for n, b in ks if int(re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", b).group(1)) != 0]
+    assert not bad, bad[:3]
