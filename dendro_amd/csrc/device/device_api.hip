@@ -396,17 +396,19 @@ static void cg_iteration(dgh_solver s, hipStream_t st) {
     hipLaunchKernelGGL(cg_p_fused_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
                        static_cast<const double*>(s->zres), s->p.p);
   launch_cg(m, s->p.p, s->w.p, s->ptile.p, st);
-  hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED2_THREADS), 0, st, int(m->op.plan.tiles.size()),
-                     (const double*)s->ptile.p, S);
+  // alpha from the tile partials is formed inside the update kernel
+  const int nt = int(m->op.plan.tiles.size());
   const bool x1 = s->nlevels > 1 && s->prm.nu_pre > 0;   // first sweep of the V-cycle fused
   if (mixed)
-    hipLaunchKernelGGL(cg_xr_fused_kernel<float>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
-                       (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p, s->v32[top].b.p,
-                       float(s->om_level[top]), (const float*)s->v32[top].invd.p, x1 ? s->v32[top].xa.p : s->v32[top].xb.p);
+    hipLaunchKernelGGL(cg_xr_fused_kernel<float>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, nt,
+                       (const double*)s->ptile.p, S, (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p,
+                       s->v32[top].b.p, float(s->om_level[top]), (const float*)s->v32[top].invd.p,
+                       x1 ? s->v32[top].xa.p : s->v32[top].xb.p);
   else
-    hipLaunchKernelGGL(cg_xr_fused_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
-                       (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p, (float*)nullptr,
-                       s->om_level[top], (const double*)s->v64[top].invd.p, x1 ? s->v64[top].xa.p : s->v64[top].xb.p);
+    hipLaunchKernelGGL(cg_xr_fused_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, nt,
+                       (const double*)s->ptile.p, S, (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p,
+                       (float*)nullptr, s->om_level[top], (const double*)s->v64[top].invd.p,
+                       x1 ? s->v64[top].xa.p : s->v64[top].xb.p);
   const void* z = precondition_raw(s, s->r.p, st, x1);
   if (z != s->zres) fail(DGH_ERR_STATE, "V-cycle result buffer changed");
   if (mixed)

@@ -252,23 +252,6 @@ cg_init_fused_kernel(const double* __restrict__ part, uint32_t n, int null_space
   S[S_RATIO] = 0.0;
 }
 
-// alpha = rz / pw with pw the sum of the operator kernel's tile partials (one workgroup)
-static __global__ void __launch_bounds__(RED2_THREADS)
-cg_alpha_kernel(int ntiles, const double* __restrict__ part, double* __restrict__ S) {
-  __shared__ double red[RED2_THREADS / 64];
-  double v = 0.0;
-  for (int k = threadIdx.x; k < ntiles; k += RED2_THREADS) v += part[k];
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int k = 0; k < RED2_THREADS / 64; ++k) t += red[k];
-    S[S_PW] = t;
-    S[S_ALPHA] = S[S_RZ] / t;
-  }
-}
-
 // p = (z - mu) + beta p  (the projected z with VecAYPX)
 template <class TZ>
 __global__ void cg_p_fused_kernel(uint32_t n, const double* __restrict__ S, const TZ* __restrict__ z,
@@ -277,16 +260,33 @@ __global__ void cg_p_fused_kernel(uint32_t n, const double* __restrict__ S, cons
   if (i < n) p[i] = (double(z[i]) - S[S_MU]) + S[S_RATIO] * p[i];
 }
 
-// x = x + alpha p;  r = r + (-alpha) w;  then the V-cycle's first smoothing sweep from zero,
-// x1 = omega (r invd), in the V-cycle's precision TV (mixed: also the FP32 copy of r, r32)
+// alpha = rz / p.w with p.w the sum of the operator's tile partials, formed by every
+// workgroup in the same fixed order (one RED2_THREADS-thread pass);
+// workgroup 0 stores p.w and alpha.  Then x = x + alpha p;  r = r + (-alpha) w;  and the
+// V-cycle's first smoothing sweep from zero, x1 = omega (r invd), in the V-cycle's precision
+// TV (mixed: also the FP32 copy of r, r32).
+static_assert(VBLOCK == RED2_THREADS, "one partial sum pass per update workgroup");
 template <class TV>
-__global__ void cg_xr_fused_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ p,
-                                   const double* __restrict__ w, double* __restrict__ x, double* __restrict__ r,
-                                   float* __restrict__ r32, TV omega, const TV* __restrict__ invd,
-                                   TV* __restrict__ x1) {
+__global__ void __launch_bounds__(VBLOCK)
+cg_xr_fused_kernel(uint32_t n, int ntiles, const double* __restrict__ part, double* __restrict__ S,
+                   const double* __restrict__ p, const double* __restrict__ w, double* __restrict__ x,
+                   double* __restrict__ r, float* __restrict__ r32, TV omega, const TV* __restrict__ invd,
+                   TV* __restrict__ x1) {
+  __shared__ double red[RED2_THREADS / 64];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < ntiles; k += RED2_THREADS) v += part[k];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double pw = 0.0;
+  for (int k = 0; k < RED2_THREADS / 64; ++k) pw += red[k];
+  const double a = S[S_RZ] / pw;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S[S_PW] = pw;
+    S[S_ALPHA] = a;
+  }
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double a = S[S_ALPHA];
   x[i] = x[i] + a * p[i];
   const double ri = r[i] + (-a) * w[i];
   r[i] = ri;
