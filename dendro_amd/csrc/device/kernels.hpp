@@ -324,7 +324,7 @@ __device__ __forceinline__ void eval_rows_lane(const Rec& R, uint32_t own_bytes,
   }
 }
 
-constexpr int XCHUNK = 4;   // x gathers per thread in flight together
+constexpr int XCHUNK = 8;   // x gathers per thread in flight together
 
 // KF: [18][128] child-0 full stencil rows of precision T (Laplacian, then mass).
 // kappa/mass: per record (nullptr: uniform lap_fac / mass_fac).  MASS: mass term present.
