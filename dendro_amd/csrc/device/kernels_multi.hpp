@@ -75,11 +75,30 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   const uint32_t n_own = rec.n_own;
   const size_t k = size_t(io.k);
   uint32_t xhi = 0;   // high word of max |x| of this thread's column (threadIdx.x % KSL)
-  for (uint32_t i = threadIdx.x; i < rec.n_touch * KSL; i += MB) {
-    const uint32_t nd = i / KSL, c = i % KSL;
-    const T v = io.x[size_t(nodes[nd]) * k + size_t(sl) * KSL + c];
-    xs[nd * RS + c] = v;
-    xhi = max(xhi, uint32_t(uint64_t(__double_as_longlong(double(v))) >> 32) & 0x7fffffffu);
+  // MGU gathers per thread in flight together: node ids, then their x values
+  constexpr int MGU = 4;
+  const uint32_t nx = rec.n_touch * KSL;
+  for (uint32_t i0 = threadIdx.x; i0 < nx; i0 += MGU * MB) {
+    uint32_t nid[MGU];
+    T v[MGU];
+#pragma unroll
+    for (int u = 0; u < MGU; ++u) {
+      const uint32_t i = i0 + u * MB;
+      nid[u] = i < nx ? nodes[i / KSL] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < MGU; ++u) {
+      const uint32_t i = i0 + u * MB;
+      v[u] = i < nx ? io.x[size_t(nid[u]) * k + size_t(sl) * KSL + i % KSL] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < MGU; ++u) {
+      const uint32_t i = i0 + u * MB;
+      if (i < nx) {
+        xs[(i / KSL) * RS + i % KSL] = v[u];
+        xhi = max(xhi, uint32_t(uint64_t(__double_as_longlong(double(v[u]))) >> 32) & 0x7fffffffu);
+      }
+    }
   }
   for (uint32_t i = threadIdx.x; i < n_own * RS; i += MB) ys[i] = TA(0);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -134,17 +153,40 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n_own * KSL; i += MB) {
-    const uint32_t nd = i / KSL, cc = i % KSL;
-    const size_t gi = size_t(nodes[nd]) * k + size_t(sl) * KSL + cc;
-    const T v = finite ? T(fx_value<T>(ys[nd * RS + cc], esc)) : T(__builtin_nan(""));   // cc == c
-    if (MODE == OP_MATVEC) {
-      io.y[gi] = v;
-    } else if (MODE == OP_RESID) {
-      io.y[gi] = io.b[gi] - v;
-    } else {
-      const T rr = io.b[gi] - v;
-      io.y[gi] = xs[nd * RS + cc] + io.omega * (rr * io.invd[nodes[nd]]);
+  // stores, MSU per thread with their loads in flight together (i % KSL == c: MB is a
+  // multiple of KSL)
+  constexpr int MSU = 4;
+  const uint32_t no = n_own * KSL;
+  for (uint32_t i0 = threadIdx.x; i0 < no; i0 += MSU * MB) {
+    uint32_t nid[MSU];
+    T bv[MSU], dv[MSU];
+#pragma unroll
+    for (int u = 0; u < MSU; ++u) {
+      const uint32_t i = i0 + u * MB;
+      nid[u] = i < no ? nodes[i / KSL] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < MSU; ++u) {
+      const uint32_t i = i0 + u * MB;
+      const size_t gi = size_t(nid[u]) * k + size_t(sl) * KSL + c;
+      bv[u] = (MODE != OP_MATVEC && i < no) ? io.b[gi] : T(0);
+      dv[u] = (MODE == OP_SMOOTH && i < no) ? io.invd[nid[u]] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < MSU; ++u) {
+      const uint32_t i = i0 + u * MB;
+      if (i >= no) break;
+      const uint32_t nd = i / KSL;
+      const size_t gi = size_t(nid[u]) * k + size_t(sl) * KSL + c;
+      const T v = finite ? T(fx_value<T>(ys[nd * RS + c], esc)) : T(__builtin_nan(""));
+      if (MODE == OP_MATVEC) {
+        io.y[gi] = v;
+      } else if (MODE == OP_RESID) {
+        io.y[gi] = bv[u] - v;
+      } else {
+        const T rr = bv[u] - v;
+        io.y[gi] = xs[nd * RS + c] + io.omega * (rr * dv[u]);
+      }
     }
   }
 }
@@ -188,9 +230,26 @@ tile_multi_mfma_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslice
   {
     const float4* X = reinterpret_cast<const float4*>(io.x);
     float4* xsv = reinterpret_cast<float4*>(xs);
-    for (uint32_t i = threadIdx.x; i < rec.n_touch * NV; i += MB) {
-      const uint32_t nd = i / NV, part = i % NV;
-      xsv[i] = X[size_t(nodes[nd]) * kv + size_t(sl) * NV + part];
+    constexpr int MGU = 4;   // gathers per thread in flight together
+    const uint32_t nx = rec.n_touch * NV;
+    for (uint32_t i0 = threadIdx.x; i0 < nx; i0 += MGU * MB) {
+      uint32_t nid[MGU];
+      float4 v[MGU];
+#pragma unroll
+      for (int u = 0; u < MGU; ++u) {
+        const uint32_t i = i0 + u * MB;
+        nid[u] = i < nx ? nodes[i / NV] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < MGU; ++u) {
+        const uint32_t i = i0 + u * MB;
+        if (i < nx) v[u] = X[size_t(nid[u]) * kv + size_t(sl) * NV + i % NV];
+      }
+#pragma unroll
+      for (int u = 0; u < MGU; ++u) {
+        const uint32_t i = i0 + u * MB;
+        if (i < nx) xsv[i] = v[u];
+      }
     }
   }
   for (uint32_t i = threadIdx.x; i < n_own * KS; i += MB) ys[i] = 0.0;
