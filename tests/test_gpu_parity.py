@@ -261,3 +261,38 @@ def test_pcg_graph_recaptured_after_retiling(dev, hier_small):
     # the operator sums are order-independent: the same operator gives the same iterates
     assert st2.converged and st2.iterations == st1.iterations
     assert torch.equal(x2, x1)
+
+
+@pytest.mark.parametrize("prec", [da.F64, da.F32])
+def test_lane_stencils_agree(dev, hier_small, prec):
+    """The two stencil-row sources agree to rounding (the row sums may be scheduled
+    differently), and a solver captured with one source re-captures after the switch and
+    reproduces the solve."""
+    h = hier_small
+    coefs = var_coefficients(h, mass_base=1.0)
+    ms, ts = _meshes(h, coefs)
+    dt = torch.float64 if prec == da.F64 else torch.float32
+    x = _t(np.random.default_rng(41).uniform(-1, 1, h.fine.n_node), dev, dt)
+    ys = []
+    for lane in (0, 1):
+        ms[-1].set_lane_stencils(lane)
+        y = torch.empty_like(x)
+        ms[-1].matvec(x, y, prec=prec)
+        ys.append(_np(y))
+    err = rel_err(ys[1], ys[0])
+    print(f"lane vs wavefront-type stencil rows: {err:.2e}")
+    assert err <= (1e-14 if prec == da.F64 else 1e-6)
+    b = _t(np.random.default_rng(42).uniform(-1, 1, h.fine.n_node), dev)
+    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=50, rtol=1e-10,
+                          norm_type=da.NORM_UNPRECONDITIONED, use_graph=1, vcycle_prec=prec)
+    s = da.Solver(ms, ts, p)
+    out = []
+    for lane in (0, 1):
+        for m in ms:
+            m.set_lane_stencils(lane)
+        xo = torch.zeros_like(b)
+        st, _ = s.solve(b, xo)
+        assert st.converged
+        out.append((st.iterations, _np(xo)))
+    assert out[0][0] == out[1][0]
+    assert rel_err(out[1][1], out[0][1]) <= 1e-8
