@@ -411,10 +411,18 @@ sym_sum_kernel(int n, int nb, int npad, const double* __restrict__ Q, const T* _
                const double* __restrict__ c1, T* __restrict__ y) {
   __shared__ double red[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // loads are issued SU at a time and added in index order (the same sums as one at a time)
+  constexpr int SU = 8;
   double mu = 0.0;
   if (c1) {
     double sb = 0.0;
-    for (int j = threadIdx.x; j < n; j += 256) sb += double(b[j]);
+    for (int j0 = threadIdx.x; j0 < n; j0 += 256 * SU) {
+      double v[SU];
+#pragma unroll
+      for (int u = 0; u < SU; ++u) v[u] = j0 + 256 * u < n ? double(b[j0 + 256 * u]) : 0.0;
+#pragma unroll
+      for (int u = 0; u < SU; ++u) sb += v[u];
+    }
     sb = wave_sum(sb);
     if (lane == 0) red[wv][0] = sb;
     __syncthreads();
@@ -424,8 +432,13 @@ sym_sum_kernel(int n, int nb, int npad, const double* __restrict__ Q, const T* _
   const int i = blockIdx.x * 64 + lane;
   double acc = 0.0;
   if (i < npad) {
-#pragma unroll 4
-    for (int K = wv; K < nb; K += 4) acc += Q[size_t(K) * npad + i];
+    for (int K0 = wv; K0 < nb; K0 += 4 * SU) {
+      double v[SU];
+#pragma unroll
+      for (int u = 0; u < SU; ++u) v[u] = K0 + 4 * u < nb ? Q[size_t(K0 + 4 * u) * npad + i] : 0.0;
+#pragma unroll
+      for (int u = 0; u < SU; ++u) acc += v[u];
+    }
   }
   red[wv][lane] = acc;
   __syncthreads();
