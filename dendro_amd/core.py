@@ -24,11 +24,12 @@ EXPORTS = [
     "dgh_octree_from_points", "dgh_octree_from_leaves", "dgh_octree_balance", "dgh_octree_coarsen",
     "dgh_octree_size", "dgh_octree_get", "dgh_octree_free",
     "dgh_damg_build", "dgh_damg_num_levels", "dgh_damg_level_info", "dgh_damg_level_arrays",
-    "dgh_damg_level_octants", "dgh_damg_transfer_arrays", "dgh_damg_coarsen_coefficients",
-    "dgh_damg_free",
+    "dgh_damg_level_octants", "dgh_damg_level_boundary", "dgh_damg_transfer_arrays",
+    "dgh_damg_coarsen_coefficients", "dgh_damg_free",
     "dgh_init", "dgh_finalize", "dgh_alloc", "dgh_free", "dgh_copy_to_device", "dgh_copy_to_host",
     "dgh_synchronize",
-    "dgh_mesh_create", "dgh_mesh_set_coefficients", "dgh_mesh_info", "dgh_mesh_destroy",
+    "dgh_mesh_create", "dgh_mesh_set_coefficients", "dgh_mesh_set_boundary", "dgh_matvec_dirichlet",
+    "dgh_diagonal_dirichlet", "dgh_mesh_info", "dgh_mesh_destroy",
     "dgh_mesh_set_tile", "dgh_mesh_tile_info", "dgh_mesh_autotune", "dgh_mesh_set_lane_stencils",
     "dgh_mesh_lane_stencils",
     "dgh_matvec", "dgh_diagonal", "dgh_matvec_multi", "dgh_mesh_set_tile_multi",
@@ -248,6 +249,12 @@ class DAMG:
         check(lib().dgh_damg_level_octants(self.h, int(l), ptr(anc), ptr(lev), ptr(fl)))
         return anc, lev, fl
 
+    def boundary(self, l):
+        """uint8 flags per node of level l: 1 on the boundary of the unit cube"""
+        b = np.zeros(self.info(l).n_node, np.uint8)
+        check(lib().dgh_damg_level_boundary(self.h, int(l), ptr(b)))
+        return b
+
     def transfer(self, fine_level):
         li = self.info(fine_level)
         ce = np.zeros(li.n_node, np.uint32)
@@ -340,6 +347,17 @@ class Mesh:
 
     def diagonal(self, d, prec=F64, stream=None):
         check(lib().dgh_diagonal(self.h, int(prec), ptr(d), _stream(stream)))
+
+    def set_boundary(self, bdy):
+        """boundary flags [N] (nonzero = boundary) for the Dirichlet operators; None removes them"""
+        b = None if bdy is None else np.ascontiguousarray(bdy, np.uint8)
+        check(lib().dgh_mesh_set_boundary(self.h, ptr(b)))
+
+    def matvec_dirichlet(self, x, y, prec=F64, stream=None):
+        check(lib().dgh_matvec_dirichlet(self.h, int(prec), ptr(x), ptr(y), _stream(stream)))
+
+    def diagonal_dirichlet(self, d, prec=F64, stream=None):
+        check(lib().dgh_diagonal_dirichlet(self.h, int(prec), ptr(d), _stream(stream)))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -789,6 +789,64 @@ int dgh_diagonal(dgh_mesh m, int prec, void* d, void* stream) {
   });
 }
 
+int dgh_mesh_set_boundary(dgh_mesh m, const uint8_t* bdy) {
+  return guarded([&] {
+    require_init();
+    if (!m) fail(DGH_ERR_ARG, "NULL mesh");
+    m->bdy.release();
+    m->dir_x.release();
+    if (!bdy || !m->N) return;
+    m->bdy.upload(bdy, m->N);
+    m->dir_x.alloc(m->N);
+  });
+}
+
+int dgh_matvec_dirichlet(dgh_mesh m, int prec, const void* x, void* y, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!m || !x || !y) fail(DGH_ERR_ARG, "NULL argument");
+    if (x == y) fail(DGH_ERR_ARG, "x and y must not alias");
+    if (m->N && !m->bdy.p) fail(DGH_ERR_STATE, "no boundary flags (dgh_mesh_set_boundary)");
+    if (!m->N) return;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    auto run = [&](auto* xt, auto* yt) {
+      using T = std::remove_const_t<std::remove_pointer_t<decltype(xt)>>;
+      T* xm = reinterpret_cast<T*>(m->dir_x.p);
+      dirichlet_mask_kernel<T><<<blocks_for(m->N), VBLOCK, 0, st>>>(m->N, m->bdy.p, xt, xm);
+      HIP_CHECK(hipGetLastError());
+      launch_matvec<T>(m, xm, yt, st);
+      dirichlet_rows_kernel<T><<<blocks_for(m->N), VBLOCK, 0, st>>>(m->N, m->bdy.p, xt, yt);
+      HIP_CHECK(hipGetLastError());
+    };
+    if (prec == DGH_F64)
+      run(static_cast<const double*>(x), static_cast<double*>(y));
+    else if (prec == DGH_F32)
+      run(static_cast<const float*>(x), static_cast<float*>(y));
+    else
+      fail(DGH_ERR_ARG, "unknown precision %d", prec);
+  });
+}
+
+int dgh_diagonal_dirichlet(dgh_mesh m, int prec, void* d, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!m || !d) fail(DGH_ERR_ARG, "NULL argument");
+    if (m->N && !m->bdy.p) fail(DGH_ERR_STATE, "no boundary flags (dgh_mesh_set_boundary)");
+    if (!m->N) return;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (prec == DGH_F64) {
+      launch_diag<double>(m, static_cast<double*>(d), st);
+      dirichlet_rows_kernel<double><<<blocks_for(m->N), VBLOCK, 0, st>>>(m->N, m->bdy.p, nullptr, static_cast<double*>(d));
+    } else if (prec == DGH_F32) {
+      launch_diag<float>(m, static_cast<float*>(d), st);
+      dirichlet_rows_kernel<float><<<blocks_for(m->N), VBLOCK, 0, st>>>(m->N, m->bdy.p, nullptr, static_cast<float*>(d));
+    } else {
+      fail(DGH_ERR_ARG, "unknown precision %d", prec);
+    }
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
 int dgh_transfer_create(dgh_mesh coarse, dgh_mesh fine, const uint32_t* coarse_elem, const uint8_t* pos,
                         dgh_transfer* out) {
   return guarded([&] {

@@ -321,6 +321,23 @@ int dgh_damg_level_octants(dgh_damg h, int level, uint32_t* anchor, uint8_t* lev
   });
 }
 
+// A node is on the domain boundary when its anchor has a coordinate 0 or 2^D (the unit cube
+// spans [0, 2^D] in DA coordinates); every node is the anchor of exactly one octant.
+int dgh_damg_level_boundary(dgh_damg h, int level, uint8_t* bdy) {
+  return guarded([&] {
+    if (!bdy) fail(DGH_ERR_ARG, "NULL argument");
+    const DALevel& da = level_of(h, level);
+    const uint32_t lim = 1u << da.D;
+    for (size_t i = 0; i < da.octs.size(); ++i) {
+      const uint32_t n = da.node_of[i];
+      if (n == kNoNode) continue;
+      uint32_t x, y, z;
+      demorton(da.octs[i].m, x, y, z);
+      bdy[n] = (x == 0 || y == 0 || z == 0 || x == lim || y == lim || z == lim) ? 1 : 0;
+    }
+  });
+}
+
 int dgh_damg_transfer_arrays(dgh_damg h, int fine_level, uint32_t* coarse_elem, uint8_t* pos) {
   return guarded([&] {
     const DALevel& da = level_of(h, fine_level);

@@ -132,6 +132,11 @@ int dgh_damg_level_arrays(dgh_damg h, int level, uint32_t* nlist, uint8_t* child
  * anchor[n_oct][3], lev[n_oct], flags[n_oct] (bit 7 = regular node, bit 6 = pseudo-octant). */
 int dgh_damg_level_octants(dgh_damg h, int level, uint32_t* anchor, uint8_t* lev,
                            uint8_t* flags);
+/* Boundary flags of the nodes of one level: bdy[n_node], 1 for a node on the boundary of the
+ * unit cube, else 0.  Replaces assignBoundaryFlags (src/oda/odaUtils.C:1054-1283), whose
+ * face/edge/corner codes the Dirichlet operators only test for nonzero
+ * (examples/src/backend/omgJac.C:508-586, 859-1160). */
+int dgh_damg_level_boundary(dgh_damg h, int level, uint8_t* bdy);
 /* Transfer map from level fine_level-1 (coarse) to fine_level: for every fine node, the
  * index of a coarse element whose closure holds it and the node's position code
  * pos = tx + 3 ty + 9 tz (t in {0,1,2}: offset 0, h/2, h inside that coarse element). */
@@ -205,6 +210,22 @@ int dgh_matvec(dgh_mesh m, int prec, const void* x_dev, void* y_dev, void* strea
 /* d = diag(A).  Replaces Jacobian1MatGetDiagonal (odaJac.C:160-223) and
  * Jacobian2MatGetDiagonal (omgJac.C:1594-1662). */
 int dgh_diagonal(dgh_mesh m, int prec, void* d_dev, void* stream);
+/* Boundary nodes of the mesh for the Dirichlet operators: bdy[N] host flags (nonzero = on
+ * the boundary, as dgh_damg_level_boundary returns them); NULL removes them.  Allocates an
+ * N-vector of scratch for dgh_matvec_dirichlet.  Replaces SetDirichletJacContexts
+ * (examples/src/backend/omgJac.C:508-586). */
+int dgh_mesh_set_boundary(dgh_mesh m, const uint8_t* bdy);
+/* Dirichlet-masked y = A x: boundary rows are the identity (y_i = x_i) and boundary columns
+ * are dropped from the other rows.  Replaces DirichletLaplacianMatMult and
+ * DirichletJacobianMatMult (examples/src/backend/omgJac.C:859-1056): coefficients set
+ * uniform with lap_fac 1 and mass_fac 0 give the first, mass_fac 1 the second; per-element
+ * coefficients apply the same masking to the Jacobian2 operator.  Needs
+ * dgh_mesh_set_boundary. */
+int dgh_matvec_dirichlet(dgh_mesh m, int prec, const void* x_dev, void* y_dev, void* stream);
+/* Diagonal of the Dirichlet-masked operator: 1 on boundary nodes.  Replaces
+ * DirichletLaplacianMatGetDiagonal and DirichletJacobianMatGetDiagonal
+ * (examples/src/backend/omgJac.C:1058-1160). */
+int dgh_diagonal_dirichlet(dgh_mesh m, int prec, void* d_dev, void* stream);
 /* Y = A X for k right-hand sides at once (k a multiple of 8, at most 32): X and Y are
  * [N][k] row-major (node-major, right-hand side fastest).  The reference applies
  * Jacobian1MatMult / Jacobian2MatMult (odaJac.C:290-374, omgJac.C:1687-1775) to one vector

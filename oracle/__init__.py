@@ -90,6 +90,32 @@ def diagonal(level, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0):
     return d
 
 
+def matvec_dirichlet(level, bdy, x, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0):
+    """Dirichlet-masked product (or_matvec_dirichlet); bdy[n_node] nonzero on boundary nodes."""
+    st = _st()
+    x = np.ascontiguousarray(x, np.float64)
+    b = np.ascontiguousarray(bdy, np.uint8)
+    y = np.zeros(level.n_node)
+    k, m = _coef(level, kappa, mass)
+    lib().or_matvec_dirichlet(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), C.c_uint32(level.max_depth),
+                              _p(level.nlist), _p(level.child), _p(level.hnmask), _p(level.lev), _p(b), _p(k),
+                              _p(m), C.c_double(lap_fac), C.c_double(mass_fac), _p(st["lap2"]), _p(st["mass2"]),
+                              _p(x), _p(y))
+    return y
+
+
+def diagonal_dirichlet(level, bdy, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0):
+    st = _st()
+    b = np.ascontiguousarray(bdy, np.uint8)
+    d = np.zeros(level.n_node)
+    k, m = _coef(level, kappa, mass)
+    lib().or_diagonal_dirichlet(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), C.c_uint32(level.max_depth),
+                                _p(level.nlist), _p(level.child), _p(level.hnmask), _p(level.lev), _p(b), _p(k),
+                                _p(m), C.c_double(lap_fac), C.c_double(mass_fac), _p(st["lap2"]),
+                                _p(st["mass2"]), _p(d))
+    return d
+
+
 class _OrLevel(C.Structure):
     _fields_ = [("E", C.c_uint32), ("N", C.c_uint32), ("max_depth", C.c_uint32),
                 ("nlist", C.c_void_p), ("child", C.c_void_p), ("hnmask", C.c_void_p),

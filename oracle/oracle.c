@@ -176,6 +176,71 @@ void or_matvec(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist,
   }
 }
 
+/* DirichletLaplacianMatMult / DirichletJacobianMatMult (examples/src/backend/omgJac.C:859-1056):
+ * the element loop of or_matvec in which a boundary row k is assigned out[idx[k]] = in[idx[k]]
+ * and a non-boundary row adds only the columns j whose node is not on the boundary.  The
+ * reference forms fac1 = h/2 and fac2 = h^3/8 (uniform coefficients 1); kappa/mass and
+ * lap_fac/mass_fac scale them as in or_matvec. */
+void or_matvec_dirichlet(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist,
+                         const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+                         const uint8_t* bdy, const double* kappa, const double* mass, double lap_fac,
+                         double mass_fac, const double* lap2, const double* mass2, const double* x,
+                         double* y) {
+  memset(y, 0, sizeof(double) * N);
+  const double hFac = 1.0 / ((double)(1u << (maxD - 1)));
+  for (uint32_t e = 0; e < E; ++e) {
+    const double h = hFac * (double)(1u << (maxD - lev[e]));
+    const double k1 = kappa ? kappa[e] : lap_fac;
+    const double k2 = kappa ? (mass ? mass[e] : 0.0) : mass_fac;
+    const double fac1 = k1 * h / 2.0;
+    const double fac2 = k2 * h * h * h / 8.0;
+    const int c = child[e];
+    int t = 0;
+    if (hnmask[e]) t = or_etype(c, hnmask[e]);
+    const double* L = lap2 + ((size_t)c * 18 + t) * 64;
+    const double* M = mass2 + ((size_t)c * 18 + t) * 64;
+    const uint32_t* idx = nlist + (size_t)e * 8;
+    for (int k = 0; k < 8; ++k) {
+      if (bdy[idx[k]]) {
+        y[idx[k]] = x[idx[k]];
+        continue;
+      }
+      for (int j = 0; j < 8; ++j)
+        if (!bdy[idx[j]]) y[idx[k]] += ((fac1 * L[k * 8 + j]) + (fac2 * M[k * 8 + j])) * x[idx[j]];
+    }
+  }
+}
+
+/* DirichletLaplacianMatGetDiagonal / DirichletJacobianMatGetDiagonal (omgJac.C:1058-1160):
+ * boundary entries are set to 1, the others accumulate the element diagonals. */
+void or_diagonal_dirichlet(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist,
+                           const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+                           const uint8_t* bdy, const double* kappa, const double* mass,
+                           double lap_fac, double mass_fac, const double* lap2, const double* mass2,
+                           double* d) {
+  memset(d, 0, sizeof(double) * N);
+  const double hFac = 1.0 / ((double)(1u << (maxD - 1)));
+  for (uint32_t e = 0; e < E; ++e) {
+    const double h = hFac * (double)(1u << (maxD - lev[e]));
+    const double k1 = kappa ? kappa[e] : lap_fac;
+    const double k2 = kappa ? (mass ? mass[e] : 0.0) : mass_fac;
+    const double fac1 = k1 * h / 2.0;
+    const double fac2 = k2 * h * h * h / 8.0;
+    const int c = child[e];
+    int t = 0;
+    if (hnmask[e]) t = or_etype(c, hnmask[e]);
+    const double* L = lap2 + ((size_t)c * 18 + t) * 64;
+    const double* M = mass2 + ((size_t)c * 18 + t) * 64;
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t n = nlist[(size_t)e * 8 + k];
+      if (bdy[n])
+        d[n] = 1.0;
+      else
+        d[n] += ((fac1 * L[k * 9]) + (fac2 * M[k * 9]));
+    }
+  }
+}
+
 /* Jacobian1MatGetDiagonal (odaJac.C:160-223), JAC_TYPE2_ELEM_DIAG_BLOCK (omgJac.C:1594-1612) */
 void or_diagonal(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist,
                  const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,

@@ -13,6 +13,10 @@
  *   or_matvec/diagonal  regular-grid closed form (textbook trilinear Q1 element), brute
  *                       force conforming assembly on tiny meshes, A*1 = mass*1
  *   or_restrict/prolong R^T = P exactly, R*1 = 1 on fine nodes, coarse-node injection
+ *   or_*_dirichlet      the unmasked or_matvec/or_diagonal with the boundary columns zeroed
+ *                       and the boundary rows replaced by the identity (a different loop),
+ *                       symmetry of the interior block; boundary flags against the node
+ *                       anchors' coordinates
  *   or_mg_* / or_pcg    textbook CG / V-cycle checks: symmetric preconditioner,
  *                       monotone A-norm error, dense-solve agreement on tiny meshes.
  *                       PETSc (the reference's KSPCG/PCMG/Richardson/Jacobi/LU) is not in
@@ -40,6 +44,16 @@ void or_diagonal(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nli
                  const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
                  const double* kappa, const double* mass, double lap_fac, double mass_fac,
                  const double* lap2, const double* mass2, double* d);
+void or_matvec_dirichlet(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist,
+                         const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+                         const uint8_t* bdy, const double* kappa, const double* mass, double lap_fac,
+                         double mass_fac, const double* lap2, const double* mass2, const double* x,
+                         double* y);
+void or_diagonal_dirichlet(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist,
+                           const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+                           const uint8_t* bdy, const double* kappa, const double* mass,
+                           double lap_fac, double mass_fac, const double* lap2, const double* mass2,
+                           double* d);
 
 /* One DA level as the transfer loops see it. */
 typedef struct {
