@@ -165,6 +165,7 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="workgroup size with --tile (0: library default)")
     ap.add_argument("--sweep-tiles", action="store_true", help="time the MatVec for several tile sizes")
     ap.add_argument("--no-multi", action="store_true", help="skip the 32 right-hand-side batch")
+    ap.add_argument("--no-dirichlet", action="store_true", help="skip the Dirichlet-masked MatVec")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-damping PCG")
     ap.add_argument("--pcg-trace", action="store_true",
                     help="only three eager (no hipGraph) mixed-precision PCG solves, for a kernel trace")
@@ -295,6 +296,24 @@ def main():
     torch.cuda.synchronize()
     ms32 = ev0.elapsed_time(ev1) / args.steps
 
+    # Dirichlet-masked FP64 MatVec on the same mesh: boundary columns dropped, identity rows
+    # (a mask pass over x, the operator, and the boundary rows)
+    dirichlet = {}
+    if not args.no_dirichlet:
+        bdy = damg.boundary(damg.nlevels - 1)
+        mf.set_boundary(bdy)
+        for _ in range(args.warmup):
+            mf.matvec_dirichlet(x, y, stream=stream)
+        ev0.record(stream)
+        for _ in range(args.steps):
+            mf.matvec_dirichlet(x, y, stream=stream)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        msd = ev0.elapsed_time(ev1) / args.steps
+        log(f"Dirichlet MatVec FP64: {msd*1e3:.1f} us/call ({int(bdy.sum())} boundary nodes), "
+            f"{N / (msd * 1e-3) / 1e9:.2f} GDoF/s")
+        dirichlet = {"matvec_dirichlet_f64_us": round(msd * 1e3, 2), "n_boundary_nodes": int(bdy.sum())}
+
     extra = {}
     if not args.no_pcg:
         b = torch.tensor(np.random.default_rng(22).uniform(-1, 1, N), dtype=torch.float64, device=dev)
@@ -347,6 +366,7 @@ def main():
                      "cold_cache_frac": round(alg_bytes / (cold_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "matvec_fp32_ms": round(ms32, 5),
         "matvec_fp32_gdofs": round(N / (ms32 * 1e-3) / 1e9, 4),
+        **dirichlet,
     }
     out.update(extra)
     if not args.no_cpu:
