@@ -794,10 +794,8 @@ int dgh_mesh_set_boundary(dgh_mesh m, const uint8_t* bdy) {
     require_init();
     if (!m) fail(DGH_ERR_ARG, "NULL mesh");
     m->bdy.release();
-    m->dir_x.release();
     if (!bdy || !m->N) return;
     m->bdy.upload(bdy, m->N);
-    m->dir_x.alloc(m->N);
   });
 }
 
@@ -809,14 +807,10 @@ int dgh_matvec_dirichlet(dgh_mesh m, int prec, const void* x, void* y, void* str
     if (m->N && !m->bdy.p) fail(DGH_ERR_STATE, "no boundary flags (dgh_mesh_set_boundary)");
     if (!m->N) return;
     hipStream_t st = static_cast<hipStream_t>(stream);
+    // one operator launch: the gather stages boundary x as 0, the store writes x on boundary rows
     auto run = [&](auto* xt, auto* yt) {
       using T = std::remove_const_t<std::remove_pointer_t<decltype(xt)>>;
-      T* xm = reinterpret_cast<T*>(m->dir_x.p);
-      dirichlet_mask_kernel<T><<<blocks_for(m->N), VBLOCK, 0, st>>>(m->N, m->bdy.p, xt, xm);
-      HIP_CHECK(hipGetLastError());
-      launch_matvec<T>(m, xm, yt, st);
-      dirichlet_rows_kernel<T><<<blocks_for(m->N), VBLOCK, 0, st>>>(m->N, m->bdy.p, xt, yt);
-      HIP_CHECK(hipGetLastError());
+      launch_op<T, OP_DIRMV>(m, OpIO<T>{xt, yt, nullptr, nullptr, T(0), nullptr, m->bdy.p}, st);
     };
     if (prec == DGH_F64)
       run(static_cast<const double*>(x), static_cast<double*>(y));

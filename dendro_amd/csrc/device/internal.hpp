@@ -133,7 +133,6 @@ struct dgh_mesh_s {
   int mvariant = 0;             // batched FP32 products: 0 vector ALUs, 1 matrix cores
   uint64_t gen = 0;             // bumped whenever device tilings or coefficients are rebuilt
   DevBuf<uint8_t> bdy;          // [N] boundary flags of the Dirichlet operators (empty: none)
-  DevBuf<double> dir_x;         // [N] x with the boundary entries zeroed (FP64 or FP32)
 };
 
 struct dgh_transfer_s {

@@ -46,7 +46,9 @@ __device__ __forceinline__ void elem_factors(uint32_t lev, uint32_t maxD, double
 // MATVEC: y = A x;  DIAG: y = diag(A);  RESID: y = b - A x;
 // SMOOTH: y = x + omega (invd (b - A x))  (one Richardson + Jacobi sweep, y must not alias x);
 // CG: y = A x and the tile's partial sum of x.y (the CG product p.Ap).
-enum OpMode { OP_MATVEC = 0, OP_DIAG = 1, OP_RESID = 2, OP_SMOOTH = 3, OP_CG = 4 };
+// OP_DIRMV: the Dirichlet-masked MatVec (examples/src/backend/omgJac.C:859-1056): x of
+// boundary nodes is staged as 0 (their columns drop out) and their rows return x itself.
+enum OpMode { OP_MATVEC = 0, OP_DIAG = 1, OP_RESID = 2, OP_SMOOTH = 3, OP_CG = 4, OP_DIRMV = 5 };
 
 template <class T>
 struct OpIO {
@@ -56,6 +58,7 @@ struct OpIO {
   const T* invd;   // inverse diagonal (SMOOTH)
   T omega;         // Richardson scale (SMOOTH)
   double* part;    // CG: per-tile partial sums of x.y
+  const uint8_t* bdy;   // DIRMV: boundary flags per node (nonzero = boundary)
 };
 
 // Symmetric 8x8 stencils are kept as their upper triangles (36 entries, row-major):
@@ -399,6 +402,16 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         v[q] = (i < rec.n_touch) ? x[nid[q]] : T(0);
       }
+      if (MODE == OP_DIRMV) {   // flags loaded beside x (both wait on the node ids only)
+        uint8_t bf[XCHUNK];
+#pragma unroll
+        for (int q = 0; q < XCHUNK; ++q) {
+          const uint32_t i = base + threadIdx.x + q * BLOCK;
+          bf[q] = (i < rec.n_touch) ? io.bdy[nid[q]] : uint8_t(0);
+        }
+#pragma unroll
+        for (int q = 0; q < XCHUNK; ++q) v[q] = bf[q] ? T(0) : v[q];
+      }
 #pragma unroll
       for (int q = 0; q < XCHUNK; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
@@ -456,6 +469,8 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
     const T v = finite ? T(s) : T(__builtin_nan(""));
     if (MODE == OP_MATVEC || MODE == OP_DIAG) {
       io.y[n] = v;
+    } else if (MODE == OP_DIRMV) {
+      io.y[n] = io.bdy[n] ? x[n] : v;
     } else if (MODE == OP_RESID) {
       io.y[n] = io.b[n] - v;
     } else if (MODE == OP_SMOOTH) {

@@ -469,16 +469,7 @@ __global__ void symmetrize_kernel(int n, int ld, T* __restrict__ A) {
   A[size_t(j) * ld + i] = A[size_t(i) * ld + j];
 }
 
-// Dirichlet operators (examples/src/backend/omgJac.C:859-1160): the element loops drop the
-// columns of boundary nodes and set the rows of boundary nodes to the identity.  The same
-// product is A applied to x with its boundary entries zeroed, followed by the identity rows.
-template <class T>
-__global__ void __launch_bounds__(VBLOCK)
-dirichlet_mask_kernel(uint32_t n, const uint8_t* __restrict__ bdy, const T* __restrict__ x, T* __restrict__ xm) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) xm[i] = bdy[i] ? T(0) : x[i];
-}
-
+// Boundary rows of the Dirichlet operators (examples/src/backend/omgJac.C:1058-1160):
 // y_i = x_i on boundary nodes (x == nullptr: y_i = 1, the diagonal); other entries untouched
 template <class T>
 __global__ void __launch_bounds__(VBLOCK)

@@ -211,8 +211,8 @@ int dgh_matvec(dgh_mesh m, int prec, const void* x_dev, void* y_dev, void* strea
  * Jacobian2MatGetDiagonal (omgJac.C:1594-1662). */
 int dgh_diagonal(dgh_mesh m, int prec, void* d_dev, void* stream);
 /* Boundary nodes of the mesh for the Dirichlet operators: bdy[N] host flags (nonzero = on
- * the boundary, as dgh_damg_level_boundary returns them); NULL removes them.  Allocates an
- * N-vector of scratch for dgh_matvec_dirichlet.  Replaces SetDirichletJacContexts
+ * the boundary, as dgh_damg_level_boundary returns them), uploaded once (1 byte per node);
+ * NULL removes them.  Replaces SetDirichletJacContexts
  * (examples/src/backend/omgJac.C:508-586). */
 int dgh_mesh_set_boundary(dgh_mesh m, const uint8_t* bdy);
 /* Dirichlet-masked y = A x: boundary rows are the identity (y_i = x_i) and boundary columns
