@@ -109,12 +109,18 @@ def _mesh(h, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["laplacian", "jacobian", "var"])
-def test_matvec_dirichlet_gpu(dev, hier_small, mode):
+@pytest.mark.parametrize("mode,lane,block", [("laplacian", 0, 0), ("jacobian", 0, 0), ("var", 0, 0),
+                                             ("var", 1, 0), ("jacobian", 1, 256), ("var", 0, 1024)])
+def test_matvec_dirichlet_gpu(dev, hier_small, mode, lane, block):
+    """lane: stencil rows per lane from LDS; block: 256 / 1024 threads per tile (0: default)"""
     h = hier_small
     F = h.fine
     b = h.damg.boundary(h.n - 1)
     m, kw = _mesh(h, mode)
+    if block:
+        m.set_tile(2048, block)
+    m.set_lane_stencils(lane)
+    assert m.tile_info()["lane_stencils"] == lane
     m.set_boundary(b)
     x = np.random.default_rng(11).uniform(-1, 1, F.n_node)
     y_ref = oracle.matvec_dirichlet(F, b, x, **kw)
