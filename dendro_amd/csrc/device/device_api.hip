@@ -828,16 +828,15 @@ int dgh_diagonal_dirichlet(dgh_mesh m, int prec, void* d, void* stream) {
     if (m->N && !m->bdy.p) fail(DGH_ERR_STATE, "no boundary flags (dgh_mesh_set_boundary)");
     if (!m->N) return;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (prec == DGH_F64) {
-      launch_diag<double>(m, static_cast<double*>(d), st);
-      dirichlet_rows_kernel<double><<<blocks_for(m->N), VBLOCK, 0, st>>>(m->N, m->bdy.p, nullptr, static_cast<double*>(d));
-    } else if (prec == DGH_F32) {
-      launch_diag<float>(m, static_cast<float*>(d), st);
-      dirichlet_rows_kernel<float><<<blocks_for(m->N), VBLOCK, 0, st>>>(m->N, m->bdy.p, nullptr, static_cast<float*>(d));
-    } else {
+    // the diagonal's store writes 1 on boundary nodes
+    if (prec == DGH_F64)
+      launch_op<double, OP_DIAG>(m, OpIO<double>{nullptr, static_cast<double*>(d), nullptr, nullptr, 0.0, nullptr,
+                                                 m->bdy.p}, st);
+    else if (prec == DGH_F32)
+      launch_op<float, OP_DIAG>(m, OpIO<float>{nullptr, static_cast<float*>(d), nullptr, nullptr, 0.f, nullptr,
+                                               m->bdy.p}, st);
+    else
       fail(DGH_ERR_ARG, "unknown precision %d", prec);
-    }
-    HIP_CHECK(hipGetLastError());
   });
 }
 

@@ -58,7 +58,7 @@ struct OpIO {
   const T* invd;   // inverse diagonal (SMOOTH)
   T omega;         // Richardson scale (SMOOTH)
   double* part;    // CG: per-tile partial sums of x.y
-  const uint8_t* bdy;   // DIRMV: boundary flags per node (nonzero = boundary)
+  const uint8_t* bdy;   // DIRMV, and DIAG if set: boundary flags per node (nonzero = boundary)
 };
 
 // Symmetric 8x8 stencils are kept as their upper triangles (36 entries, row-major):
@@ -467,8 +467,10 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
     const uint32_t n = nodes[i];
     const double s = fx_value<T>(acc[i], es);
     const T v = finite ? T(s) : T(__builtin_nan(""));
-    if (MODE == OP_MATVEC || MODE == OP_DIAG) {
+    if (MODE == OP_MATVEC) {
       io.y[n] = v;
+    } else if (MODE == OP_DIAG) {   // with boundary flags, the Dirichlet diagonal: 1 there
+      io.y[n] = (io.bdy && io.bdy[n]) ? T(1) : v;
     } else if (MODE == OP_DIRMV) {
       io.y[n] = io.bdy[n] ? x[n] : v;
     } else if (MODE == OP_RESID) {

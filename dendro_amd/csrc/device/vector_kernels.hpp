@@ -469,13 +469,4 @@ __global__ void symmetrize_kernel(int n, int ld, T* __restrict__ A) {
   A[size_t(j) * ld + i] = A[size_t(i) * ld + j];
 }
 
-// Boundary rows of the Dirichlet operators (examples/src/backend/omgJac.C:1058-1160):
-// y_i = x_i on boundary nodes (x == nullptr: y_i = 1, the diagonal); other entries untouched
-template <class T>
-__global__ void __launch_bounds__(VBLOCK)
-dirichlet_rows_kernel(uint32_t n, const uint8_t* __restrict__ bdy, const T* __restrict__ x, T* __restrict__ y) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && bdy[i]) y[i] = x ? x[i] : T(1);
-}
-
 }  // namespace dgh
