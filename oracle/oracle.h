@@ -17,11 +17,17 @@
  *                       and the boundary rows replaced by the identity (a different loop),
  *                       symmetry of the interior block; boundary flags against the node
  *                       anchors' coordinates
- *   or_mg_* / or_pcg    textbook CG / V-cycle checks: symmetric preconditioner,
- *                       monotone A-norm error, dense-solve agreement on tiny meshes.
- *                       PETSc (the reference's KSPCG/PCMG/Richardson/Jacobi/LU) is not in
- *                       the reference tree, so its iterates are "parity unpinned" beyond
- *                       these mathematical checks.
+ *   or_diagonal         diag(A) of the constrained trilinear FEM assembly
+ *   or_mg_vcycle        a dense textbook multiplicative V-cycle (Richardson + Jacobi
+ *                       smoothing from zero guesses, exact coarsest solve) on the assembled
+ *                       level matrices and interpolation matrices, for several omega /
+ *                       sweep counts and with a constant null space; wrong omega or sweep
+ *                       counts fail (negative controls)
+ *   or_pcg              iterate k equals the A-norm minimiser over K_k(BA, B r_0) of the
+ *                       dense V-cycle B (k = 1..5), stopping norms |r_k| / |B r_k|; another
+ *                       preconditioner's minimisers fail (negative control).  PETSc's own
+ *                       floating-point order (the reference's KSPCG/PCMG/LU, not in the
+ *                       reference tree) is pinned only up to rounding.
  */
 #ifndef DENDRO_ORACLE_H
 #define DENDRO_ORACLE_H

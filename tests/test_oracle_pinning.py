@@ -7,13 +7,23 @@
   the node lists, the hanging masks and the element factors h/2, h^3/8 together.
 - The prolongation equals trilinear interpolation of the constrained coarse FE function at
   the fine nodes; restriction is its exact transpose.
-- The V-cycle with equal pre/post sweeps is a symmetric operator; PCG converges to the
-  dense solution.
-PETSc's own arithmetic is not in the reference tree: beyond these checks the solver
-iterates are "parity unpinned".
+- The diagonal equals diag(A) of that assembled matrix; the Dirichlet-masked product and
+  diagonal equal the masked matrix (boundary rows identity, boundary columns dropped).
+- One V-cycle equals a dense textbook multiplicative V-cycle (Briggs, Henson and McCormick,
+  "A Multigrid Tutorial", ch. 3; PETSc PCMG with Richardson + Jacobi smoothing, zero initial
+  guesses on every level, exact coarsest solve) built from the assembled level matrices
+  and the interpolation matrices pinned above, for several omega / sweep counts and with a
+  constant null space; a wrong omega or sweep count fails (negative controls).
+- The PCG iterate x_k equals the A-norm minimiser of the error over the Krylov space
+  K_k(BA, B r_0) of the dense V-cycle B (Saad, "Iterative Methods for Sparse Linear
+  Systems", Alg. 9.1 and Prop. 6.11), for k = 1..5, and its stopping-test norms are
+  |r_k| or |B r_k| of those iterates; the minimiser of another preconditioner fails.
+PETSc's own floating-point order (ddot, LU pivoting) is not in the reference tree; the
+iterates are pinned to these definitions up to rounding.
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import dendro_amd as da
 import oracle
@@ -107,6 +117,20 @@ def test_operator_is_constrained_q1(tiny):
     Ao = dense_from_oracle(L, N, kappa=kappa, mass=mass)
     assert np.abs(Ao - A).max() <= 1e-13 * np.abs(A).max()
     assert np.abs(Ao - Ao.T).max() <= 1e-15 * np.abs(Ao).max()
+    # the Jacobi diagonal is diag(A) of the constrained assembly
+    d = oracle.diagonal(L, kappa=kappa, mass=mass)
+    assert np.abs(d - np.diag(A)).max() <= 1e-13 * np.abs(A).max()
+    # Dirichlet masking: boundary rows are the identity, boundary columns are dropped
+    bdy = h.damg.boundary(h.n - 1).astype(bool)
+    assert 0 < bdy.sum() < N
+    Ad = np.where(bdy[:, None] | bdy[None, :], 0.0, A)
+    Ad[bdy, bdy] = 1.0
+    dd = oracle.diagonal_dirichlet(L, bdy, kappa=kappa, mass=mass)
+    assert np.abs(dd - np.diag(Ad)).max() <= 1e-13 * np.abs(A).max()
+    x = np.random.default_rng(3).uniform(-1, 1, N)
+    yd = oracle.matvec_dirichlet(L, bdy, x, kappa=kappa, mass=mass)
+    assert np.abs(yd - Ad @ x).max() <= 1e-13 * np.abs(A).max() * np.abs(x).sum() / N * 8
+    assert np.array_equal(yd[bdy], x[bdy])
 
 
 def test_prolongation_is_constrained_interpolation(tiny):
@@ -184,3 +208,170 @@ def test_vcycle_null_space_mean_free():
     assert conv and abs(x.mean()) <= 1e-12
     r = (b - b.mean()) - mg.apply(h.n - 1, x)
     assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(b - b.mean())
+
+
+# ------------------------------------------------------------------ dense V-cycle and PCG
+def _sparse_from_columns(apply, n, m=None):
+    rows, cols, vals = [], [], []
+    e = np.zeros(n)
+    for j in range(n):
+        e[:] = 0.0
+        e[j] = 1.0
+        c = apply(e)
+        nz = np.nonzero(c)[0]
+        rows.append(nz)
+        cols.append(np.full(len(nz), j))
+        vals.append(c[nz])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(m or n, n))
+
+
+class DenseMG:
+    """Textbook multiplicative V-cycle on assembled matrices: A_l from the oracle operator
+    (pinned to the constrained Q1 assembly), P_l from the oracle prolongation (pinned to
+    constrained interpolation), R_l = P_l^T, Jacobi-preconditioned Richardson smoothing
+    x <- x + omega D^-1 (b - A x) from a zero guess, an exact coarsest solve (the mean-free
+    one with a constant null space, whose right-hand side is projected first)."""
+
+    def __init__(self, h, coefs, null_space, P=None):
+        self.h, self.ns = h, null_space
+        self.A = []
+        for l in range(h.n):
+            L = h.levels[l]
+            k, m = coefs[l]
+            self.A.append(_sparse_from_columns(lambda e, L=L, k=k, m=m: oracle.matvec(L, e, kappa=k, mass=m),
+                                               L.n_node))
+        self.P = P or []
+        for l in range(len(self.P) + 1, h.n):
+            C, F = h.levels[l - 1], h.levels[l]
+            self.P.append(_sparse_from_columns(lambda e, C=C, F=F: oracle.prolong(C, F, e), C.n_node, F.n_node))
+        A0 = self.A[0].toarray()
+        self.A0inv = np.linalg.pinv(A0) if null_space else np.linalg.inv(A0)
+
+    def vcycle(self, B, omega, nu_pre, nu_post, l=None):
+        top = l is None
+        l = self.h.n - 1 if top else l
+        if l == 0:
+            X = self.A0inv @ ((B - B.mean(0)) if self.ns else B)
+        else:
+            A = self.A[l]
+            dinv = (1.0 / A.diagonal())[:, None]
+            X = np.zeros_like(B)
+            for _ in range(nu_pre):
+                X = X + omega * dinv * (B - A @ X)
+            Xc = self.vcycle(self.P[l - 1].T @ (B - A @ X), omega, nu_pre, nu_post, l - 1)
+            X = X + self.P[l - 1] @ Xc
+            for _ in range(nu_post):
+                X = X + omega * dinv * (B - A @ X)
+        if top and self.ns:
+            X = X - X.mean(0)
+        return X
+
+    def krylov_minimisers(self, b, kmax, omega, nu):
+        """x_k = argmin |x - x*|_A over K_k(BA, B b) for k = 1..kmax (B: the V-cycle with
+        nu + nu sweeps, symmetric), from an orthonormal basis grown one vector at a time."""
+        A = self.A[-1]
+        if self.ns:
+            b = b - b.mean()
+        Bv = lambda v: self.vcycle(v[:, None], omega, nu, nu)[:, 0]
+        q = Bv(b)
+        Q = [q / np.linalg.norm(q)]
+        xs = []
+        for k in range(1, kmax + 1):
+            V = np.stack(Q, 1)
+            xs.append(V @ np.linalg.solve(V.T @ (A @ V), V.T @ b))
+            w = Bv(A @ Q[-1])
+            for _ in range(2):   # Gram-Schmidt, twice
+                for qq in Q:
+                    w = w - (qq @ w) * qq
+            Q.append(w / np.linalg.norm(w))
+        return xs
+
+
+@pytest.fixture(scope="module")
+def dense_tiny(hier_tiny):
+    h = hier_tiny
+    coefs = var_coefficients(h, mass_base=1.0)
+    return h, coefs, DenseMG(h, coefs, False)
+
+
+@pytest.fixture(scope="module")
+def dense_tiny_ns(hier_tiny, dense_tiny):
+    h = hier_tiny
+    coefs = var_coefficients(h, mass_base=None)
+    return h, coefs, DenseMG(h, coefs, True, P=dense_tiny[2].P)
+
+
+def _probe(n, seed):
+    """8 basis vectors and 8 random vectors as the columns of one block"""
+    rng = np.random.default_rng(seed)
+    E = np.zeros((n, 8))
+    E[rng.choice(n, 8, replace=False), np.arange(8)] = 1.0
+    return np.concatenate([E, rng.uniform(-1, 1, (n, 8))], 1)
+
+
+def _oracle_vcycle(mg, B):
+    return np.stack([mg.vcycle(B[:, j]) for j in range(B.shape[1])], 1)
+
+
+@pytest.mark.parametrize("ns", [False, True])
+@pytest.mark.parametrize("omega,nu_pre,nu_post", [(2.0 / 3.0, 2, 2), (0.6, 1, 3), (0.8, 3, 0)])
+def test_vcycle_equals_dense_textbook(dense_tiny, dense_tiny_ns, ns, omega, nu_pre, nu_post):
+    h, coefs, D = dense_tiny_ns if ns else dense_tiny
+    mg = oracle.MG([(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)],
+                   omega=omega, nu_pre=nu_pre, nu_post=nu_post, null_space=ns)
+    B = _probe(h.fine.n_node, 1)
+    if ns:
+        B = B - B.mean(0)
+    Zo = _oracle_vcycle(mg, B)
+    Zd = D.vcycle(B, omega, nu_pre, nu_post)
+    err = np.abs(Zo - Zd).max() / np.abs(Zd).max()
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("wrong", [dict(omega=0.5), dict(nu_pre=1), dict(nu_post=3)])
+def test_vcycle_negative_controls(dense_tiny, wrong):
+    """An oracle V-cycle with another omega or sweep count differs from the textbook one
+    with omega 2/3 and 2 + 2 sweeps far beyond the tolerance above."""
+    h, coefs, D = dense_tiny
+    kw = dict(omega=2.0 / 3.0, nu_pre=2, nu_post=2)
+    kw.update(wrong)
+    mg = oracle.MG([(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)], **kw)
+    B = _probe(h.fine.n_node, 2)
+    Zd = D.vcycle(B, 2.0 / 3.0, 2, 2)
+    assert np.abs(_oracle_vcycle(mg, B) - Zd).max() / np.abs(Zd).max() > 1e-4
+
+
+@pytest.mark.parametrize("ns", [False, True])
+@pytest.mark.parametrize("norm_type", [0, 1])
+def test_pcg_iterates_are_krylov_minimisers(dense_tiny, dense_tiny_ns, ns, norm_type):
+    h, coefs, D = dense_tiny_ns if ns else dense_tiny
+    mg = oracle.MG([(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)],
+                   omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=ns)
+    b = np.random.default_rng(12).uniform(-1, 1, h.fine.n_node)
+    xs = D.krylov_minimisers(b, 5, 2.0 / 3.0, 2)
+    bp = b - b.mean() if ns else b
+    A = D.A[-1]
+    for k in range(1, 6):
+        xo, its, _, hist = mg.pcg(b, rtol=1e-30, max_it=k, norm_type=norm_type)
+        assert its == k
+        assert rel_err(xo, xs[k - 1]) <= 1e-10, (k, rel_err(xo, xs[k - 1]))
+        r = bp - A @ xs[k - 1]
+        nrm = np.linalg.norm(D.vcycle(r[:, None], 2.0 / 3.0, 2, 2)[:, 0]) if norm_type else np.linalg.norm(r)
+        # recurrence residual against the residual of the minimiser: rounding of order
+        # eps |A| |x|, stated relative to the initial norm
+        assert abs(hist[k] - nrm) <= 1e-10 * hist[0], (k, hist[k], nrm)
+    r0 = np.linalg.norm(D.vcycle(bp[:, None], 2.0 / 3.0, 2, 2)[:, 0]) if norm_type else np.linalg.norm(bp)
+    assert abs(hist[0] - r0) <= 1e-12 * r0
+
+
+def test_pcg_negative_control(dense_tiny):
+    """The Krylov minimisers of a different preconditioner (omega 0.5) are not the
+    iterates of the oracle's PCG with omega 2/3."""
+    h, coefs, D = dense_tiny
+    mg = oracle.MG([(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)],
+                   omega=2.0 / 3.0, nu_pre=2, nu_post=2)
+    b = np.random.default_rng(12).uniform(-1, 1, h.fine.n_node)
+    xs = D.krylov_minimisers(b, 3, 0.5, 2)
+    for k in (2, 3):
+        xo, _, _, _ = mg.pcg(b, rtol=1e-30, max_it=k, norm_type=0)
+        assert rel_err(xo, xs[k - 1]) > 1e-6
