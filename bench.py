@@ -1,20 +1,25 @@
 """Benchmark of the octree FEM solve path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2] [--no-cpu] ...
 
-Workload (one GPU): octree from 1M seeded log-normal points (per-axis lognormal sigma=1,
-rescaled to the unit cube), max depth 12, 2:1 balanced; variable-coefficient Poisson with
-element coefficient kappa_e = 10^U(-1,1) (seed 21), zero mass (constant null space
-projected); 5-level GMG-PCG, damped Jacobi omega=2/3 with 2+2 sweeps, b ~ U[-1,1] (seed 22).
+Headline workload (default, "c3"): octree from 8M seeded points (4M U(0,1)^3 with seed 3 and
+4M in 8 Gaussian clusters of sd 0.02 with centres from seed 31), max depth 14, 2:1 balanced;
+Helmholtz operator (Jacobian1, lambda = mu = 1); 8-level GMG-PCG, damped Jacobi omega = 2/3
+with 2+2 sweeps, b ~ U[-1,1] (seed 12).  A step is one FP64 operator application over the
+whole finest mesh (44.3M elements, 31.5M nodes); the timed region is K steps with x already
+in HBM.  The line also carries the PCG time to a 1e-8 true relative residual on the same
+octree (plain FP64, FP32 V-cycle, FP32 V-cycle with learned damping), the roofline of the
+MatVec kernel and the CPU oracle timed on a bounded sample of the same MatVec (one thread
+and all host threads).
 
-A step is one FP64 operator application (Jacobian2 MatVec) over the whole finest mesh; the
-timed region is K steps with the input already in HBM.  The line also carries the PCG
-time to a 1e-8 true relative residual (plain FP64 and FP32 V-cycle), the roofline of the
-MatVec kernel and the CPU oracle timed on a bounded sample of the same MatVec.
+Secondary workload ("c2", also run by default under the key "c2"): octree from 1M log-normal
+points, max depth 12, kappa_e = 10^U(-1,1), zero mass with the constant null space projected,
+5 levels; its MatVec, PCG, Dirichlet-masked MatVec and the 32 right-hand-side batch.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -26,85 +31,220 @@ sys.path.insert(0, ROOT)
 METRIC = "PCG s to 1e-8 rel resid; MatVec GDoF/s + % HBM peak; vs plain FP64 GPU"
 HBM_PEAK_GBS = 8000.0
 
+WORKLOADS = {
+    "c3": dict(name="mixed8M_d14", kind="MIXED", npts=8000000, seed=3, cluster_seed=31, depth=14, nlevels=8,
+               coef="helmholtz", null_space=0, rhs_seed=12,
+               desc="Helmholtz (Jacobian1, lambda=mu=1) MatVec + 8-level GMG-PCG, octree from 8M points "
+                    "(4M uniform + 4M in 8 Gaussian clusters), max depth 14, FP64"),
+    "c2": dict(name="lognormal1M_d12", kind="LOGNORMAL", npts=1000000, seed=2, cluster_seed=0, depth=12,
+               nlevels=5, coef="lognormal", null_space=1, rhs_seed=22,
+               desc="Jacobian2 MatVec + 5-level GMG-PCG, octree from 1M log-normal points, max depth 12, "
+                    "kappa=10^U(-1,1), zero mass, constant null space, FP64"),
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_problem(npts, depth, nlevels):
-    import dendro_amd as da
-    t0 = time.time()
-    pts = da.points(da.PTS_LOGNORMAL, npts, 2)
-    damg = da.DAMG(da.Octree.from_points(pts, depth), nlevels, 500)
-    levels = [damg.level(l) for l in range(damg.nlevels)]
-    kf = 10.0 ** np.random.default_rng(21).uniform(-1, 1, levels[-1].n_elem)
-    coefs = [damg.coarsen_coefficients(kf, None, l) for l in range(damg.nlevels)]
-    trans = [damg.transfer(l) for l in range(1, damg.nlevels)]
-    log(f"mesh: {damg.nlevels} levels, finest E={levels[-1].n_elem} N={levels[-1].n_node} "
-        f"hanging={levels[-1].n_hanging}, coarsest N={levels[0].n_node}, setup {time.time() - t0:.1f}s")
-    return damg, levels, coefs, trans
+class Problem:
+    """A DA hierarchy with its coefficients, device meshes and transfers."""
+
+    def __init__(self, w):
+        import dendro_amd as da
+        self.w = w
+        t0 = time.time()
+        pts = da.points(getattr(da, "PTS_" + w["kind"]), w["npts"], w["seed"], w["cluster_seed"])
+        self.damg = da.DAMG(da.Octree.from_points(pts, w["depth"]), w["nlevels"], 500)
+        del pts
+        self.levels = [self.damg.level(l) for l in range(self.damg.nlevels)]
+        F = self.levels[-1]
+        if w["coef"] == "lognormal":
+            kf = 10.0 ** np.random.default_rng(21).uniform(-1, 1, F.n_elem)
+            self.coefs = [(self.damg.coarsen_coefficients(kf, None, l)[0], None) for l in range(self.damg.nlevels)]
+        else:
+            self.coefs = [(None, None)] * self.damg.nlevels
+        self.host_seconds = time.time() - t0
+        self.meshes = []
+        for l, L in enumerate(self.levels):
+            m = da.Mesh(L)
+            if self.coefs[l][0] is None:
+                m.set_coefficients(None, None, 1.0, 1.0)
+            else:
+                m.set_coefficients(self.coefs[l][0], None)
+            self.meshes.append(m)
+        self.trans = [da.Transfer(self.meshes[l - 1], self.meshes[l], *self.damg.transfer(l))
+                      for l in range(1, len(self.levels))]
+        log(f"{w['name']}: {self.damg.nlevels} levels, finest E={F.n_elem} N={F.n_node} hanging={F.n_hanging}, "
+            f"coarsest N={self.levels[0].n_node}, host build {self.host_seconds:.1f}s")
+
+    @property
+    def fine(self):
+        return self.levels[-1]
+
+    @property
+    def uniform(self):
+        return self.coefs[-1][0] is None
+
+    def alg_bytes(self, b_v=8):
+        """Algorithmic bytes of one MatVec: u32 nlist[8] + packed stencil id/level (2 B) per
+        element, x read once and y written once, plus kappa per element for Jacobian2."""
+        F = self.fine
+        return F.n_elem * (32 + 2) + F.n_node * 2 * b_v + (0 if self.uniform else F.n_elem * b_v)
+
+    def kernel_name(self, block, lane):
+        mass = "true" if self.uniform else "false"
+        uni = "true" if self.uniform else "false"
+        return f"dgh::tile_op_kernel<double, double, {mass}, 0, {block}, {uni}, {'true' if lane else 'false'}>"
+
+    def params(self, **kw):
+        import dendro_amd as da
+        p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
+                              norm_type=da.NORM_UNPRECONDITIONED, null_space=self.w["null_space"], use_graph=1)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
 
 
-def cpu_baseline(F, kappa, budget_s=15.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(P, reps=5):
+    """The oracle's MatVec (the reference's element loop restated in C, -O3) on the full
+    finest mesh of the workload: one warm-up, then the median of `reps` timed calls, on one
+    thread and on every host thread this process may use (at most 16, the host share of one
+    GPU)."""
     import oracle
+    F = P.fine
+    kappa = P.coefs[-1][0]
     x = np.random.default_rng(11).uniform(-1, 1, F.n_node)
-    oracle.matvec(F, x, kappa=kappa, mass=None)          # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        oracle.matvec(F, x, kappa=kappa, mass=None)
-        n += 1
-        dt = time.perf_counter() - t0
-        if dt >= budget_s or n >= 200:
-            break
-    return {"value": F.n_node * n / dt / 1e9, "unit": "GDoF/s", "cores": 1, "kind": "port",
-            "sample": f"{n} single-thread oracle MatVecs (Jacobian2) of the full finest mesh "
-                      f"(E={F.n_elem}, N={F.n_node}), {dt:.1f}s"}
+    threads = min(16, len(os.sched_getaffinity(0)))
+    out = {}
+    for name, fn in (("1", lambda: oracle.matvec(F, x, kappa=kappa)),
+                     ("all", lambda: oracle.matvec_mt(F, x, kappa=kappa, threads=threads))):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        out[name] = sorted(ts)[len(ts) // 2]
+    g1, ga = F.n_node / out["1"] / 1e9, F.n_node / out["all"] / 1e9
+    log(f"CPU oracle MatVec: 1 thread {out['1']*1e3:.0f} ms ({g1:.4f} GDoF/s), {threads} threads "
+        f"{out['all']*1e3:.0f} ms ({ga:.4f} GDoF/s)")
+    return {"value": round(ga, 5), "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "single_thread_value": round(g1, 5), "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"oracle MatVec (C restatement of the reference element loop, -O3) over the full finest "
+                      f"mesh (E={F.n_elem}, N={F.n_node}), median of {reps} calls after a warm-up, on 1 thread "
+                      f"({out['1']:.2f} s per call) and on {threads} threads ({out['all']:.3f} s per call)"}
 
 
-def bench_learned(meshes, trans, N, dev, extra):
-    """FP32 V-cycle PCG with the learned damping: the network is trained on the device on
-    200 batches of solver-generated samples (seeds 5000..5199), then every level keeps the
-    learned weights only where they beat plain Jacobi on fresh samples."""
+def time_matvec(mesh, x, y, steps, warmup, stream, prec=None):
     import torch
     import dendro_amd as da
-    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
-                          norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=da.F32,
-                          use_graph=1, smoother=da.SMOOTHER_LEARNED)
-    s = da.Solver(meshes, trans, p)
-    s.learned_init(1)
-    tr = s.learned_train(steps=200, samples=8, nu=2, refresh=1, lr=3e-3, seed=5000)
-    mask = s.learned_apply(True, samples=8, seed=6000)
-    b = torch.tensor(np.random.default_rng(22).uniform(-1, 1, N), dtype=torch.float64, device=dev)
-    xs = torch.zeros(N, dtype=torch.float64, device=dev)
-    s.solve(b, xs)
-    st, _ = s.solve(b, xs)
-    out = {"seconds": round(st.seconds, 5), "iterations": st.iterations, "rel_true": st.rel_true,
-           "converged": bool(st.converged), "levels_learned_mask": mask,
-           "train_seconds": round(tr.seconds, 3), "objective_first": round(tr.loss_first, 5),
-           "objective_last": round(tr.loss_last, 5)}
-    if "pcg_fp64" in extra:
-        out["speedup_vs_fp64"] = round(extra["pcg_fp64"]["seconds"] / st.seconds, 3)
-    log(f"PCG mixed+learned: {st.iterations} its, {st.seconds*1e3:.1f} ms (levels {mask:#x}); training "
-        f"{tr.seconds:.2f}s objective {tr.loss_first:.4f} -> {tr.loss_last:.4f}")
-    del s
+    prec = da.F64 if prec is None else prec
+    for _ in range(warmup):
+        mesh.matvec(x, y, prec=prec, stream=stream)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        mesh.matvec(x, y, prec=prec, stream=stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps, time.perf_counter() - t0
+
+
+def cold_matvec_ms(mesh, x, y, stream):
+    """Each launch alone after reading a 1 GiB buffer, which evicts the 256 MB Infinity
+    Cache (a read, not a fill: a fill's dirty lines would be written back inside the timed
+    launch)."""
+    import torch
+    flush = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=x.device)
+    for _ in range(2):
+        flush.sum()
+    cold = []
+    for it in range(12):
+        flush.sum()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        mesh.matvec(x, y, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if it >= 2:
+            cold.append(e0.elapsed_time(e1))
+    del flush
+    return sorted(cold)[len(cold) // 2]
+
+
+def bench_pcg(P, dev, learned=True):
+    """PCG to a 1e-8 true relative residual: plain FP64, FP32 V-cycle, and FP32 V-cycle with
+    the learned damping (network trained on the device on 200 batches of solver-generated
+    samples, seeds 5000..5199; each level keeps it only where it beats plain Jacobi on fresh
+    samples).  The second of two identical solves is reported (the first captures the
+    hipGraph)."""
+    import torch
+    import dendro_amd as da
+    N = P.fine.n_node
+    b = torch.tensor(np.random.default_rng(P.w["rhs_seed"]).uniform(-1, 1, N), dtype=torch.float64, device=dev)
+    out = {}
+    xs = {}
+    for name, prec in (("fp64", da.F64), ("mixed", da.F32)):
+        s = da.Solver(P.meshes, P.trans, P.params(vcycle_prec=prec))
+        x = torch.zeros(N, dtype=torch.float64, device=dev)
+        s.solve(b, x)
+        st, _ = s.solve(b, x)
+        out[f"pcg_{name}"] = {"seconds": round(st.seconds, 5), "iterations": st.iterations,
+                              "rel_true": st.rel_true, "converged": bool(st.converged)}
+        xs[name] = x
+        log(f"PCG {name}: {st.iterations} its, {st.seconds*1e3:.2f} ms, rel_true {st.rel_true:.2e}")
+        del s
+    out["pcg_mixed"]["solution_rel_diff_vs_fp64"] = float(torch.linalg.norm(xs["mixed"] - xs["fp64"]) /
+                                                          torch.linalg.norm(xs["fp64"]))
+    out["pcg_speedup_mixed_vs_fp64"] = round(out["pcg_fp64"]["seconds"] / out["pcg_mixed"]["seconds"], 3)
+    if learned:
+        s = da.Solver(P.meshes, P.trans, P.params(vcycle_prec=da.F32, smoother=da.SMOOTHER_LEARNED))
+        s.learned_init(1)
+        tr = s.learned_train(steps=200, samples=8, nu=2, refresh=1, lr=3e-3, seed=5000)
+        mask = s.learned_apply(True, samples=8, seed=6000)
+        x = torch.zeros(N, dtype=torch.float64, device=dev)
+        s.solve(b, x)
+        st, _ = s.solve(b, x)
+        out["pcg_mixed_learned"] = {
+            "seconds": round(st.seconds, 5), "iterations": st.iterations, "rel_true": st.rel_true,
+            "converged": bool(st.converged), "levels_learned_mask": mask, "train_seconds": round(tr.seconds, 3),
+            "objective_first": round(tr.loss_first, 5), "objective_last": round(tr.loss_last, 5),
+            "speedup_vs_fp64": round(out["pcg_fp64"]["seconds"] / st.seconds, 3)}
+        log(f"PCG mixed+learned: {st.iterations} its, {st.seconds*1e3:.2f} ms (levels {mask:#x}); training "
+            f"{tr.seconds:.2f}s objective {tr.loss_first:.4f} -> {tr.loss_last:.4f}")
+        del s
     return out
 
 
-def bench_multi(args, meshes, trans, N, dev, stream, extra):
-    """32 right-hand sides (seeds 400..431): the batched FP32 operator and the batched
-    PCG (one FP32 V-cycle per iteration for all columns, FP64 outer CG, one hipGraph)."""
+def bench_multi(P, dev, stream, steps):
+    """32 right-hand sides (seeds 400..431): the batched FP32 operator and the batched PCG
+    (one FP32 V-cycle per iteration for all columns, FP64 outer CG, one hipGraph)."""
     import torch
     import dendro_amd as da
     k = 32
-    tuned = [m.autotune_multi(k) for m in meshes[1:]]
-    mf = meshes[-1]
-    log(f"batched tiles (autotuned, k={k}): {[t['tile_elems'] for t in tuned]}")
+    N = P.fine.n_node
+    tuned = [m.autotune_multi(k) for m in P.meshes[1:]]
+    mf = P.meshes[-1]
     X = torch.tensor(np.random.default_rng(11).uniform(-1, 1, (N, k)), dtype=torch.float32, device=dev)
     Y = torch.empty_like(X)
     for _ in range(5):
         mf.matvec_multi(X, Y, k, prec=da.F32, stream=stream)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = max(10, args.steps // 10)
+    reps = max(10, steps // 10)
     ev0.record(stream)
     for _ in range(reps):
         mf.matvec_multi(X, Y, k, prec=da.F32, stream=stream)
@@ -115,32 +255,30 @@ def bench_multi(args, meshes, trans, N, dev, stream, extra):
     B = torch.tensor(np.stack([np.random.default_rng(400 + j).uniform(-1, 1, N) for j in range(k)], 1),
                      dtype=torch.float64, device=dev)
     Xs = torch.zeros_like(B)
-    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
-                          norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=da.F32, use_graph=1)
-    s = da.Solver(meshes, trans, p)
-    s.solve_multi(B, Xs, k)                    # warm-up (buffers, graph)
+    s = da.Solver(P.meshes, P.trans, P.params(vcycle_prec=da.F32))
+    s.solve_multi(B, Xs, k)
     st = s.solve_multi(B, Xs, k)
     secs = st[0].seconds
-    single = extra.get("pcg_mixed", {}).get("seconds")
+    F = P.fine
+    alg = F.n_elem * 34 + k * N * 2 * 4 + (0 if P.uniform else F.n_elem * 4)
     out = {"k": k, "matvec_f32_us": round(ms * 1e3, 2), "matvec_f32_gdofs": round(N * k / (ms * 1e-3) / 1e9, 3),
-           "tile_elems": [t["tile_elems"] for t in tuned], "solve_seconds": round(secs, 5),
-           "seconds_per_rhs": round(secs / k, 6), "iterations": [x.iterations for x in st],
-           "max_rel_true": max(x.rel_true for x in st), "all_converged": all(x.converged for x in st)}
-    if single:
-        out["speedup_per_rhs_vs_single_mixed"] = round(single / (secs / k), 2)
-    log(f"batched k={k}: MatVec FP32 {ms*1e3:.1f} us ({out['matvec_f32_gdofs']} GDoF/s), "
-        f"PCG {secs*1e3:.1f} ms for {k} RHS, its {min(out['iterations'])}-{max(out['iterations'])}, "
-        f"max rel_true {out['max_rel_true']:.2e}")
+           "matvec_f32_hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "tile_elems": [t["tile_elems"] for t in tuned], "variants": [t["variant"] for t in tuned],
+           "solve_seconds": round(secs, 5), "seconds_per_rhs": round(secs / k, 6),
+           "iterations": [x.iterations for x in st], "max_rel_true": max(x.rel_true for x in st),
+           "all_converged": all(x.converged for x in st)}
+    log(f"batched k={k}: MatVec FP32 {ms*1e3:.1f} us ({out['matvec_f32_gdofs']} GDoF/s, "
+        f"{100*out['matvec_f32_hbm_frac']:.1f}% HBM), PCG {secs*1e3:.1f} ms for {k} RHS, its "
+        f"{min(out['iterations'])}-{max(out['iterations'])}, max rel_true {out['max_rel_true']:.2e}")
     del s
     return out
 
 
-def measured_traffic(kernel, tile_elems, block):
+def measured_traffic(workload, kernel, tile_elems, block):
     """HBM bytes per FP64 MatVec launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     passes over this workload (already scaled by the copy-kernel calibration), when they were
     taken with the same kernel and tiling; None otherwise."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "tile_op_matvec_f64_lognormal1M_d12_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"tile_op_matvec_f64_{workload}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
@@ -151,22 +289,76 @@ def measured_traffic(kernel, tile_elems, block):
     return t.get("bytes_per_launch")
 
 
+def matvec_section(P, args, dev, stream):
+    """Autotuned FP64 MatVec: K timed steps, cold-cache launches, FP32 rate, roofline."""
+    import torch
+    import dendro_amd as da
+    F = P.fine
+    mf = P.meshes[-1]
+    if args.tile:
+        mf.set_tile(args.tile, args.block)
+        tuned = {"tile_elems": args.tile, "block": args.block or 512}
+    else:
+        t0 = time.time()
+        for m in P.meshes:
+            tuned = m.autotune(da.F64)
+        log(f"autotuned tilings in {time.time() - t0:.1f}s: finest {tuned}")
+    ti = mf.tile_info()
+    log(f"tiling: {ti}")
+    x = torch.tensor(np.random.default_rng(11).uniform(-1, 1, F.n_node), dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    ms, wall = time_matvec(mf, x, y, args.steps, args.warmup, stream)
+    alg = P.alg_bytes()
+    achieved = alg / (ms * 1e-3) / 1e9
+    log(f"MatVec FP64: {ms*1e3:.1f} us/launch, {F.n_node / (ms * 1e-3) / 1e9:.2f} GDoF/s, {achieved:.0f} GB/s "
+        f"algorithmic ({100*achieved/HBM_PEAK_GBS:.1f}% of HBM peak); wall {wall:.3f}s")
+    cold = cold_matvec_ms(mf, x, y, stream)
+    log(f"MatVec FP64 after an Infinity Cache flush: {cold*1e3:.1f} us "
+        f"({100 * alg / (cold * 1e-3) / 1e9 / HBM_PEAK_GBS:.1f}% of HBM peak)")
+    if args.calib_copy:
+        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        c = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        del a, c
+    x32, y32 = x.float(), torch.empty(F.n_node, dtype=torch.float32, device=dev)
+    ms32, _ = time_matvec(mf, x32, y32, args.steps, args.warmup, stream, prec=da.F32)
+    del x32, y32
+    kernel = P.kernel_name(tuned["block"], ti["lane_stencils"])
+    res = {
+        "ms": ms, "gdofs": F.n_node / (ms * 1e-3) / 1e9, "achieved": achieved, "alg": alg,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(P.w["name"], kernel, ti["tile_elems"], tuned["block"]),
+                     "algorithmic_bytes_per_launch": alg, "kernel": kernel,
+                     "tile_elems": ti["tile_elems"], "block": tuned["block"], "n_tiles": ti["n_tiles"],
+                     "n_records": ti["n_records"], "cold_cache_us": round(cold * 1e3, 2),
+                     "cold_cache_frac": round(alg / (cold * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "matvec_fp32_us": round(ms32 * 1e3, 2),
+        "matvec_fp32_gdofs": round(F.n_node / (ms32 * 1e-3) / 1e9, 4),
+        "x": x, "y": y,
+    }
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--npts", type=int, default=1000000)
-    ap.add_argument("--depth", type=int, default=12)
-    ap.add_argument("--nlevels", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-damping PCG")
+    ap.add_argument("--no-c2", action="store_true", help="skip the secondary 1M log-normal workload")
+    ap.add_argument("--no-multi", action="store_true", help="skip the 32 right-hand-side batch (secondary)")
+    ap.add_argument("--no-dirichlet", action="store_true", help="skip the Dirichlet-masked MatVec (secondary)")
+    ap.add_argument("--matvec-only", action="store_true",
+                    help="only the headline MatVec (for profiler runs): implies --no-cpu --no-pcg --no-c2")
     ap.add_argument("--tile", type=int, default=0, help="operator tile size (0: autotune)")
     ap.add_argument("--block", type=int, default=0, help="workgroup size with --tile (0: library default)")
-    ap.add_argument("--sweep-tiles", action="store_true", help="time the MatVec for several tile sizes")
-    ap.add_argument("--no-multi", action="store_true", help="skip the 32 right-hand-side batch")
-    ap.add_argument("--no-dirichlet", action="store_true", help="skip the Dirichlet-masked MatVec")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-damping PCG")
     ap.add_argument("--pcg-trace", action="store_true",
                     help="only three eager (no hipGraph) mixed-precision PCG solves, for a kernel trace")
     ap.add_argument("--calib-copy", action="store_true",
@@ -176,201 +368,90 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solve path is "
               "not sharded across GPUs in this build.", file=sys.stderr)
         sys.exit(2)
+    if args.matvec_only:
+        args.no_cpu = args.no_pcg = args.no_c2 = True
 
     import torch
     import dendro_amd as da
 
     dev = torch.device("cuda:0")
     da.init(0)
-    damg, levels, coefs, trans_arr = build_problem(args.npts, args.depth, args.nlevels)
-    F = levels[-1]
-    E, N = F.n_elem, F.n_node
-    meshes = []
-    for l, L in enumerate(levels):
-        m = da.Mesh(L)
-        m.set_coefficients(coefs[l][0], None)
-        meshes.append(m)
-    trans = [da.Transfer(meshes[l - 1], meshes[l], *trans_arr[l - 1]) for l in range(1, len(levels))]
-    mf = meshes[-1]
-    tuned = {"tile_elems": args.tile, "block": args.block or 512}
-    if args.tile:
-        mf.set_tile(args.tile, args.block)
-    else:
-        t0 = time.time()
-        for m in meshes:
-            tuned = m.autotune(da.F64)
-        log(f"autotuned tilings in {time.time() - t0:.1f}s: finest {tuned} {mf.tile_info()}")
-    log(f"tiling: {mf.tile_info()}")
+    stream = torch.cuda.current_stream()
+    P = Problem(WORKLOADS[args.workload])
+    F = P.fine
+
     if args.pcg_trace:
-        log("levels: " + ", ".join(f"{l}: E={L.n_elem} {meshes[l].tile_info()}" for l, L in enumerate(levels)))
-        b = torch.tensor(np.random.default_rng(22).uniform(-1, 1, N), dtype=torch.float64, device=dev)
-        p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
-                              norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=da.F32,
-                              use_graph=0)
-        s = da.Solver(meshes, trans, p)
-        xs = torch.zeros(N, dtype=torch.float64, device=dev)
+        for m in P.meshes:
+            m.autotune(da.F64)
+        log("levels: " + ", ".join(f"{l}: E={L.n_elem} {P.meshes[l].tile_info()}" for l, L in enumerate(P.levels)))
+        b = torch.tensor(np.random.default_rng(P.w["rhs_seed"]).uniform(-1, 1, F.n_node), dtype=torch.float64,
+                         device=dev)
+        s = da.Solver(P.meshes, P.trans, P.params(vcycle_prec=da.F32, use_graph=0))
+        xs = torch.zeros(F.n_node, dtype=torch.float64, device=dev)
         for _ in range(3):
             st, _ = s.solve(b, xs)
             log(f"eager mixed PCG: {st.iterations} its, {st.seconds * 1e3:.2f} ms")
         return
 
-    # ---- MatVec: K steps, inputs resident in HBM, events on the launch stream
-    stream = torch.cuda.current_stream()
-    x = torch.tensor(np.random.default_rng(11).uniform(-1, 1, N), dtype=torch.float64, device=dev)
-    y = torch.empty_like(x)
-    for _ in range(args.warmup):
-        mf.matvec(x, y, stream=stream)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        mf.matvec(x, y, stream=stream)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = ev0.elapsed_time(ev1)
-    ms_per = ms_total / args.steps
-    gdofs = N / (ms_per * 1e-3) / 1e9
-    alg_bytes = E * (32 + 2) + N * 2 * 8 + E * 8     # nlist + stencil/level, x read + y written, kappa
-    achieved = alg_bytes / (ms_per * 1e-3) / 1e9
-    log(f"MatVec FP64: {ms_per*1e3:.1f} us/launch, {gdofs:.2f} GDoF/s, {achieved:.0f} GB/s algorithmic "
-        f"({100*achieved/HBM_PEAK_GBS:.1f}% of HBM peak); wall {wall:.3f}s")
-    # The operator's data (about 110 MB) fits in the 256 MB Infinity Cache, so back-to-back
-    # launches read much of it from there.  Cold: reading a 1 GiB buffer between launches
-    # evicts it and each launch is timed alone.  The flush reads rather than writes: a fill
-    # would leave 1 GiB of dirty lines whose write-back lands inside the timed launch (a plain
-    # 110 MiB read then takes 40 us instead of 21).
-    flush = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=dev)
-    for _ in range(2):
-        flush.sum()
-    cold = []
-    for it in range(12):
-        flush.sum()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        mf.matvec(x, y, stream=stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        if it >= 2:
-            cold.append(e0.elapsed_time(e1))
-    del flush
-    cold_ms = sorted(cold)[len(cold) // 2]
-    log(f"MatVec FP64 after an Infinity Cache flush: {cold_ms*1e3:.1f} us ({alg_bytes / (cold_ms * 1e-3) / 1e9:.0f} GB/s "
-        f"algorithmic, {100 * alg_bytes / (cold_ms * 1e-3) / 1e9 / HBM_PEAK_GBS:.1f}% of HBM peak)")
-
-    if args.sweep_tiles:
-        for te, blk in [(t_, b_) for t_ in (512, 1024, 2048, 4096) for b_ in (256, 512, 1024)]:
-            try:
-                mf.set_tile(te, blk)
-            except da.DendroError as e:
-                log(f"tile {te}: {e}")
-                continue
-            for _ in range(5):
-                mf.matvec(x, y, stream=stream)
-            ev0.record(stream)
-            for _ in range(50):
-                mf.matvec(x, y, stream=stream)
-            ev1.record(stream)
-            torch.cuda.synchronize()
-            ti = mf.tile_info()
-            log(f"tile {te} block {blk}: {ev0.elapsed_time(ev1) / 50 * 1e3:.1f} us  {ti}")
-        mf.autotune(da.F64)
-
-    if args.calib_copy:
-        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev)
-        c = torch.empty_like(a)
-        a.fill_(1.0)
-        for _ in range(3):
-            c.copy_(a)
-        torch.cuda.synchronize()
-
-    # FP32 MatVec rate (V-cycle precision of the mixed solver)
-    x32, y32 = x.float(), torch.empty(N, dtype=torch.float32, device=dev)
-    for _ in range(args.warmup):
-        mf.matvec(x32, y32, prec=da.F32, stream=stream)
-    ev0.record(stream)
-    for _ in range(args.steps):
-        mf.matvec(x32, y32, prec=da.F32, stream=stream)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms32 = ev0.elapsed_time(ev1) / args.steps
-
-    # Dirichlet-masked FP64 MatVec on the same mesh: boundary columns dropped, identity rows
-    # (a mask pass over x, the operator, and the boundary rows)
-    dirichlet = {}
-    if not args.no_dirichlet:
-        bdy = damg.boundary(damg.nlevels - 1)
-        mf.set_boundary(bdy)
-        for _ in range(args.warmup):
-            mf.matvec_dirichlet(x, y, stream=stream)
-        ev0.record(stream)
-        for _ in range(args.steps):
-            mf.matvec_dirichlet(x, y, stream=stream)
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        msd = ev0.elapsed_time(ev1) / args.steps
-        log(f"Dirichlet MatVec FP64: {msd*1e3:.1f} us/call ({int(bdy.sum())} boundary nodes), "
-            f"{N / (msd * 1e-3) / 1e9:.2f} GDoF/s")
-        dirichlet = {"matvec_dirichlet_f64_us": round(msd * 1e3, 2), "n_boundary_nodes": int(bdy.sum())}
-
-    extra = {}
-    if not args.no_pcg:
-        b = torch.tensor(np.random.default_rng(22).uniform(-1, 1, N), dtype=torch.float64, device=dev)
-        for name, prec in (("fp64", da.F64), ("mixed", da.F32)):
-            p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
-                                  norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=prec,
-                                  use_graph=1)
-            s = da.Solver(meshes, trans, p)
-            xs = torch.zeros(N, dtype=torch.float64, device=dev)
-            s.solve(b, xs)                                   # warm-up solve
-            st, _ = s.solve(b, xs)
-            extra[f"pcg_{name}"] = {"seconds": round(st.seconds, 5), "iterations": st.iterations,
-                                    "rel_true": st.rel_true, "converged": bool(st.converged)}
-            log(f"PCG {name}: {st.iterations} its, {st.seconds*1e3:.1f} ms, rel_true {st.rel_true:.2e}")
-            del s
-        extra["pcg_speedup_mixed_vs_fp64"] = round(extra["pcg_fp64"]["seconds"] / extra["pcg_mixed"]["seconds"], 3)
-
-    if not args.no_pcg and not args.no_learned:
-        extra["pcg_mixed_learned"] = bench_learned(meshes, trans, N, dev, extra)
-
-    if not args.no_multi:
-        extra["multi_rhs"] = bench_multi(args, meshes, trans, N, dev, stream, extra)
-
-    lane = "true" if mf.tile_info()["lane_stencils"] else "false"
-    kernel = f"dgh::tile_op_kernel<double, double, false, 0, {tuned['block']}, false, {lane}>"
-    traffic = measured_traffic(kernel, mf.tile_info()["tile_elems"], tuned["block"])
+    mv = matvec_section(P, args, dev, stream)
+    del mv["x"], mv["y"]
     out = {
         "metric": METRIC,
-        "value": round(gdofs, 4),
+        "value": round(mv["gdofs"], 4),
         "unit": "GDoF/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per, 5),
+        "ms_per_step": round(mv["ms"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "Jacobian2 MatVec + 5-level GMG-PCG, octree from 1M log-normal points, "
-                               "max depth 12, kappa=10^U(-1,1), FP64",
-                   "n_elem": E, "n_node": N, "n_levels": len(levels), "n_hanging_elem": F.n_hanging},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "kernel": kernel,
-                     "tile_elems": mf.tile_info()["tile_elems"], "block": tuned["block"],
-                     "n_tiles": mf.tile_info()["n_tiles"], "n_records": mf.tile_info()["n_records"],
-                     "cold_cache_us": round(cold_ms * 1e3, 2),
-                     "cold_cache_frac": round(alg_bytes / (cold_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "matvec_fp32_ms": round(ms32, 5),
-        "matvec_fp32_gdofs": round(N / (ms32 * 1e-3) / 1e9, 4),
-        **dirichlet,
+        "config": {"workload": P.w["desc"], "name": P.w["name"], "n_elem": F.n_elem, "n_node": F.n_node,
+                   "n_levels": len(P.levels), "n_hanging_elem": F.n_hanging,
+                   "coarsest_nodes": P.levels[0].n_node, "host_build_seconds": round(P.host_seconds, 1)},
+        "roofline": mv["roofline"],
+        "matvec_fp32_us": mv["matvec_fp32_us"],
+        "matvec_fp32_gdofs": mv["matvec_fp32_gdofs"],
     }
-    out.update(extra)
+    if not args.no_pcg:
+        out.update(bench_pcg(P, dev, learned=not args.no_learned))
+
+    if not args.no_c2 and args.workload != "c2":
+        Q = Problem(WORKLOADS["c2"])
+        q = matvec_section(Q, argparse.Namespace(**{**vars(args), "tile": 0, "calib_copy": False}), dev, stream)
+        c2 = {"workload": Q.w["desc"], "n_elem": Q.fine.n_elem, "n_node": Q.fine.n_node,
+              "matvec_f64_us": round(q["ms"] * 1e3, 2), "matvec_f64_gdofs": round(q["gdofs"], 4),
+              "hbm_frac": q["roofline"]["frac"], "cold_cache_us": q["roofline"]["cold_cache_us"],
+              "cold_cache_frac": q["roofline"]["cold_cache_frac"], "matvec_fp32_us": q["matvec_fp32_us"]}
+        if not args.no_dirichlet:
+            bdy = Q.damg.boundary(Q.damg.nlevels - 1)
+            mq = Q.meshes[-1]
+            mq.set_boundary(bdy)
+            for _ in range(args.warmup):
+                mq.matvec_dirichlet(q["x"], q["y"], stream=stream)
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            for _ in range(args.steps):
+                mq.matvec_dirichlet(q["x"], q["y"], stream=stream)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            msd = ev0.elapsed_time(ev1) / args.steps
+            mq.set_boundary(None)
+            log(f"c2 Dirichlet MatVec FP64: {msd*1e3:.1f} us ({int(bdy.sum())} boundary nodes)")
+            c2["matvec_dirichlet_f64_us"] = round(msd * 1e3, 2)
+            c2["n_boundary_nodes"] = int(bdy.sum())
+        del q["x"], q["y"]
+        if not args.no_pcg:
+            c2.update(bench_pcg(Q, dev, learned=False))
+        if not args.no_multi:
+            c2["multi_rhs"] = bench_multi(Q, dev, stream, args.steps)
+        out["c2"] = c2
+        del Q
+
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(F, coefs[-1][0])
+        out["cpu_baseline"] = cpu_baseline(P)
     print(json.dumps(out), flush=True)
 
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "dgh_diagonal_dirichlet", "dgh_mesh_info", "dgh_mesh_destroy",
     "dgh_mesh_set_tile", "dgh_mesh_tile_info", "dgh_mesh_autotune", "dgh_mesh_set_lane_stencils",
     "dgh_mesh_lane_stencils",
-    "dgh_matvec", "dgh_diagonal", "dgh_matvec_multi", "dgh_mesh_set_tile_multi",
+    "dgh_matvec", "dgh_matvec_dot", "dgh_diagonal", "dgh_matvec_multi", "dgh_mesh_set_tile_multi",
     "dgh_mesh_autotune_multi",
     "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add",
     "dgh_solver_default_params", "dgh_solver_create", "dgh_solver_destroy", "dgh_solver_solve",
@@ -331,6 +331,10 @@ class Mesh:
 
     def matvec(self, x, y, prec=F64, stream=None):
         check(lib().dgh_matvec(self.h, int(prec), ptr(x), ptr(y), _stream(stream)))
+
+    def matvec_dot(self, x, y, xy, stream=None):
+        """y = A x and xy[0] = x.y (FP64; xy a one-element device tensor)."""
+        check(lib().dgh_matvec_dot(self.h, ptr(x), ptr(y), ptr(xy), _stream(stream)))
 
     def matvec_multi(self, x, y, k, prec=F64, stream=None):
         """Y = A X for k columns stored node-major ([N][k])."""

@@ -775,6 +775,20 @@ int dgh_matvec(dgh_mesh m, int prec, const void* x, void* y, void* stream) {
   });
 }
 
+int dgh_matvec_dot(dgh_mesh m, const double* x, double* y, double* xy, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!m || !x || !y || !xy) fail(DGH_ERR_ARG, "NULL argument");
+    if (x == y) fail(DGH_ERR_ARG, "x and y must not alias");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t nt = m->op.plan.tiles.size();
+    if (m->dotp.n < std::max<size_t>(nt, 1)) m->dotp.alloc(std::max<size_t>(nt, 1));
+    if (nt) launch_cg(m, x, y, m->dotp.p, st);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(RED2_THREADS), 0, st, int(nt), (const double*)m->dotp.p, xy);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
 int dgh_diagonal(dgh_mesh m, int prec, void* d, void* stream) {
   return guarded([&] {
     require_init();

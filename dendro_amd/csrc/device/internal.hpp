@@ -133,6 +133,7 @@ struct dgh_mesh_s {
   int mvariant = 0;             // batched FP32 products: 0 vector ALUs, 1 matrix cores
   uint64_t gen = 0;             // bumped whenever device tilings or coefficients are rebuilt
   DevBuf<uint8_t> bdy;          // [N] boundary flags of the Dirichlet operators (empty: none)
+  DevBuf<double> dotp;          // per-tile partials of dgh_matvec_dot
 };
 
 struct dgh_transfer_s {

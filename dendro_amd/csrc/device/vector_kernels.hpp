@@ -94,6 +94,23 @@ reduce2_kernel(int nq, const double* __restrict__ part, double* __restrict__ out
   }
 }
 
+// out[0] = part[0] + ... + part[n-1] in the order cg_xr_fused_kernel sums the operator's tile
+// partials (one workgroup of RED2_THREADS): the same value CG forms for p.w.
+static __global__ void __launch_bounds__(RED2_THREADS)
+sum_partials_kernel(int n, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[RED2_THREADS / 64];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += RED2_THREADS) v += part[k];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < RED2_THREADS / 64; ++k) t += red[k];
+    out[0] = t;
+  }
+}
+
 // v = v - sum[0] / n  (constant-null-space projection)
 template <class T>
 __global__ void subtract_mean_kernel(uint32_t n, const double* __restrict__ sum, T* __restrict__ v) {

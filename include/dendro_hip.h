@@ -207,6 +207,13 @@ int dgh_mesh_destroy(dgh_mesh m);
 /* y = A x.  Replaces Jacobian1MatMult (odaJac.C:290-374) and Jacobian2MatMult
  * (omgJac.C:1687-1775).  prec selects FP64 or FP32 vectors and arithmetic. */
 int dgh_matvec(dgh_mesh m, int prec, const void* x_dev, void* y_dev, void* stream);
+/* y = A x and xy = x.y (FP64) in one operator launch plus one fixed-order sum of its per-tile
+ * partials: the MatMult + VecDot pair of every CG iteration (PETSc KSPSolve_CG behind
+ * DAMGSolve, src/omg/omg.C:1166, with -ksp_type cg from bin/options:50-53), for callers that
+ * run their own Krylov loop.  xy_dev is one device double.  The partials sit in a buffer the
+ * library allocates for the mesh on the first call (and again after a re-tiling that adds
+ * tiles). */
+int dgh_matvec_dot(dgh_mesh m, const double* x_dev, double* y_dev, double* xy_dev, void* stream);
 /* d = diag(A).  Replaces Jacobian1MatGetDiagonal (odaJac.C:160-223) and
  * Jacobian2MatGetDiagonal (omgJac.C:1594-1662). */
 int dgh_diagonal(dgh_mesh m, int prec, void* d_dev, void* stream);

@@ -28,7 +28,9 @@ def lib():
         _lib.or_mg_create.restype = C.c_void_p
         _lib.or_mg_create.argtypes = [C.c_int]
         _lib.or_mg_free.argtypes = [C.c_void_p]
-        for f in ["or_mg_set_level", "or_mg_setup", "or_mg_vcycle", "or_pcg", "or_mg_apply"]:
+        for f in ["or_mg_set_level", "or_mg_setup", "or_mg_setup_ex", "or_mg_vcycle", "or_pcg",
+                  "or_mg_apply", "or_mg_coarse_matrix", "or_mg_coarse_set_lu", "or_mg_coarse_size",
+                  "or_mg_set_threads", "or_matvec_mt"]:
             getattr(_lib, f).argtypes = None
         _lib.or_learn_loss.restype = C.c_double
     return _lib
@@ -80,6 +82,22 @@ def matvec(level, x, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0):
     return y
 
 
+def matvec_mt(level, x, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0, threads=None):
+    """or_matvec on `threads` host threads (default: all CPUs of this process)."""
+    st = _st()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(level.n_node)
+    k, m = _coef(level, kappa, mass)
+    nt = threads or len(os.sched_getaffinity(0))
+    rc = lib().or_matvec_mt(C.c_int(nt), C.c_uint32(level.n_elem), C.c_uint32(level.n_node),
+                            C.c_uint32(level.max_depth), _p(level.nlist), _p(level.child), _p(level.hnmask),
+                            _p(level.lev), _p(k), _p(m), C.c_double(lap_fac), C.c_double(mass_fac),
+                            _p(st["lap2"]), _p(st["mass2"]), _p(x), _p(y))
+    if rc:
+        raise RuntimeError(f"or_matvec_mt failed ({rc})")
+    return y
+
+
 def diagonal(level, kappa=None, mass=None, lap_fac=1.0, mass_fac=1.0):
     st = _st()
     d = np.zeros(level.n_node)
@@ -127,6 +145,38 @@ def _orlevel(L):
                     L.hnmask.ctypes.data, L.lev.ctypes.data, L.anchor.ctypes.data)
 
 
+_ST1 = None
+
+
+def stencils_type1():
+    """Type1 element stencils lap1/mass1 [8][18][8][8][8] and the shape functions at the
+    3-point Gauss nodes shfn [8][18][8][3][3][3]."""
+    global _ST1
+    if _ST1 is None:
+        L, M, S = np.zeros(73728), np.zeros(73728), np.zeros(31104)
+        lib().or_stencils_type1(_p(L), _p(M))
+        lib().or_shfn(_p(S))
+        _ST1 = {"lap1": L, "mass1": M, "shfn": S}
+    return {"lap1": _ST1["lap1"].reshape(8, 18, 8, 8, 8), "mass1": _ST1["mass1"].reshape(8, 18, 8, 8, 8),
+            "shfn": _ST1["shfn"].reshape(8, 18, 8, 3, 3, 3)}
+
+
+def matvec_jac3(coarse, fine, x, kappa_fine, mass_fine=None, tol=1e-12):
+    """Jacobian3 product on the coarse level with the fine level's element coefficients."""
+    st = _st()
+    s1 = stencils_type1()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(coarse.n_node)
+    k, m = _coef(fine, kappa_fine, mass_fine)
+    oc, of = _orlevel(coarse), _orlevel(fine)
+    rc = lib().or_matvec_jac3(C.byref(oc), C.byref(of), _p(k), _p(m), C.c_double(tol), _p(st["lap2"]),
+                              _p(st["mass2"]), _p(np.ascontiguousarray(s1["lap1"].reshape(-1))),
+                              _p(np.ascontiguousarray(s1["mass1"].reshape(-1))), _p(x), _p(y))
+    if rc:
+        raise RuntimeError(f"or_matvec_jac3 failed ({rc})")
+    return y
+
+
 def restrict(coarse, fine, f):
     st = _st()
     f = np.ascontiguousarray(f, np.float64)
@@ -153,7 +203,11 @@ class MG:
     """Oracle multigrid hierarchy; levels: list (coarsest first) of (Level, kappa, mass,
     lap_fac, mass_fac)."""
 
-    def __init__(self, levels, omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=False):
+    # coarsest levels above this size are factored by LAPACK (scipy) instead of the oracle's
+    # unblocked loop; both are partial-pivoting LU of the same assembled matrix
+    LAPACK_ABOVE = 2500
+
+    def __init__(self, levels, omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=False, threads=1):
         self._keep = []
         self.h = lib().or_mg_create(len(levels))
         for i, (L, kappa, mass, lf, mf) in enumerate(levels):
@@ -164,10 +218,22 @@ class MG:
                                        _p(L.child), _p(L.hnmask), _p(L.lev), _p(L.anchor), _p(k), _p(m),
                                        C.c_double(lf), C.c_double(mf))
             assert rc == 0
-        rc = lib().or_mg_setup(C.c_void_p(self.h), C.c_double(omega), C.c_int(nu_pre), C.c_int(nu_post),
-                               C.c_int(int(null_space)))
+        if threads > 1:
+            lib().or_mg_set_threads(C.c_void_p(self.h), C.c_int(threads))
+        big = levels[0][0].n_node > self.LAPACK_ABOVE
+        rc = lib().or_mg_setup_ex(C.c_void_p(self.h), C.c_double(omega), C.c_int(nu_pre), C.c_int(nu_post),
+                                  C.c_int(int(null_space)), C.c_int(0 if big else 1))
         if rc:
             raise RuntimeError(f"or_mg_setup failed ({rc})")
+        if big:
+            import scipy.linalg
+            n = lib().or_mg_coarse_size(C.c_void_p(self.h))
+            A = np.zeros((n, n))
+            assert lib().or_mg_coarse_matrix(C.c_void_p(self.h), _p(A)) == 0
+            lu, piv = scipy.linalg.lu_factor(A, overwrite_a=True, check_finite=False)
+            lu = np.ascontiguousarray(lu)
+            piv = np.ascontiguousarray(piv, np.int32)
+            assert lib().or_mg_coarse_set_lu(C.c_void_p(self.h), _p(lu), _p(piv)) == 0
         self.n = levels[-1][0].n_node
 
     def vcycle(self, r):

@@ -9,6 +9,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -147,6 +148,54 @@ void or_stencils(double* lap2, double* mass2, double* rmat2, double* rmat1) {
     }
 }
 
+/* Matlab/computeLapStencilsType1.m, computeMassStencilsType1.m: the coarse element's shape
+ * functions integrated over the octant of fine child cf only (2-point Gauss on that octant,
+ * volume factor 1/8), lap1/mass1[c][t][cf][i][j] */
+void or_stencils_type1(double* lap1, double* mass1) {
+  const double g[2] = {-1.0 / sqrt(3.0), 1.0 / sqrt(3.0)};
+  for (int c = 0; c < 8; ++c)
+    for (int t = 0; t < 18; ++t) {
+      double phi[8][8];
+      coarse_coeffs(c, kMask[c][t], phi);
+      for (int cf = 0; cf < 8; ++cf) {
+        /* octant cf of [-1,1]^3 spans [b-1, b] along an axis with child bit b (b = 0: [-1,0]) */
+        double mid[3];
+        for (int d = 0; d < 3; ++d) mid[d] = ((cf >> d) & 1) ? 0.5 : -0.5;
+        size_t base = (((size_t)c * 18 + t) * 8 + cf) * 64;
+        for (int i = 0; i < 8; ++i)
+          for (int j = 0; j < 8; ++j) {
+            double L = 0.0, M = 0.0;
+            for (int m = 0; m < 2; ++m)
+              for (int n = 0; n < 2; ++n)
+                for (int p = 0; p < 2; ++p) {
+                  const double X = mid[0] + 0.5 * g[m], Y = mid[1] + 0.5 * g[n], Z = mid[2] + 0.5 * g[p];
+                  for (int d = 0; d < 3; ++d) L += eval_grad(phi[i], d, X, Y, Z) * eval_grad(phi[j], d, X, Y, Z);
+                  M += eval_phi(phi[i], X, Y, Z) * eval_phi(phi[j], X, Y, Z);
+                }
+            if (lap1) lap1[base + i * 8 + j] = L / 8.0;
+            if (mass1) mass1[base + i * 8 + j] = M / 8.0;
+          }
+      }
+    }
+}
+
+/* Matlab/computeShFnMat.m: shfn[c][t][j][m][n][p] = phi_j at the 3-point Gauss nodes
+ * (g[m], g[n], g[p]), g = {0, +sqrt(3/5), -sqrt(3/5)} in the order ComputeRHS2 uses them
+ * (examples/src/backend/omgRhs.C:1141-1180) */
+void or_shfn(double* shfn) {
+  const double g[3] = {0.0, sqrt(3.0 / 5.0), -sqrt(3.0 / 5.0)};
+  for (int c = 0; c < 8; ++c)
+    for (int t = 0; t < 18; ++t) {
+      double phi[8][8];
+      coarse_coeffs(c, kMask[c][t], phi);
+      for (int j = 0; j < 8; ++j)
+        for (int m = 0; m < 3; ++m)
+          for (int n = 0; n < 3; ++n)
+            for (int p = 0; p < 3; ++p)
+              shfn[((((size_t)c * 18 + t) * 8 + j) * 3 + m) * 9 + n * 3 + p] = eval_phi(phi[j], g[m], g[n], g[p]);
+    }
+}
+
 /* ------------------------------------------------------------------ operator */
 /* JACOBIAN_TYPE1_MULT_BLOCK (examples/src/backend/odaJac.C:271-288) and
  * JAC_TYPE2_ELEM_MULT_BLOCK (examples/src/backend/omgJac.C:1665-1685):
@@ -174,6 +223,134 @@ void or_matvec(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist,
       for (int j = 0; j < 8; ++j)
         y[idx[k]] += ((fac1 * L[k * 8 + j]) + (fac2 * M[k * 8 + j])) * x[idx[j]];
   }
+}
+
+/* or_matvec on nthreads threads (the all-core CPU baseline; not a reference restatement of
+ * its own).  Thread t runs the element loop over a contiguous range of elements.  Node ids
+ * follow the Morton order of the node anchors and an element's vertices lie at or after its
+ * anchor, so the rows of a range fall mostly in [first node of the range, first node of the
+ * next range): those are added in place, the others are kept in a list that the calling
+ * thread adds afterwards in thread order.  The sums of a row can come in another order than
+ * in or_matvec (rounding-level differences). */
+typedef struct {
+  uint32_t e0, e1, n0, n1;
+  uint32_t maxD;
+  const uint32_t* nlist;
+  const uint8_t *child, *hnmask, *lev;
+  const double *kappa, *mass, *lap2, *mass2, *x;
+  double lap_fac, mass_fac;
+  double* y;
+  uint32_t* dx;     /* deferred rows */
+  double* dv;
+  size_t nd, cap;
+  int fail;
+} mt_part;
+
+static void* mt_worker(void* arg) {
+  mt_part* P = (mt_part*)arg;
+  const double hFac = 1.0 / ((double)(1u << (P->maxD - 1)));
+  for (uint32_t e = P->e0; e < P->e1; ++e) {
+    const double h = hFac * (double)(1u << (P->maxD - P->lev[e]));
+    const double k1 = P->kappa ? P->kappa[e] : P->lap_fac;
+    const double k2 = P->kappa ? (P->mass ? P->mass[e] : 0.0) : P->mass_fac;
+    const double fac1 = k1 * h / 2.0;
+    const double fac2 = k2 * h * h * h / 8.0;
+    const int c = P->child[e];
+    int t = 0;
+    if (P->hnmask[e]) t = or_etype(c, P->hnmask[e]);
+    const double* L = P->lap2 + ((size_t)c * 18 + t) * 64;
+    const double* M = P->mass2 + ((size_t)c * 18 + t) * 64;
+    const uint32_t* idx = P->nlist + (size_t)e * 8;
+    for (int k = 0; k < 8; ++k) {
+      double s = 0.0;
+      for (int j = 0; j < 8; ++j) s += ((fac1 * L[k * 8 + j]) + (fac2 * M[k * 8 + j])) * P->x[idx[j]];
+      const uint32_t n = idx[k];
+      if (n >= P->n0 && n < P->n1) {
+        P->y[n] += s;
+      } else {
+        if (P->nd == P->cap) {
+          size_t nc = P->cap ? 2 * P->cap : 4096;
+          uint32_t* dx = (uint32_t*)realloc(P->dx, nc * sizeof(uint32_t));
+          double* dv = (double*)realloc(P->dv, nc * sizeof(double));
+          if (!dx || !dv) {
+            P->dx = dx ? dx : P->dx;
+            P->dv = dv ? dv : P->dv;
+            P->fail = 1;
+            return NULL;
+          }
+          P->dx = dx;
+          P->dv = dv;
+          P->cap = nc;
+        }
+        P->dx[P->nd] = n;
+        P->dv[P->nd++] = s;
+      }
+    }
+  }
+  return NULL;
+}
+
+int or_matvec_mt(int nthreads, uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist,
+                 const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+                 const double* kappa, const double* mass, double lap_fac, double mass_fac,
+                 const double* lap2, const double* mass2, const double* x, double* y) {
+  if (nthreads < 1 || nthreads > 256) return -1;
+  if ((uint32_t)nthreads > E) nthreads = E ? (int)E : 1;
+  memset(y, 0, sizeof(double) * N);
+  mt_part* P = (mt_part*)calloc((size_t)nthreads, sizeof(mt_part));
+  pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+  if (!P || !th) {
+    free(P);
+    free(th);
+    return -1;
+  }
+  for (int t = 0; t < nthreads; ++t) {
+    mt_part* q = &P[t];
+    q->e0 = (uint32_t)((uint64_t)E * t / nthreads);
+    q->e1 = (uint32_t)((uint64_t)E * (t + 1) / nthreads);
+    q->maxD = maxD;
+    q->nlist = nlist;
+    q->child = child;
+    q->hnmask = hnmask;
+    q->lev = lev;
+    q->kappa = kappa;
+    q->mass = mass;
+    q->lap2 = lap2;
+    q->mass2 = mass2;
+    q->x = x;
+    q->lap_fac = lap_fac;
+    q->mass_fac = mass_fac;
+    q->y = y;
+    /* first row of the range: the smallest vertex id of its first element */
+    uint32_t m = N;
+    if (q->e0 < q->e1)
+      for (int k = 0; k < 8; ++k) m = nlist[(size_t)q->e0 * 8 + k] < m ? nlist[(size_t)q->e0 * 8 + k] : m;
+    q->n0 = (t == 0) ? 0 : m;
+  }
+  /* row ranges disjoint and increasing: [n0_t, n0_{t+1}) with n0 made non-decreasing */
+  for (int t = 1; t < nthreads; ++t)
+    if (P[t].n0 < P[t - 1].n0) P[t].n0 = P[t - 1].n0;
+  for (int t = 0; t < nthreads; ++t) P[t].n1 = (t + 1 < nthreads) ? P[t + 1].n0 : N;
+  int rc = 0, started = 0;
+  for (int t = 0; t < nthreads; ++t) {
+    if (pthread_create(&th[t], NULL, mt_worker, &P[t])) {
+      rc = -1;
+      break;
+    }
+    ++started;
+  }
+  for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
+  for (int t = 0; t < nthreads && rc == 0; ++t) {
+    if (P[t].fail) rc = -1;
+    for (size_t q = 0; q < P[t].nd; ++q) y[P[t].dx[q]] += P[t].dv[q];
+  }
+  for (int t = 0; t < nthreads; ++t) {
+    free(P[t].dx);
+    free(P[t].dv);
+  }
+  free(P);
+  free(th);
+  return rc;
 }
 
 /* DirichletLaplacianMatMult / DirichletJacobianMatMult (examples/src/backend/omgJac.C:859-1056):
@@ -279,9 +456,46 @@ typedef struct {
 } taken_t;
 #define kTake 64
 
-static int transfer(const or_level* C, const or_level* F, const double* rmat2,
-                    const double* rmat1, const double* in, double* out, int restrict_dir) {
+/* The (fine node, coarse node, weight) pairs the loop keeps, in loop order; pairs of weight
+ * zero are dropped (they add 0 to a sum: the same value for finite inputs).  The pairs do
+ * not depend on the vector, so the multigrid hierarchy builds them once per level. */
+typedef struct {
+  size_t n, cap;
+  uint32_t *fn, *cn;
+  double* w;
+} pairs_t;
+
+static void pairs_free(pairs_t* P) {
+  free(P->fn);
+  free(P->cn);
+  free(P->w);
+  memset(P, 0, sizeof(*P));
+}
+
+static int pairs_push(pairs_t* P, uint32_t fn, uint32_t cn, double w) {
+  if (P->n == P->cap) {
+    size_t nc = P->cap ? 2 * P->cap : 1 << 16;
+    uint32_t* a = (uint32_t*)realloc(P->fn, nc * sizeof(uint32_t));
+    if (!a) return -1;
+    P->fn = a;
+    uint32_t* b = (uint32_t*)realloc(P->cn, nc * sizeof(uint32_t));
+    if (!b) return -1;
+    P->cn = b;
+    double* c = (double*)realloc(P->w, nc * sizeof(double));
+    if (!c) return -1;
+    P->w = c;
+    P->cap = nc;
+  }
+  P->fn[P->n] = fn;
+  P->cn[P->n] = cn;
+  P->w[P->n++] = w;
+  return 0;
+}
+
+static int transfer_pairs(const or_level* C, const or_level* F, const double* rmat2,
+                          const double* rmat1, pairs_t* out) {
   taken_t tk;
+  memset(out, 0, sizeof(*out));
   tk.cap = (uint32_t*)calloc(F->N, sizeof(uint32_t));
   tk.ids = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)F->N * kTake);
   if (!tk.cap || !tk.ids) {
@@ -289,7 +503,6 @@ static int transfer(const or_level* C, const or_level* F, const double* rmat2,
     free(tk.ids);
     return -1;
   }
-  memset(out, 0, sizeof(double) * (restrict_dir ? C->N : F->N));
   uint32_t f = 0;
   int rc = 0;
   for (uint32_t c = 0; c < C->E && rc == 0; ++c) {
@@ -306,12 +519,12 @@ static int transfer(const or_level* C, const or_level* F, const double* rmat2,
     if (C->hnmask[c]) ctype = or_etype(cNumC, C->hnmask[c]);
     const uint32_t* cIdx = C->nlist + (size_t)c * 8;
     const int nsub = (F->lev[f] == C->lev[c]) ? 1 : 8;
-    for (int cNumF = 0; cNumF < nsub; ++cNumF, ++f) {
+    for (int cNumF = 0; cNumF < nsub && rc == 0; ++cNumF, ++f) {
       const double* R = (nsub == 1) ? rmat2 + ((size_t)cNumC * 18 + ctype) * 64
                                     : rmat1 + (((size_t)cNumC * 8 + cNumF) * 18 + ctype) * 64;
       const uint8_t fh = F->hnmask[f];
       const uint32_t* fIdx = F->nlist + (size_t)f * 8;
-      for (int fC = 0; fC < 8; ++fC) {
+      for (int fC = 0; fC < 8 && rc == 0; ++fC) {
         if (fh & (1 << fC)) continue;
         const uint32_t fn = fIdx[fC];
         for (int cC = 0; cC < 8; ++cC) {
@@ -330,17 +543,38 @@ static int transfer(const or_level* C, const or_level* F, const double* rmat2,
           }
           ids[tk.cap[fn]++] = cn;
           const double Rval = R[cC * 8 + fC];
-          if (restrict_dir)
-            out[cn] += Rval * in[fn];
-          else
-            out[fn] += Rval * in[cn];
+          if (Rval != 0.0 && pairs_push(out, fn, cn, Rval)) {
+            rc = -1;
+            break;
+          }
         }
       }
     }
   }
   free(tk.cap);
   free(tk.ids);
+  if (rc) pairs_free(out);
   return rc;
+}
+
+/* restrict: out[cn] += w in[fn]; prolong: out[fn] += w in[cn]; pairs in loop order */
+static void transfer_apply(const pairs_t* P, uint32_t n_out, const double* in, double* out,
+                           int restrict_dir) {
+  memset(out, 0, sizeof(double) * n_out);
+  if (restrict_dir)
+    for (size_t q = 0; q < P->n; ++q) out[P->cn[q]] += P->w[q] * in[P->fn[q]];
+  else
+    for (size_t q = 0; q < P->n; ++q) out[P->fn[q]] += P->w[q] * in[P->cn[q]];
+}
+
+static int transfer(const or_level* C, const or_level* F, const double* rmat2,
+                    const double* rmat1, const double* in, double* out, int restrict_dir) {
+  pairs_t P;
+  int rc = transfer_pairs(C, F, rmat2, rmat1, &P);
+  if (rc) return rc;
+  transfer_apply(&P, restrict_dir ? C->N : F->N, in, out, restrict_dir);
+  pairs_free(&P);
+  return 0;
 }
 
 int or_restrict(const or_level* C, const or_level* F, const double* rmat2, const double* rmat1,
@@ -352,6 +586,78 @@ int or_prolong(const or_level* C, const or_level* F, const double* rmat2, const 
   return transfer(C, F, rmat2, rmat1, c, f, 0);
 }
 
+/* ------------------------------------------------------------------ Jacobian3 */
+/* JAC_TYPE3_MULT_BLOCK (examples/src/backend/omgJac.C:2139-2233): the coarse operator formed
+ * from the next finer level's element coefficients.  The coarse and fine element loops run
+ * together (the fine element advanced until its anchor matches the coarse one).  A coarse
+ * element that is also a fine element takes the Type2 stencils of its (child, eType) with that
+ * fine element's coefficients; one with 8 fine children takes the Type2 stencils with their
+ * mean coefficients when max - min <= tol max for both, else the sum over the children of the
+ * Type1 stencils [child][eType][fine child] with each child's coefficients.  kf/mf: fine
+ * element coefficients (mf NULL: zero mass). */
+int or_matvec_jac3(const or_level* C, const or_level* F, const double* kf, const double* mf, double tol,
+                   const double* lap2, const double* mass2, const double* lap1, const double* mass1,
+                   const double* x, double* y) {
+  memset(y, 0, sizeof(double) * C->N);
+  const uint32_t maxD = C->max_depth;
+  const double hFac = 1.0 / ((double)(1u << (maxD - 1)));
+  uint32_t f = 0;
+  for (uint32_t c = 0; c < C->E; ++c) {
+    const uint32_t* ca = C->anchor + 3 * (size_t)c;
+    while (f < F->E && (F->anchor[3 * (size_t)f] != ca[0] || F->anchor[3 * (size_t)f + 1] != ca[1] ||
+                        F->anchor[3 * (size_t)f + 2] != ca[2]))
+      ++f;
+    if (f >= F->E) return -2;
+    const unsigned lev = C->lev[c];
+    const double h = hFac * (double)(1u << (maxD - lev));
+    const uint32_t* idx = C->nlist + (size_t)c * 8;
+    const int cn = C->child[c];
+    int t = 0;
+    if (C->hnmask[c]) t = or_etype(cn, C->hnmask[c]);
+    const double* L2 = lap2 + ((size_t)cn * 18 + t) * 64;
+    const double* M2 = mass2 + ((size_t)cn * 18 + t) * 64;
+    if (F->lev[f] == lev) {
+      const double fac1 = kf[f] * h / 2.0;
+      const double fac2 = (mf ? mf[f] : 0.0) * h * h * h / 8.0;
+      for (int k = 0; k < 8; ++k)
+        for (int j = 0; j < 8; ++j) y[idx[k]] += ((fac1 * L2[k * 8 + j]) + (fac2 * M2[k * 8 + j])) * x[idx[j]];
+      ++f;
+      continue;
+    }
+    if (f + 8 > F->E) return -2;
+    double p1[8], p2[8];
+    for (int q = 0; q < 8; ++q, ++f) {
+      p1[q] = kf[f];
+      p2[q] = mf ? mf[f] : 0.0;
+    }
+    double max1 = p1[0], min1 = p1[0], max2 = p2[0], min2 = p2[0], tot1 = 0.0, tot2 = 0.0;
+    for (int q = 0; q < 8; ++q) {
+      if (p1[q] > max1) max1 = p1[q];
+      if (p2[q] > max2) max2 = p2[q];
+      if (p1[q] < min1) min1 = p1[q];
+      if (p2[q] < min2) min2 = p2[q];
+      tot1 += p1[q];
+      tot2 += p2[q];
+    }
+    if (((max1 - min1) <= (tol * max1)) && ((max2 - min2) <= (tol * max2))) {
+      const double fac1 = tot1 * h / 16.0;
+      const double fac2 = tot2 * h * h * h / 64.0;
+      for (int k = 0; k < 8; ++k)
+        for (int j = 0; j < 8; ++j) y[idx[k]] += ((fac1 * L2[k * 8 + j]) + (fac2 * M2[k * 8 + j])) * x[idx[j]];
+    } else {
+      for (int q = 0; q < 8; ++q) {
+        const double fac1 = p1[q] * h / 2.0;
+        const double fac2 = p2[q] * h * h * h / 8.0;
+        const double* L1 = lap1 + ((((size_t)cn * 18 + t) * 8 + q) * 64);
+        const double* M1 = mass1 + ((((size_t)cn * 18 + t) * 8 + q) * 64);
+        for (int k = 0; k < 8; ++k)
+          for (int j = 0; j < 8; ++j) y[idx[k]] += ((fac1 * L1[k * 8 + j]) + (fac2 * M1[k * 8 + j])) * x[idx[j]];
+      }
+    }
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------------------ multigrid + CG */
 typedef struct {
   or_level lv;
@@ -359,6 +665,7 @@ typedef struct {
   double lap_fac, mass_fac;
   double* invdiag;
   double *b, *x, *r, *z;   /* work vectors of this level */
+  pairs_t tp;              /* transfer pairs from the next coarser level (level >= 1) */
 } mg_level;
 
 struct or_mg_s {
@@ -371,6 +678,8 @@ struct or_mg_s {
   int nc;
   double* lu;
   int* piv;
+  int factored;
+  int nthreads;  /* > 1: level products by or_matvec_mt */
 };
 
 or_mg* or_mg_create(int nlevels) {
@@ -394,6 +703,7 @@ void or_mg_free(or_mg* mg) {
     free(mg->L[l].x);
     free(mg->L[l].r);
     free(mg->L[l].z);
+    pairs_free(&mg->L[l].tp);
   }
   free(mg->L);
   free(mg->lap2);
@@ -428,6 +738,11 @@ int or_mg_set_level(or_mg* mg, int l, uint32_t E, uint32_t N, uint32_t maxD, con
 
 static void apply_A(or_mg* mg, int l, const double* x, double* y) {
   mg_level* L = &mg->L[l];
+  if (mg->nthreads > 1 &&
+      or_matvec_mt(mg->nthreads, L->lv.E, L->lv.N, L->lv.max_depth, L->lv.nlist, L->lv.child,
+                   L->lv.hnmask, L->lv.lev, L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2,
+                   mg->mass2, x, y) == 0)
+    return;
   or_matvec(L->lv.E, L->lv.N, L->lv.max_depth, L->lv.nlist, L->lv.child, L->lv.hnmask, L->lv.lev,
             L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2, mg->mass2, x, y);
 }
@@ -450,29 +765,44 @@ static void project_mean(double* v, uint32_t n) {
  * (the reference's -mg_coarse_pc_type lu, bin/options:203-206).  With a constant null space
  * the factored matrix is A + a 11^T (a = trace/n^2) applied to mean-free right-hand sides,
  * which returns the mean-free solution of A x = b. */
-static int coarse_factor(or_mg* mg) {
+static int coarse_assemble(or_mg* mg) {
   mg_level* L = &mg->L[0];
   const int n = (int)L->lv.N;
   mg->nc = n;
+  mg->factored = 0;
   mg->lu = (double*)calloc((size_t)n * n, sizeof(double));
   mg->piv = (int*)malloc(sizeof(int) * n);
   if (!mg->lu || !mg->piv) return -1;
-  double* e = (double*)calloc(n, sizeof(double));
-  double* col = (double*)malloc(sizeof(double) * n);
-  for (int j = 0; j < n; ++j) { /* column j = A e_j */
-    e[j] = 1.0;
-    apply_A(mg, 0, e, col);
-    e[j] = 0.0;
-    for (int i = 0; i < n; ++i) mg->lu[(size_t)i * n + j] = col[i];
+  /* element by element: entry (idx[k], idx[j]) += fac1 L[k][j] + fac2 M[k][j], elements in
+   * storage order -- the same terms in the same order as the column products A e_j */
+  const double hFac = 1.0 / ((double)(1u << (L->lv.max_depth - 1)));
+  for (uint32_t e = 0; e < L->lv.E; ++e) {
+    const double h = hFac * (double)(1u << (L->lv.max_depth - L->lv.lev[e]));
+    const double k1 = L->kappa ? L->kappa[e] : L->lap_fac;
+    const double k2 = L->kappa ? (L->mass ? L->mass[e] : 0.0) : L->mass_fac;
+    const double fac1 = k1 * h / 2.0;
+    const double fac2 = k2 * h * h * h / 8.0;
+    const int c = L->lv.child[e];
+    int t = 0;
+    if (L->lv.hnmask[e]) t = or_etype(c, L->lv.hnmask[e]);
+    const double* Ls = mg->lap2 + ((size_t)c * 18 + t) * 64;
+    const double* Ms = mg->mass2 + ((size_t)c * 18 + t) * 64;
+    const uint32_t* idx = L->lv.nlist + (size_t)e * 8;
+    for (int k = 0; k < 8; ++k)
+      for (int j = 0; j < 8; ++j)
+        mg->lu[(size_t)idx[k] * n + idx[j]] += ((fac1 * Ls[k * 8 + j]) + (fac2 * Ms[k * 8 + j])) * 1.0;
   }
-  free(e);
-  free(col);
   if (mg->null_space) {
     double tr = 0.0;
     for (int i = 0; i < n; ++i) tr += mg->lu[(size_t)i * n + i];
     const double a = tr / ((double)n * n);
     for (size_t q = 0; q < (size_t)n * n; ++q) mg->lu[q] += a;
   }
+  return 0;
+}
+
+static int coarse_factor(or_mg* mg) {
+  const int n = mg->nc;
   for (int k = 0; k < n; ++k) { /* LU with partial pivoting, row-major */
     int p = k;
     double best = fabs(mg->lu[(size_t)k * n + k]);
@@ -499,6 +829,7 @@ static int coarse_factor(or_mg* mg) {
       for (int j = k + 1; j < n; ++j) ri[j] -= f * rk[j];
     }
   }
+  mg->factored = 1;
   return 0;
 }
 
@@ -527,6 +858,10 @@ static void coarse_solve(or_mg* mg, const double* b, double* x) {
 }
 
 int or_mg_setup(or_mg* mg, double omega, int nu_pre, int nu_post, int null_space) {
+  return or_mg_setup_ex(mg, omega, nu_pre, nu_post, null_space, 1);
+}
+
+int or_mg_setup_ex(or_mg* mg, double omega, int nu_pre, int nu_post, int null_space, int factor) {
   if (!mg) return -1;
   mg->omega = omega;
   mg->nu_pre = nu_pre;
@@ -545,7 +880,36 @@ int or_mg_setup(or_mg* mg, double omega, int nu_pre, int nu_post, int null_space
     /* PCSetUp_Jacobi: the reciprocal of the diagonal */
     for (uint32_t i = 0; i < N; ++i) L->invdiag[i] = 1.0 / L->invdiag[i];
   }
+  for (int l = 1; l < mg->nlevels; ++l) {
+    pairs_free(&mg->L[l].tp);
+    int rp = transfer_pairs(&mg->L[l - 1].lv, &mg->L[l].lv, mg->rmat2, mg->rmat1, &mg->L[l].tp);
+    if (rp) return rp;
+  }
+  int rc = coarse_assemble(mg);
+  if (rc || !factor) return rc;
   return coarse_factor(mg);
+}
+
+int or_mg_set_threads(or_mg* mg, int nthreads) {
+  if (!mg || nthreads < 1 || nthreads > 256) return -1;
+  mg->nthreads = nthreads;
+  return 0;
+}
+
+int or_mg_coarse_size(or_mg* mg) { return mg ? mg->nc : -1; }
+
+int or_mg_coarse_matrix(or_mg* mg, double* A) {
+  if (!mg || !mg->lu || mg->factored) return -1;
+  memcpy(A, mg->lu, sizeof(double) * (size_t)mg->nc * mg->nc);
+  return 0;
+}
+
+int or_mg_coarse_set_lu(or_mg* mg, const double* lu, const int* piv) {
+  if (!mg || !mg->lu) return -1;
+  memcpy(mg->lu, lu, sizeof(double) * (size_t)mg->nc * mg->nc);
+  memcpy(mg->piv, piv, sizeof(int) * mg->nc);
+  mg->factored = 1;
+  return 0;
 }
 
 /* KSPRICHARDSON + PCJACOBI, norm type none (bin/options:187-193):
@@ -584,13 +948,11 @@ static int vcycle(or_mg* mg, int l, const double* b, double* x) {
   richardson(mg, l, b, x, mg->nu_pre, 1);
   apply_A(mg, l, x, L->r);
   for (uint32_t i = 0; i < N; ++i) L->r[i] = b[i] - L->r[i];
-  int rc = or_restrict(&Lc->lv, &L->lv, mg->rmat2, mg->rmat1, L->r, Lc->b);
-  if (rc) return rc;
+  transfer_apply(&L->tp, Lc->lv.N, L->r, Lc->b, 1);
   memset(Lc->x, 0, sizeof(double) * Lc->lv.N);
-  rc = vcycle(mg, l - 1, Lc->b, Lc->x);
+  int rc = vcycle(mg, l - 1, Lc->b, Lc->x);
   if (rc) return rc;
-  rc = or_prolong(&Lc->lv, &L->lv, mg->rmat2, mg->rmat1, Lc->x, L->z);
-  if (rc) return rc;
+  transfer_apply(&L->tp, N, Lc->x, L->z, 0);
   for (uint32_t i = 0; i < N; ++i) x[i] = x[i] + L->z[i];
   richardson(mg, l, b, x, mg->nu_post, 0);
   return 0;
@@ -598,6 +960,7 @@ static int vcycle(or_mg* mg, int l, const double* b, double* x) {
 
 int or_mg_vcycle(or_mg* mg, const double* r, double* z) {
   if (!mg) return -1;
+  if (!mg->factored) return -4;
   const int top = mg->nlevels - 1;
   int rc = vcycle(mg, top, r, z);
   if (mg->null_space) project_mean(z, mg->L[top].lv.N);

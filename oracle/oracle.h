@@ -40,12 +40,21 @@ extern "C" {
 
 int or_etype(int child, int mask);
 void or_stencils(double* lap2, double* mass2, double* rmat2, double* rmat1);
+/* Type1 element stencils [8 coarse child][18][8 fine child][8][8] (Jacobian3) */
+void or_stencils_type1(double* lap1, double* mass1);
+/* shape functions at the 3-point Gauss nodes [8][18][8 vertex][3][3][3] (ComputeRHS2) */
+void or_shfn(double* shfn);
 
 /* y = A x over one DA level; kappa == NULL selects Jacobian1 with lap_fac/mass_fac. */
 void or_matvec(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist,
                const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
                const double* kappa, const double* mass, double lap_fac, double mass_fac,
                const double* lap2, const double* mass2, const double* x, double* y);
+/* or_matvec on nthreads threads (all-core CPU baseline); rows may sum in another order */
+int or_matvec_mt(int nthreads, uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist,
+                 const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
+                 const double* kappa, const double* mass, double lap_fac, double mass_fac,
+                 const double* lap2, const double* mass2, const double* x, double* y);
 void or_diagonal(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist,
                  const uint8_t* child, const uint8_t* hnmask, const uint8_t* lev,
                  const double* kappa, const double* mass, double lap_fac, double mass_fac,
@@ -77,6 +86,12 @@ int or_restrict(const or_level* coarse, const or_level* fine, const double* rmat
 int or_prolong(const or_level* coarse, const or_level* fine, const double* rmat2,
                const double* rmat1, const double* c, double* f);
 
+/* Jacobian3MatMult (omgJac.C:2139-2233): coarse-level product with the fine level's element
+ * coefficients kf/mf (mf NULL: zero mass); returns 0, or -2 when the meshes do not nest. */
+int or_matvec_jac3(const or_level* coarse, const or_level* fine, const double* kf, const double* mf,
+                   double tol, const double* lap2, const double* mass2, const double* lap1,
+                   const double* mass1, const double* x, double* y);
+
 /* Multigrid hierarchy (coarsest level 0) for the V-cycle and PCG. */
 typedef struct or_mg_s or_mg;
 or_mg* or_mg_create(int nlevels);
@@ -87,6 +102,16 @@ int or_mg_set_level(or_mg* mg, int level, uint32_t E, uint32_t N, uint32_t max_d
                     const uint8_t* lev, const uint32_t* anchor, const double* kappa,
                     const double* mass, double lap_fac, double mass_fac);
 int or_mg_setup(or_mg* mg, double omega, int nu_pre, int nu_post, int null_space);
+/* factor = 0: assemble the coarsest matrix but leave it unfactored; or_mg_coarse_matrix
+ * copies it out (n = or_mg_coarse_size) and or_mg_coarse_set_lu installs an LU factor with
+ * LAPACK getrf conventions (row-major L\U, unit L, piv[k] = row swapped with k at step k),
+ * e.g. from a blocked library factorisation of a large coarsest level. */
+int or_mg_setup_ex(or_mg* mg, double omega, int nu_pre, int nu_post, int null_space, int factor);
+int or_mg_coarse_size(or_mg* mg);
+/* level products on nthreads threads (or_matvec_mt); 1 (default) keeps or_matvec */
+int or_mg_set_threads(or_mg* mg, int nthreads);
+int or_mg_coarse_matrix(or_mg* mg, double* A);
+int or_mg_coarse_set_lu(or_mg* mg, const double* lu, const int* piv);
 int or_mg_vcycle(or_mg* mg, const double* r, double* z);
 /* PETSc-KSPCG restatement; returns iterations, *converged 0/1, history[0..its] norms */
 int or_pcg(or_mg* mg, const double* b, double* x, double rtol, int max_it, int norm_type,

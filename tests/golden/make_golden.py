@@ -6,6 +6,10 @@ stencils_ref.npz: the reference's element and restriction stencil tables, parsed
   Stencils/LapType2Stencils.inp, MassType2Stencils.inp, RmatType2Stencils.inp and
   RmatType1Stencils.inp with the same whitespace-separated "%lf" reading as
   examples/src/backend/handleType2Stencils.C:776-794 and src/omg/RmatType{1,2}StencilsAux.C.
+stencils_ref2.npz: LapType1Stencils.inp, MassType1Stencils.inp ([coarse child][eType][fine
+  child][8][8], examples/src/backend/handleType1Stencils.C:439-520), ShFnStencils.inp
+  ([child][eType][vertex][3][3][3], handleType2Stencils.C:212-225) and GDType2Stencils.inp
+  ([child][eType][24][24], handleType2Stencils.C:315-323), read the same way.
 etype_masks.json: the 18 hanging masks of each child number in eType order, parsed from
   the enum values of include/oda/cnumEtypes.h.
 Only data values are extracted (numbers), never source text.
@@ -34,6 +38,13 @@ def main():
         mass2=table("MassType2Stencils.inp", (8, 18, 8, 8)),
         rmat2=table("RmatType2Stencils.inp", (8, 18, 8, 8)),
         rmat1=table("RmatType1Stencils.inp", (8, 8, 18, 8, 8)),
+    )
+    np.savez_compressed(
+        os.path.join(OUT, "stencils_ref2.npz"),
+        lap1=table("LapType1Stencils.inp", (8, 18, 8, 8, 8)),
+        mass1=table("MassType1Stencils.inp", (8, 18, 8, 8, 8)),
+        shfn=table("ShFnStencils.inp", (8, 18, 8, 3, 3, 3)),
+        gd2=table("GDType2Stencils.inp", (8, 18, 24, 24)),
     )
     txt = open(os.path.join(REF, "include", "oda", "cnumEtypes.h")).read()
     parts = re.split(r"class cNumEtype<(\d)>", txt)

@@ -1,14 +1,12 @@
 """The benchmark configurations at their full sizes on the GPU.
 
-Full-size parity uses what does not depend on size: the FP64 MatVec against the oracle on
-the whole finest mesh (<= 1e-12), the FP64 true residual of every solve, agreement between
-the FP64, mixed-precision and learned-damping solves, and convergence of every right-hand
-side of a batch.  Configurations 1 and 2 build in about a second and always run; the
-8M-point (mixed precision) and 4M-point (learned damping) octrees take about a minute of
-host time each and run when DGH_FULL_CONFIGS=1.
+Full-size parity: the FP64 MatVec against the oracle on the whole finest mesh (<= 1e-12)
+in every configuration; on the 1M log-normal octree also the V-cycle and the PCG iterates
+k = 1, 3, 5 against the oracle's (<= 1e-12) and the 32 batched MatVec columns; the FP64
+true residual of every solve; agreement between the FP64, mixed-precision and
+learned-damping solves; convergence of every right-hand side of a batch.  The 8M-point and
+4M-point octrees take about a minute of host time each to build.
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -19,8 +17,6 @@ from tests.meshes import rel_err
 pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
-
-FULL = os.environ.get("DGH_FULL_CONFIGS") == "1"
 
 
 @pytest.fixture(scope="module")
@@ -105,7 +101,99 @@ def test_config2_lognormal_poisson(dev):
     assert rel_err(x32, x64) <= 1e-8
 
 
-@pytest.mark.skipif(not FULL, reason="8M-point octree: set DGH_FULL_CONFIGS=1")
+@pytest.fixture(scope="module")
+def config2():
+    damg, levels = _build(da.PTS_LOGNORMAL, 1000000, 12, 5, 2)
+    kf = 10.0 ** np.random.default_rng(21).uniform(-1, 1, levels[-1].n_elem)
+    return damg, levels, kf
+
+
+def test_config2_iterates_vs_oracle(dev, config2):
+    """The benchmark's solve on the whole 1M log-normal octree (Jacobian2, zero mass,
+    constant null space, 5 levels): one V-cycle and the PCG iterates k = 1, 3, 5 of the
+    plain FP64 GPU path against the oracle (<= 1e-12), with the stopping-test norms; the
+    FP32 V-cycle's iterates stated against the same oracle iterates."""
+    damg, levels, kf = config2
+    F = levels[-1]
+    ms, ts = _device_hierarchy(damg, levels, kf)
+    co = [damg.coarsen_coefficients(kf, None, l)[0] for l in range(len(levels))]
+    mg = oracle.MG([(levels[l], co[l], None, 1.0, 1.0) for l in range(len(levels))],
+                   omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=True)
+    b = np.random.default_rng(22).uniform(-1, 1, F.n_node)
+    bt = torch.tensor(b, dtype=torch.float64, device=dev)
+    r = b - b.mean()
+    s = da.Solver(ms, ts, da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=1))
+    z = torch.empty_like(bt)
+    s.vcycle(torch.tensor(r, device=dev), z)
+    torch.cuda.synchronize()
+    ev = rel_err(z.cpu().numpy(), mg.vcycle(r))
+    print(f"config 2 V-cycle vs oracle: {ev:.2e}")
+    assert ev <= 1e-11
+    for k in (1, 3, 5):
+        xo, its, _, ho = mg.pcg(b, rtol=1e-30, max_it=k, norm_type=da.NORM_UNPRECONDITIONED)
+        for prec in (da.F64, da.F32):
+            p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=k, rtol=1e-30,
+                                  norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=prec)
+            s = da.Solver(ms, ts, p)
+            x = torch.zeros_like(bt)
+            st, hist = s.solve(bt, x, history_len=k + 1)
+            torch.cuda.synchronize()
+            err = rel_err(x.cpu().numpy(), xo)
+            print(f"config 2 PCG iterate k={k} prec={prec}: rel err vs oracle {err:.2e}")
+            assert st.iterations == its == k
+            if prec == da.F64:
+                assert err <= 1e-12
+                assert np.allclose(hist, ho, rtol=1e-10, atol=0)
+            else:
+                assert err <= 1e-5
+
+
+def test_config4_batched_32rhs(dev, config2):
+    """Config 4: the 1M log-normal octree with 32 right-hand sides (seeds 400..431) in one
+    batched solve (FP32 batched V-cycle captured in a hipGraph, FP64 outer CG per column):
+    every column meets its own 1e-8 true relative residual; the batched FP64 MatVec's 32
+    columns match the oracle (<= 1e-12) and the FP32 ones are stated."""
+    damg, levels, kf = config2
+    F = levels[-1]
+    N = F.n_node
+    k = 32
+    ms, ts = _device_hierarchy(damg, levels, kf)
+    X = np.random.default_rng(11).uniform(-1, 1, (N, k))
+    Xt = torch.tensor(X, dtype=torch.float64, device=dev)
+    Yt = torch.empty_like(Xt)
+    ms[-1].matvec_multi(Xt, Yt, k, prec=da.F64)
+    X32 = Xt.float()
+    Y32 = torch.empty_like(X32)
+    ms[-1].matvec_multi(X32, Y32, k, prec=da.F32)
+    torch.cuda.synchronize()
+    Y, Y32 = Yt.cpu().numpy(), Y32.double().cpu().numpy()
+    worst64 = worst32 = 0.0
+    for j in range(k):
+        ref = oracle.matvec(F, X[:, j], kappa=kf)
+        worst64 = max(worst64, rel_err(Y[:, j], ref))
+        worst32 = max(worst32, rel_err(Y32[:, j], ref))
+    print(f"config 4 batched MatVec, 32 columns vs oracle: FP64 worst {worst64:.2e}, FP32 worst {worst32:.2e}")
+    assert worst64 <= 1e-12 and worst32 <= 1e-5
+    del Xt, Yt, X32
+    B = np.stack([np.random.default_rng(400 + j).uniform(-1, 1, N) for j in range(k)], 1)
+    Bt = torch.tensor(B, dtype=torch.float64, device=dev)
+    Xs = torch.zeros_like(Bt)
+    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=500, rtol=1e-8,
+                          norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=da.F32, use_graph=1)
+    s = da.Solver(ms, ts, p)
+    st = s.solve_multi(Bt, Xs, k)
+    its = [x.iterations for x in st]
+    print(f"config 4: 32 RHS in {st[0].seconds*1e3:.1f} ms, iterations {min(its)}-{max(its)}, "
+          f"max rel_true {max(x.rel_true for x in st):.2e}")
+    assert all(x.converged and x.rel_true <= 1e-8 for x in st)
+    # a column of the batch against the single-RHS FP64 solve of the same column
+    _, st1, x1 = _solve(ms, ts, B[:, 7], dev, null_space=1)
+    torch.cuda.synchronize()
+    e = rel_err(Xs[:, 7].cpu().numpy(), x1)
+    print(f"config 4 column 7 vs single FP64 solve: {e:.2e}")
+    assert e <= 1e-6
+
+
 def test_config3_mixed_8m(dev):
     """8M points (4M uniform + 4M in 8 Gaussian clusters), depth 14: mixed-precision PCG
     against the FP64 solve on the same octree."""
@@ -128,7 +216,6 @@ def test_config3_mixed_8m(dev):
     assert rel_err(x32, x64) <= 1e-6
 
 
-@pytest.mark.skipif(not FULL, reason="4M-point octree: set DGH_FULL_CONFIGS=1")
 def test_config5_learned_4m(dev):
     """4M points in 8 Gaussian clusters, depth 13, log-normal kappa with contrast 1e3:
     learned damping (trained on 200 solver-generated sample batches, seeds 5000..5199)
@@ -138,6 +225,23 @@ def test_config5_learned_4m(dev):
     z = np.random.default_rng(52).standard_normal(F.n_elem)
     kf = np.clip(np.exp(np.log(10 ** 1.5) / 2.0 * z), 10 ** -1.5, 10 ** 1.5)
     ms, ts = _device_hierarchy(damg, levels, kf)
+    x = np.random.default_rng(11).uniform(-1, 1, F.n_node)
+    xt = torch.tensor(x, device=dev)
+    yt = torch.empty_like(xt)
+    ms[-1].matvec(xt, yt)
+    y_ref = oracle.matvec(F, x, kappa=kf)
+    y = yt.cpu().numpy()
+    e = rel_err(y, y_ref)
+    # per node, against the scale of the terms its row sums: for these zero-row-sum rows
+    # sum_j |A_nj| = 2 A_nn, and |x| <= 1.  Every term is rounded to 2^-50 of its tile's bound,
+    # so a low-kappa row in a tile with kappa 1e3 times larger keeps about 2^-50 * 1e3 ~ 1e-12
+    # of its own scale: bounded here at 1e-11.
+    row_scale = 2.0 * np.abs(oracle.diagonal(F, kappa=kf))
+    worst = float(np.max(np.abs(y - y_ref) / row_scale))
+    print(f"config 5 MatVec vs oracle: {e:.2e}; worst node error / (sum_j |A_nj|) {worst:.2e}")
+    assert e <= 1e-12
+    assert worst <= 1e-11
+    del xt, yt
     b = np.random.default_rng(22).uniform(-1, 1, F.n_node)
     s, stj, xj = _solve(ms, ts, b, dev, null_space=1, vcycle_prec=da.F32)
     s.learned_init(1)

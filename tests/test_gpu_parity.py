@@ -10,7 +10,7 @@ import pytest
 
 import dendro_amd as da
 import oracle
-from tests.meshes import hierarchy, rel_err, var_coefficients
+from tests.meshes import hierarchy, matvec_longdouble, node_coords, rel_err, var_coefficients
 
 pytestmark = pytest.mark.gpu
 
@@ -76,6 +76,45 @@ def test_matvec_fp64(dev, hier_small, mode, lane):
     assert rel_err(_np(yt), y_ref) <= TOL64
 
 
+@pytest.mark.parametrize("case", ["cos_mass", "near_constant"])
+def test_matvec_smooth_x(dev, hier_small, case):
+    """Smooth inputs, where A x is much smaller than the terms each row sums.  The fixed-point
+    row sums round every element-row term to 2^-50 of its tile's bound (the largest element
+    factor times stencil row sum in the tile, times max |x| over the tile's nodes, each rounded
+    up to a power of two), not to 2^-53 of the result: with kappa spread over 10^(-1..1) in a
+    tile that is about 2^-45 of a row's |A||x|, where FP64 summation keeps about 2^-54.
+    Against an extended-precision product (64-bit significand) the GPU error is bounded by
+    2^-43 of |A||x| (a 4x margin on the derivation), and by 1e-12 of |A x| when A x is not
+    dominated by cancellation.  cos_mass: x = cos(pi X) cos(pi Y) cos(pi Z) with mass;
+    near_constant: x = 1 + 1e-3 of the same with zero mass (constants are in the Laplacian's
+    null space, |A x| ~ 1e-5 |A||x|: no FP64 evaluation, the oracle's included, keeps 1e-12 of
+    |A x| there)."""
+    h = hier_small
+    F = h.fine
+    xyz = node_coords(F)
+    w = np.cos(np.pi * xyz[:, 0]) * np.cos(np.pi * xyz[:, 1]) * np.cos(np.pi * xyz[:, 2])
+    c = var_coefficients(h, mass_base=1.0 if case == "cos_mass" else None)[-1]
+    x = w if case == "cos_mass" else 1.0 + 1e-3 * w
+    ex = matvec_longdouble(F, x, kappa=c[0], mass=c[1])
+    scale = np.linalg.norm(matvec_longdouble(F, x, kappa=c[0], mass=c[1], absolute=True).astype(np.float64))
+    y_or = oracle.matvec(F, x, kappa=c[0], mass=c[1])
+    m = da.Mesh(F)
+    m.set_coefficients(c[0], c[1])
+    xt = _t(x, dev)
+    yt = torch.empty_like(xt)
+    m.matvec(xt, yt)
+    y = _np(yt)
+    e_gpu = float(np.linalg.norm((y - ex).astype(np.float64)))
+    e_or = float(np.linalg.norm((y_or - ex).astype(np.float64)))
+    nrm = float(np.linalg.norm(ex.astype(np.float64)))
+    print(f"{case}: |Ax|/(|A||x|) = {nrm / scale:.2e}; error vs extended precision relative to |A||x|: "
+          f"GPU {e_gpu / scale:.2e} (2^{np.log2(e_gpu / scale):.1f}), oracle {e_or / scale:.2e}; "
+          f"relative to |Ax|: GPU {e_gpu / nrm:.2e}, oracle {e_or / nrm:.2e}")
+    assert e_gpu <= 2.0 ** -43 * scale
+    if case == "cos_mass":
+        assert e_gpu <= 1e-12 * nrm
+
+
 @pytest.mark.parametrize("prec", [da.F64, da.F32])
 def test_matvec_reproducible_and_scale_exact(dev, hier_small, prec):
     """The operator sums owned-node rows in 64-bit fixed point with a per-tile scale taken
@@ -105,6 +144,30 @@ def test_matvec_reproducible_and_scale_exact(dev, hier_small, prec):
     m.matvec(xn, yn, prec=prec)
     torch.cuda.synchronize()
     assert torch.isnan(yn[F.n_node // 2])
+
+
+@pytest.mark.parametrize("lane", [0, 1])
+def test_matvec_dot(dev, hier_small, lane):
+    """y = A x with x.y from the operator's tile partials: y bit-identical to dgh_matvec,
+    both against the oracle (<= 1e-12), and x.y against the FP64 dot of the oracle's y."""
+    h = hier_small
+    F = h.fine
+    c = var_coefficients(h, mass_base=1.0)[-1]
+    x = np.random.default_rng(17).uniform(-1, 1, F.n_node)
+    y_ref = oracle.matvec(F, x, kappa=c[0], mass=c[1])
+    m = da.Mesh(F)
+    m.set_coefficients(c[0], c[1])
+    m.set_lane_stencils(lane)
+    xt = _t(x, dev)
+    y1, y2 = torch.empty_like(xt), torch.empty_like(xt)
+    xy = torch.full((1,), float("nan"), dtype=torch.float64, device=dev)
+    m.matvec(xt, y1)
+    m.matvec_dot(xt, y2, xy)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    assert rel_err(_np(y2), y_ref) <= TOL64
+    d_ref = float(np.dot(x, y_ref))
+    assert abs(float(xy.item()) - d_ref) <= 1e-12 * abs(d_ref)
 
 
 @pytest.mark.parametrize("lane", [0, 1])
@@ -188,6 +251,50 @@ def test_pcg_iterates_fp64(dev, hier_tiny, norm_type, lane):
         assert st.iterations == its == k
         assert rel_err(_np(x), xo) <= TOL64, k
         assert np.allclose(hist, ho, rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("norm_type", [da.NORM_PRECONDITIONED, da.NORM_UNPRECONDITIONED])
+def test_pcg_iterates_jac2_null_space(dev, hier_small, norm_type):
+    """The operator the benchmark solves: Jacobian2 with kappa = 10^U(-1,1), zero mass and the
+    constant null space projected, FP64.  V-cycle and PCG iterates k = 1, 3, 5 against the
+    oracle (<= 1e-12), stopping-test norms to 1e-10; the FP32 V-cycle's iterates and its
+    converged solution are stated against the same oracle run."""
+    h = hier_small
+    coefs = var_coefficients(h, mass_base=None)
+    ms, ts = _meshes(h, coefs)
+    mg = oracle.MG([(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)],
+                   omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=True)
+    b = np.random.default_rng(22).uniform(-1, 1, h.fine.n_node)
+    bt = _t(b, dev)
+    r = b - b.mean()
+    s = da.Solver(ms, ts, da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=1))
+    z = torch.empty_like(bt)
+    s.vcycle(_t(r, dev), z)
+    assert rel_err(_np(z), mg.vcycle(r)) <= 1e-11
+    for k in (1, 3, 5):
+        xo, its, _, ho = mg.pcg(b, rtol=1e-30, max_it=k, norm_type=norm_type)
+        for prec in (da.F64, da.F32):
+            p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=k, rtol=1e-30,
+                                  norm_type=norm_type, null_space=1, vcycle_prec=prec)
+            s = da.Solver(ms, ts, p)
+            x = torch.zeros_like(bt)
+            st, hist = s.solve(bt, x, history_len=k + 1)
+            err = rel_err(_np(x), xo)
+            assert st.iterations == its == k
+            if prec == da.F64:
+                assert err <= TOL64, (k, err)
+                assert np.allclose(hist, ho, rtol=1e-10, atol=0)
+            else:
+                print(f"k={k} norm_type={norm_type}: FP32 V-cycle iterate vs FP64 oracle {err:.2e}")
+                assert err <= 1e-5
+    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=200, rtol=1e-10,
+                          norm_type=norm_type, null_space=1, vcycle_prec=da.F32)
+    x = torch.zeros_like(bt)
+    st, _ = da.Solver(ms, ts, p).solve(bt, x)
+    xo, its, conv, _ = mg.pcg(b, rtol=1e-10, max_it=200, norm_type=norm_type)
+    err = rel_err(_np(x), xo)
+    print(f"converged to 1e-10: mixed {st.iterations} its, oracle FP64 {its} its, solution error {err:.2e}")
+    assert st.converged and conv and err <= 1e-6
 
 
 @pytest.mark.parametrize("lane", [0, 1])

@@ -21,6 +21,8 @@
 PETSc's own floating-point order (ddot, LU pivoting) is not in the reference tree; the
 iterates are pinned to these definitions up to rounding.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -375,3 +377,74 @@ def test_pcg_negative_control(dense_tiny):
     for k in (2, 3):
         xo, _, _, _ = mg.pcg(b, rtol=1e-30, max_it=k, norm_type=0)
         assert rel_err(xo, xs[k - 1]) > 1e-6
+
+
+def test_threaded_matvec_equals_serial(hier_small):
+    """or_matvec_mt (the all-core CPU baseline) is the serial element loop split over element
+    ranges: the same products, rows summed in another order."""
+    F = hier_small.fine
+    c = var_coefficients(hier_small, mass_base=1.0)[-1]
+    x = np.random.default_rng(3).uniform(-1, 1, F.n_node)
+    ref = oracle.matvec(F, x, kappa=c[0], mass=c[1])
+    for t in (1, 3, 8, 64):
+        assert rel_err(oracle.matvec_mt(F, x, kappa=c[0], mass=c[1], threads=t), ref) <= 1e-14, t
+
+
+@pytest.mark.parametrize("ns", [False, True])
+def test_lapack_coarse_factor_equals_oracle_lu(hier_tiny, ns, monkeypatch):
+    """A large coarsest level is factored by LAPACK getrf (scipy) instead of the oracle's
+    unblocked loop; both factor the same assembled matrix with partial pivoting, so the
+    V-cycles agree to rounding."""
+    h = hier_tiny
+    coefs = var_coefficients(h, mass_base=None if ns else 1.0)
+    lv = [(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)]
+    r = np.random.default_rng(9).uniform(-1, 1, h.fine.n_node)
+    if ns:
+        r -= r.mean()
+    z_c = oracle.MG(lv, null_space=ns).vcycle(r)
+    monkeypatch.setattr(oracle.MG, "LAPACK_ABOVE", 0)
+    z_l = oracle.MG(lv, null_space=ns).vcycle(r)
+    assert rel_err(z_l, z_c) <= 1e-12
+
+
+# ------------------------------------------------------------------ Jacobian3
+def test_type1_and_shape_function_stencils_match_reference_tables():
+    """or_stencils_type1 / or_shfn against the reference's LapType1Stencils.inp,
+    MassType1Stencils.inp and ShFnStencils.inp (fixture), and what fixes them: the Type1
+    stencils of a coarse element sum over its 8 fine octants to its Type2 stencils, and the
+    shape functions are a partition of unity at every Gauss node."""
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "stencils_ref2.npz"))
+    s1 = oracle.stencils_type1()
+    s2 = oracle.stencils()
+    assert np.abs(s1["lap1"] - ref["lap1"]).max() <= 1e-15
+    assert np.abs(s1["mass1"] - ref["mass1"]).max() <= 1e-15
+    assert np.abs(s1["shfn"] - ref["shfn"]).max() <= 1e-15
+    assert np.abs(s1["lap1"].sum(2) - s2["lap2"]).max() <= 1e-14
+    assert np.abs(s1["mass1"].sum(2) - s2["mass2"]).max() <= 1e-14
+    assert np.abs(s1["shfn"].sum(2) - 1.0).max() <= 1e-14
+
+
+def test_jacobian3_is_one_level_galerkin(hier_tiny):
+    """Jacobian3 on a coarse level with the fine level's element coefficients equals the
+    one-level Galerkin product P^T A_f P (A_f: the fine level's Jacobian2 with its own
+    coefficients, P: the prolongation), since the coarse space is nested in the fine one.
+    With a huge tolerance every refined element takes the mean of its children: the result is
+    then Jacobian2 on the coarse level with the coarsened coefficients.  Negative control:
+    Jacobian2 with the coarsened coefficients differs from the Galerkin product."""
+    h = hier_tiny
+    l = h.n - 1
+    C, F = h.levels[l - 1], h.levels[l]
+    kf = 10.0 ** np.random.default_rng(8).uniform(-1, 1, F.n_elem)
+    mf = np.random.default_rng(9).uniform(0.5, 2.0, F.n_elem)
+    Af = _sparse_from_columns(lambda e: oracle.matvec(F, e, kappa=kf, mass=mf), F.n_node)
+    P = _sparse_from_columns(lambda e: oracle.prolong(C, F, e), C.n_node, F.n_node)
+    G = (P.T @ Af @ P).toarray()
+    A3 = _sparse_from_columns(lambda e: oracle.matvec_jac3(C, F, e, kf, mf), C.n_node).toarray()
+    scale = np.abs(G).max()
+    assert np.abs(A3 - G).max() <= 1e-13 * scale
+    assert np.abs(A3 - A3.T).max() <= 1e-14 * scale
+    kc, mc = h.damg.coarsen_coefficients(kf, mf, l - 1)
+    A2c = _sparse_from_columns(lambda e: oracle.matvec(C, e, kappa=kc, mass=mc), C.n_node).toarray()
+    assert np.abs(A2c - G).max() > 1e-3 * scale
+    x = np.random.default_rng(10).uniform(-1, 1, C.n_node)
+    assert rel_err(oracle.matvec_jac3(C, F, x, kf, mf, tol=1e300), A2c @ x) <= 1e-14
