@@ -17,6 +17,7 @@ F64, F32 = 0, 1
 NORM_UNPRECONDITIONED, NORM_PRECONDITIONED = 0, 1
 SMOOTHER_JACOBI, SMOOTHER_LEARNED = 0, 1
 PTS_GAUSS, PTS_LOGNORMAL, PTS_MIXED, PTS_CLUSTERS, PTS_UNIFORM = range(5)
+NOALLOC_MAX_TILES = 16384     # DGH_NOALLOC_MAX_TILES
 
 # every symbol include/dendro_hip.h declares
 EXPORTS = [
@@ -32,7 +33,7 @@ EXPORTS = [
     "dgh_diagonal_dirichlet", "dgh_mesh_info", "dgh_mesh_destroy",
     "dgh_mesh_set_tile", "dgh_mesh_tile_info", "dgh_mesh_autotune", "dgh_mesh_set_lane_stencils",
     "dgh_mesh_lane_stencils",
-    "dgh_matvec", "dgh_matvec_dot", "dgh_diagonal", "dgh_matvec_multi", "dgh_mesh_set_tile_multi",
+    "dgh_matvec", "dgh_matvec_dot", "dgh_matvec_dot_noalloc", "dgh_diagonal", "dgh_matvec_multi", "dgh_mesh_set_tile_multi",
     "dgh_mesh_autotune_multi",
     "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add",
     "dgh_solver_default_params", "dgh_solver_create", "dgh_solver_destroy", "dgh_solver_solve",
@@ -335,6 +336,10 @@ class Mesh:
     def matvec_dot(self, x, y, xy, stream=None):
         """y = A x and xy[0] = x.y (FP64; xy a one-element device tensor)."""
         check(lib().dgh_matvec_dot(self.h, ptr(x), ptr(y), ptr(xy), _stream(stream)))
+
+    def matvec_dot_noalloc(self, x, y, xy, stream=None):
+        """matvec_dot with its tile partials in the library's fixed device array (no allocation)."""
+        check(lib().dgh_matvec_dot_noalloc(self.h, ptr(x), ptr(y), ptr(xy), _stream(stream)))
 
     def matvec_multi(self, x, y, k, prec=F64, stream=None):
         """Y = A X for k columns stored node-major ([N][k])."""

@@ -15,6 +15,12 @@
 
 namespace dgh {
 State g;
+
+// Tile partials of dgh_matvec_dot_noalloc: a file-scope device array, one copy per device,
+// held until the process ends.  In host code `scratch` names a host-side stand-in, not device
+// memory: host code uses only the address dgh_init obtains from hipGetSymbolAddress
+// (g.noalloc_part[device]).
+static __device__ __align__(256) double scratch[DGH_NOALLOC_MAX_TILES];
 }  // namespace dgh
 
 namespace dgh {
@@ -499,8 +505,16 @@ extern "C" {
 
 int dgh_init(int device) {
   return guarded([&] {
+    if (device < 0 || device >= State::MAX_DEVICES) fail(DGH_ERR_ARG, "device %d out of range", device);
     HIP_CHECK(hipSetDevice(device));
     g.device = device;
+    if (!g.noalloc_part[device]) {   // once per device, with that device current
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        fail(DGH_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      g.noalloc_part[device] = static_cast<double*>(p);
+    }
     int lds = 0;
     HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     g.max_lds = lds > 0 ? lds : 65536;
@@ -785,6 +799,27 @@ int dgh_matvec_dot(dgh_mesh m, const double* x, double* y, double* xy, void* str
     if (m->dotp.n < std::max<size_t>(nt, 1)) m->dotp.alloc(std::max<size_t>(nt, 1));
     if (nt) launch_cg(m, x, y, m->dotp.p, st);
     hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(RED2_THREADS), 0, st, int(nt), (const double*)m->dotp.p, xy);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int dgh_matvec_dot_noalloc(dgh_mesh m, const double* x, double* y, double* xy, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!m || !x || !y || !xy) fail(DGH_ERR_ARG, "NULL argument");
+    if (x == y) fail(DGH_ERR_ARG, "x and y must not alias");
+    const size_t nt = m->op.plan.tiles.size();
+    if (nt > size_t(DGH_NOALLOC_MAX_TILES))
+      fail(DGH_ERR_ARG, "mesh has %zu operator tiles; dgh_matvec_dot_noalloc holds at most %d (use larger tiles "
+           "or dgh_matvec_dot)", nt, DGH_NOALLOC_MAX_TILES);
+    int dev = -1;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= State::MAX_DEVICES || !g.noalloc_part[dev])
+      fail(DGH_ERR_STATE, "dgh_init has not been called on device %d", dev);
+    double* part = g.noalloc_part[dev];
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (nt) launch_cg(m, x, y, part, st);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(RED2_THREADS), 0, st, int(nt), (const double*)part, xy);
     HIP_CHECK(hipGetLastError());
   });
 }

@@ -74,6 +74,10 @@ struct State {
   DevBuf<float> KF32;
   double lrow[NSTENCIL] = {}, mrow[NSTENCIL] = {};   // max row sums of |L|, |M| per type
   rocblas_handle blas = nullptr;
+  // device address of the file-scope scratch array of dgh_matvec_dot_noalloc, per device id
+  // (looked up by dgh_init with that device current)
+  static constexpr int MAX_DEVICES = 64;
+  double* noalloc_part[MAX_DEVICES] = {};
 };
 extern State g;
 

@@ -214,6 +214,14 @@ int dgh_matvec(dgh_mesh m, int prec, const void* x_dev, void* y_dev, void* strea
  * library allocates for the mesh on the first call (and again after a re-tiling that adds
  * tiles). */
 int dgh_matvec_dot(dgh_mesh m, const double* x_dev, double* y_dev, double* xy_dev, void* stream);
+/* dgh_matvec_dot without any allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight solve loops, harnesses
+ * that forbid hipMalloc).  The per-tile partials go to a fixed device array of
+ * DGH_NOALLOC_MAX_TILES doubles inside the library (one per GPU, held until the process
+ * ends); meshes tiled into more tiles are rejected with DGH_ERR_ARG.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other threads) race. */
+#define DGH_NOALLOC_MAX_TILES 16384
+int dgh_matvec_dot_noalloc(dgh_mesh m, const double* x_dev, double* y_dev, double* xy_dev, void* stream);
 /* d = diag(A).  Replaces Jacobian1MatGetDiagonal (odaJac.C:160-223) and
  * Jacobian2MatGetDiagonal (omgJac.C:1594-1662). */
 int dgh_diagonal(dgh_mesh m, int prec, void* d_dev, void* stream);

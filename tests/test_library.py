@@ -63,3 +63,9 @@ def test_point_generators_deterministic_and_in_range():
     assert abs(g.mean() - 0.5) < 2e-3 and abs(g.std() - 0.1) < 2e-3
     ln = da.points(da.PTS_LOGNORMAL, 10000, 2)
     assert ln.min() == 0.0 and ln.max() == 1.0
+
+
+def test_noalloc_limit_matches_header():
+    txt = open(os.path.join(ROOT, "include", "dendro_hip.h")).read()
+    m = re.search(r"#define DGH_NOALLOC_MAX_TILES (\d+)", txt)
+    assert m and int(m.group(1)) == core.NOALLOC_MAX_TILES
