@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PCG s to 1e-8 rel resid; MatVec GDoF/s + % HBM peak; vs plain FP64 GPU"
 HBM_PEAK_GBS = 8000.0
+AUTOTUNE_MAX_ELEMS = 8_000_000
 
 WORKLOADS = {
     "c3": dict(name="mixed8M_d14", kind="MIXED", npts=8000000, seed=3, cluster_seed=31, depth=14, nlevels=8,
@@ -145,6 +146,58 @@ def cpu_baseline(P, reps=5):
             "sample": f"oracle MatVec (C restatement of the reference element loop, -O3) over the full finest "
                       f"mesh (E={F.n_elem}, N={F.n_node}), median of {reps} calls after a warm-up, on 1 thread "
                       f"({out['1']:.2f} s per call) and on {threads} threads ({out['all']:.3f} s per call)"}
+
+
+def cpu_table(reps=5):
+    """The CPU baseline table of the CPU-runnable workloads, timed by the oracle on this host:
+    the 100k Gaussian octree (max depth 8, Helmholtz): 100 MatVecs, and 3-level PCG to 1e-8
+    (omega 2/3, 2+2 sweeps, b ~ U[-1,1] seed 12); the 1M log-normal octree (max depth 12,
+    kappa = 10^U(-1,1), zero mass, null space): one MatVec, and 5-level PCG to 1e-10
+    (b seed 22).  One thread and all host threads (at most 16); median of `reps` runs after one
+    warm-up (PCG: median of 3).  Prints one JSON line."""
+    import dendro_amd as da
+    import oracle
+    threads = min(16, len(os.sched_getaffinity(0)))
+
+    def med(fn, n):
+        fn()
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2], r
+
+    rows = {"cpu_model": cpu_model(), "threads_all": threads}
+    for name, kind, npts, seed, depth, nlev, coef, rtol, rhs_seed in (
+            ("c1", da.PTS_GAUSS, 100000, 1, 8, 3, None, 1e-8, 12),
+            ("c2", da.PTS_LOGNORMAL, 1000000, 2, 12, 5, "lognormal", 1e-10, 22)):
+        damg = da.DAMG(da.Octree.from_points(da.points(kind, npts, seed), depth), nlev, 500)
+        levels = [damg.level(l) for l in range(damg.nlevels)]
+        F = levels[-1]
+        if coef:
+            kf = 10.0 ** np.random.default_rng(21).uniform(-1, 1, F.n_elem)
+            co = [(damg.coarsen_coefficients(kf, None, l)[0], None, 1.0, 1.0) for l in range(len(levels))]
+        else:
+            co = [(None, None, 1.0, 1.0)] * len(levels)
+        ns = coef is not None
+        x = np.random.default_rng(11).uniform(-1, 1, F.n_node)
+        nmv = 100 if name == "c1" else 1
+        r = {"n_elem": F.n_elem, "n_node": F.n_node}
+        for tag, fn in (("1", lambda: [oracle.matvec(F, x, kappa=co[-1][0]) for _ in range(nmv)]),
+                        ("all", lambda: [oracle.matvec_mt(F, x, kappa=co[-1][0], threads=threads) for _ in range(nmv)])):
+            t, _ = med(fn, reps)
+            r[f"matvec_x{nmv}_s_{tag}"] = round(t, 4)
+            log(f"{name} {nmv} MatVec(s), {tag} thread(s): {t:.3f} s")
+        b = np.random.default_rng(rhs_seed).uniform(-1, 1, F.n_node)
+        for tag, nt in (("1", 1), ("all", threads)):
+            mg = oracle.MG([(L, *co[l]) for l, L in enumerate(levels)], null_space=ns, threads=nt)
+            t, res = med(lambda: mg.pcg(b, rtol=rtol, max_it=500, norm_type=0), 3)
+            r[f"pcg_s_{tag}"] = round(t, 3)
+            r["pcg_iterations"] = int(res[1])
+            log(f"{name} PCG to {rtol:g}, {tag} thread(s) for the MatVecs: {t:.2f} s, {res[1]} its")
+        rows[name] = r
+    print(json.dumps({"cpu_table": rows}), flush=True)
 
 
 def time_matvec(mesh, x, y, steps, warmup, stream, prec=None):
@@ -299,10 +352,20 @@ def matvec_section(P, args, dev, stream):
         mf.set_tile(args.tile, args.block)
         tuned = {"tile_elems": args.tile, "block": args.block or 512}
     else:
+        # The autotuner rebuilds the host tiling once per candidate: seconds per candidate on
+        # meshes of tens of millions of elements.  Above AUTOTUNE_MAX_ELEMS (unless
+        # --autotune-all) a level takes the tiling the autotuner chose for the 44.3M-element
+        # finest level of the 8M mixed octree (4096 elements x 512 threads, 741 us per FP64
+        # MatVec in its search).
         t0 = time.time()
-        for m in P.meshes:
-            tuned = m.autotune(da.F64)
-        log(f"autotuned tilings in {time.time() - t0:.1f}s: finest {tuned}")
+        for l, m in enumerate(P.meshes):
+            if P.levels[l].n_elem > AUTOTUNE_MAX_ELEMS and not args.autotune_all:
+                m.set_tile(4096, 512)
+                tuned = {"tile_elems": 4096, "block": 512, "fixed": True}
+            else:
+                tuned = m.autotune(da.F64)
+            log(f"  level {l} (E={P.levels[l].n_elem}): {tuned}  [{time.time() - t0:.1f}s]")
+        log(f"tilings chosen in {time.time() - t0:.1f}s: finest {tuned}")
     ti = mf.tile_info()
     log(f"tiling: {ti}")
     x = torch.tensor(np.random.default_rng(11).uniform(-1, 1, F.n_node), dtype=torch.float64, device=dev)
@@ -359,8 +422,12 @@ def main():
                     help="only the headline MatVec (for profiler runs): implies --no-cpu --no-pcg --no-c2")
     ap.add_argument("--tile", type=int, default=0, help="operator tile size (0: autotune)")
     ap.add_argument("--block", type=int, default=0, help="workgroup size with --tile (0: library default)")
+    ap.add_argument("--autotune-all", action="store_true",
+                    help=f"autotune levels above {AUTOTUNE_MAX_ELEMS} elements too (minutes of host time)")
     ap.add_argument("--pcg-trace", action="store_true",
                     help="only three eager (no hipGraph) mixed-precision PCG solves, for a kernel trace")
+    ap.add_argument("--cpu-table", action="store_true",
+                    help="only the CPU baseline table of the 100k Gaussian and 1M log-normal workloads (no GPU)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -370,6 +437,9 @@ def main():
         sys.exit(2)
     if args.matvec_only:
         args.no_cpu = args.no_pcg = args.no_c2 = True
+    if args.cpu_table:
+        cpu_table()
+        return
 
     import torch
     import dendro_amd as da
@@ -416,6 +486,7 @@ def main():
         "matvec_fp32_gdofs": mv["matvec_fp32_gdofs"],
     }
     if not args.no_pcg:
+        log("PCG on the headline octree ...")
         out.update(bench_pcg(P, dev, learned=not args.no_learned))
 
     if not args.no_c2 and args.workload != "c2":
@@ -451,6 +522,7 @@ def main():
         del Q
 
     if not args.no_cpu:
+        log("CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(P)
     print(json.dumps(out), flush=True)
 

@@ -21,7 +21,8 @@ EXAMPLES = ["examples/tst_mg.cpp"]
 
 HOST_SOURCES = ["host/stencils.cpp", "host/octree.cpp", "host/da.cpp", "host/tiles.cpp",
                 "host/host_api.cpp"]
-HIP_SOURCES = ["device/device_api.hip", "device/multi_api.hip", "device/learn_api.hip"]
+HIP_SOURCES = ["device/device_api.hip", "device/multi_api.hip", "device/learn_api.hip", "device/fem_api.hip",
+               "device/elas_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed-FP32 instructions are disabled for every HIP translation unit: some of their forms

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libdendrohip.so")
 F64, F32 = 0, 1
 NORM_UNPRECONDITIONED, NORM_PRECONDITIONED = 0, 1
 SMOOTHER_JACOBI, SMOOTHER_LEARNED = 0, 1
+COARSE_JAC2, COARSE_JAC3 = 0, 1
+FN_GAUSSIANS, FN_COS_SUM, FN_POLY = 0, 1, 2
 PTS_GAUSS, PTS_LOGNORMAL, PTS_MIXED, PTS_CLUSTERS, PTS_UNIFORM = range(5)
 NOALLOC_MAX_TILES = 16384     # DGH_NOALLOC_MAX_TILES
 
@@ -26,7 +28,7 @@ EXPORTS = [
     "dgh_octree_size", "dgh_octree_get", "dgh_octree_free",
     "dgh_damg_build", "dgh_damg_num_levels", "dgh_damg_level_info", "dgh_damg_level_arrays",
     "dgh_damg_level_octants", "dgh_damg_level_boundary", "dgh_damg_transfer_arrays",
-    "dgh_damg_coarsen_coefficients", "dgh_damg_free",
+    "dgh_damg_coarsen_coefficients", "dgh_damg_save", "dgh_damg_load", "dgh_damg_free",
     "dgh_init", "dgh_finalize", "dgh_alloc", "dgh_free", "dgh_copy_to_device", "dgh_copy_to_host",
     "dgh_synchronize",
     "dgh_mesh_create", "dgh_mesh_set_coefficients", "dgh_mesh_set_boundary", "dgh_matvec_dirichlet",
@@ -35,7 +37,9 @@ EXPORTS = [
     "dgh_mesh_lane_stencils",
     "dgh_matvec", "dgh_matvec_dot", "dgh_matvec_dot_noalloc", "dgh_diagonal", "dgh_matvec_multi", "dgh_mesh_set_tile_multi",
     "dgh_mesh_autotune_multi",
-    "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add",
+    "dgh_mesh_set_geometry", "dgh_rhs_quadrature", "dgh_nodal_values", "dgh_l2_error",
+    "dgh_mesh_set_elasticity", "dgh_matvec_elasticity",
+    "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add", "dgh_matvec_jac3",
     "dgh_solver_default_params", "dgh_solver_create", "dgh_solver_destroy", "dgh_solver_solve",
     "dgh_solver_solve_multi", "dgh_solver_vcycle",
     "dgh_learn_default_params", "dgh_learned_init", "dgh_learned_get_params", "dgh_learned_set_params",
@@ -58,7 +62,21 @@ class SolverParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("omega", C.c_double), ("max_it", C.c_int),
                 ("nu_pre", C.c_int), ("nu_post", C.c_int), ("norm_type", C.c_int),
                 ("null_space", C.c_int), ("vcycle_prec", C.c_int), ("smoother", C.c_int),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("coarse_op", C.c_int)]
+
+
+class Function(C.Structure):
+    """dgh_function: FN_GAUSSIANS (ComputeRHS2's source), FN_COS_SUM
+    sum_{k<=nfreq} (a0 + 3 a1 k^2 pi^2) cos(k pi x) cos(k pi y) cos(k pi z), FN_POLY."""
+    _fields_ = [("kind", C.c_int), ("nfreq", C.c_int), ("a0", C.c_double), ("a1", C.c_double),
+                ("poly", C.c_double * 8)]
+
+
+def function(kind, nfreq=9, a0=1.0, a1=0.0, poly=None):
+    f = Function(kind, nfreq, a0, a1)
+    for i, v in enumerate(poly or []):
+        f.poly[i] = v
+    return f
 
 
 class SolveStats(C.Structure):
@@ -223,6 +241,22 @@ class DAMG:
         self.nlevels = nl.value
         self._levels = {}
 
+    @classmethod
+    def load(cls, path):
+        """A hierarchy saved by save() (versioned binary, checked on load)."""
+        h = C.c_void_p()
+        check(lib().dgh_damg_load(os.fsencode(path), C.byref(h)))
+        self = cls.__new__(cls)
+        self.h = h
+        n = C.c_int()
+        check(lib().dgh_damg_num_levels(h, C.byref(n)))
+        self.nlevels = n.value
+        self._levels = {}
+        return self
+
+    def save(self, path):
+        check(lib().dgh_damg_save(self.h, os.fsencode(path)))
+
     def info(self, l):
         li = LevelInfo()
         check(lib().dgh_damg_level_info(self.h, int(l), C.byref(li)))
@@ -333,6 +367,29 @@ class Mesh:
     def matvec(self, x, y, prec=F64, stream=None):
         check(lib().dgh_matvec(self.h, int(prec), ptr(x), ptr(y), _stream(stream)))
 
+    def set_elasticity(self, mu, lam):
+        check(lib().dgh_mesh_set_elasticity(self.h, C.c_double(mu), C.c_double(lam)))
+
+    def matvec_elasticity(self, x, y, stream=None):
+        """y = A x for the elasticity operator, x and y [N][3] FP64"""
+        check(lib().dgh_matvec_elasticity(self.h, ptr(x), ptr(y), _stream(stream)))
+
+    def set_geometry(self, anchor):
+        """element anchors [E][3] (DA coordinates) for the function-evaluating calls"""
+        a = np.ascontiguousarray(anchor, np.uint32)
+        check(lib().dgh_mesh_set_geometry(self.h, ptr(a)))
+
+    def rhs_quadrature(self, f, b, stream=None):
+        check(lib().dgh_rhs_quadrature(self.h, C.byref(f), ptr(b), _stream(stream)))
+
+    def nodal_values(self, f, u, stream=None):
+        check(lib().dgh_nodal_values(self.h, C.byref(f), ptr(u), _stream(stream)))
+
+    def l2_error(self, f, u):
+        e = C.c_double()
+        check(lib().dgh_l2_error(self.h, C.byref(f), ptr(u), C.byref(e)))
+        return e.value
+
     def matvec_dot(self, x, y, xy, stream=None):
         """y = A x and xy[0] = x.y (FP64; xy a one-element device tensor)."""
         check(lib().dgh_matvec_dot(self.h, ptr(x), ptr(y), ptr(xy), _stream(stream)))
@@ -384,6 +441,10 @@ class Transfer:
 
     def restrict(self, f, c, prec=F64, stream=None):
         check(lib().dgh_restrict(self.h, int(prec), ptr(f), ptr(c), _stream(stream)))
+
+    def matvec_jac3(self, x, y, prec=F64, stream=None):
+        """Jacobian3 on the coarse level: y = P^T A_fine P x (fine mesh coefficients)."""
+        check(lib().dgh_matvec_jac3(self.h, int(prec), ptr(x), ptr(y), _stream(stream)))
 
     def prolong_add(self, c, f_in, f_out, prec=F64, stream=None):
         check(lib().dgh_prolong_add(self.h, int(prec), ptr(c), ptr(f_in), ptr(f_out), _stream(stream)))

@@ -110,17 +110,28 @@ static void launch_smooth(dgh_mesh m, const T* x, const T* b, const T* invd, T o
 }
 
 // c = R f; with x1 given, also x1 = omega (c * invd) (the first sweep on the coarse level)
+// Boundary flags of a transfer's two meshes when both carry them (Dirichlet problem): the
+// transfers then drop every entry that touches a boundary node (the suppressed DOFs of
+// RestrictMatVec.C:75-80 and ProlongMatVec.C:76-80); nullptrs otherwise.
+static inline const uint8_t* tbdy_c(dgh_transfer t) {
+  return (t->coarse->bdy.p && t->fine->bdy.p) ? t->coarse->bdy.p : nullptr;
+}
+static inline const uint8_t* tbdy_f(dgh_transfer t) {
+  return (t->coarse->bdy.p && t->fine->bdy.p) ? t->fine->bdy.p : nullptr;
+}
+
 template <class T>
 static void launch_restrict(dgh_transfer t, const T* f, T* c, hipStream_t st, T omega = T(0),
                             const T* invd = nullptr, T* x1 = nullptr) {
   const uint32_t nc = t->coarse->N;
   if (!nc) return;
+  const uint8_t *bc = tbdy_c(t), *bf = tbdy_f(t);
   if (t->ent.p)
     hipLaunchKernelGGL((restrict_kernel<T, true>), dim3(blocks_for(size_t(nc) * RLANES)), dim3(VBLOCK), 0, st, nc,
-                       t->rowptr.p, t->ent.p, (const float*)nullptr, f, c, omega, invd, x1);
+                       t->rowptr.p, t->ent.p, (const float*)nullptr, f, c, omega, invd, x1, bc, bf);
   else
     hipLaunchKernelGGL((restrict_kernel<T, false>), dim3(blocks_for(size_t(nc) * RLANES)), dim3(VBLOCK), 0, st, nc,
-                       t->rowptr.p, t->col.p, t->w.p, f, c, omega, invd, x1);
+                       t->rowptr.p, t->col.p, t->w.p, f, c, omega, invd, x1, bc, bf);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -129,8 +140,23 @@ static void launch_prolong(dgh_transfer t, const T* c, const T* f_in, T* f_out, 
   const uint32_t nf = t->fine->N;
   if (!nf) return;
   hipLaunchKernelGGL(prolong_kernel<T>, dim3(blocks_for(nf)), dim3(VBLOCK), 0, st, nf, t->celem.p,
-                     t->pos.p, t->coarse->nl.p, t->coarse->cs.p, Wtab<T>(), c, f_in, f_out);
+                     t->pos.p, t->coarse->nl.p, t->coarse->cs.p, Wtab<T>(), c, f_in, f_out, tbdy_c(t), tbdy_f(t));
   HIP_CHECK(hipGetLastError());
+}
+
+// y = A_D x, the Dirichlet-masked operator (one launch)
+template <class T>
+static void launch_dirmv(dgh_mesh m, const T* x, T* y, hipStream_t st) {
+  launch_op<T, OP_DIRMV>(m, OpIO<T>{x, y, nullptr, nullptr, T(0), nullptr, m->bdy.p}, st);
+}
+
+// Jacobian3 on the coarse level of t: y = R (A_f (P x)) with the fine mesh's own coefficients,
+// vf and wf fine-level scratch vectors.
+template <class T>
+static void launch_jac3(dgh_transfer t, const T* x, T* y, T* vf, T* wf, hipStream_t st) {
+  launch_prolong<T>(t, x, nullptr, vf, st);
+  launch_matvec<T>(t->fine, vf, wf, st);
+  launch_restrict<T>(t, wf, y, st);
 }
 
 // Per-record element factors of a tiling (they follow its record order), and the per-tile
@@ -297,14 +323,40 @@ static T* vcycle(dgh_solver s, int l, const T* b, hipStream_t st, bool x1_ready 
   const T om = T(s->om_level[l]);
   T* xc = V.xa.p;
   T* xn = V.xb.p;
+  // Jacobian3 below the finest level: the level's operator is R A_{l+1} P, applied into V.r
+  // with the next finer level's r and t as scratch (its r is free once restricted); the
+  // Jacobi diagonal stays the level's own
+  const bool j3 = s->prm.coarse_op == DGH_COARSE_JAC3 && l < s->nlevels - 1;
+  // Dirichlet problem: the masked operator (one launch into V.r), then the sweep or residual
+  auto apply = [&](const T* xi) {
+    if (j3)
+      launch_jac3<T>(s->transfers[l], xi, V.r.p, VV[l + 1].r.p, VV[l + 1].t.p, st);
+    else
+      launch_dirmv<T>(m, xi, V.r.p, st);
+  };
+  const bool split = j3 || s->dirichlet;
+  auto sweep = [&](const T* xi, T* xo) {
+    if (!split) {
+      launch_smooth<T>(m, xi, b, V.invd.p, om, xo, st);
+      return;
+    }
+    apply(xi);
+    hipLaunchKernelGGL(smooth_vec_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, xi, b, (const T*)V.r.p,
+                       (const T*)V.invd.p, om, xo);
+  };
   if (!x1_ready)
     hipLaunchKernelGGL(smooth_first_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, om, b,
                        (const T*)V.invd.p, xc);
   for (int it = 1; it < s->prm.nu_pre; ++it) {
-    launch_smooth<T>(m, xc, b, V.invd.p, om, xn, st);
+    sweep(xc, xn);
     std::swap(xc, xn);
   }
-  launch_resid<T>(m, xc, b, V.r.p, st);
+  if (split) {
+    apply(xc);
+    hipLaunchKernelGGL(resid_vec_kernel<T>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, b, (const T*)V.r.p, V.r.p);
+  } else {
+    launch_resid<T>(m, xc, b, V.r.p, st);
+  }
   auto& C = VV[l - 1];
   if (l - 1 > 0 && s->prm.nu_pre > 0)
     launch_restrict<T>(s->transfers[l - 1], V.r.p, C.b.p, st, T(s->om_level[l - 1]), C.invd.p, C.xa.p);
@@ -313,7 +365,7 @@ static T* vcycle(dgh_solver s, int l, const T* b, hipStream_t st, bool x1_ready 
   T* xcoarse = vcycle<T>(s, l - 1, C.b.p, st, l - 1 > 0 && s->prm.nu_pre > 0);
   launch_prolong<T>(s->transfers[l - 1], xcoarse, xc, xc, st);
   for (int it = 0; it < s->prm.nu_post; ++it) {
-    launch_smooth<T>(m, xc, b, V.invd.p, om, xn, st);
+    sweep(xc, xn);
     std::swap(xc, xn);
   }
   return xc;
@@ -401,9 +453,19 @@ static void cg_iteration(dgh_solver s, hipStream_t st) {
   else
     hipLaunchKernelGGL(cg_p_fused_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
                        static_cast<const double*>(s->zres), s->p.p);
-  launch_cg(m, s->p.p, s->w.p, s->ptile.p, st);
-  // alpha from the tile partials is formed inside the update kernel
-  const int nt = int(m->op.plan.tiles.size());
+  // alpha from the partials of p.w is formed inside the update kernel: the operator's tile
+  // partials, or (Dirichlet problem) the block partials of a separate dot pass
+  int nt = int(m->op.plan.tiles.size());
+  if (s->dirichlet) {
+    launch_dirmv<double>(m, s->p.p, s->w.p, st);
+    DotArgs<double> q{};
+    q.a[0] = s->p.p;
+    q.b[0] = s->w.p;
+    hipLaunchKernelGGL(reduce1_kernel<double>, dim3(RED_BLOCKS), dim3(VBLOCK), 0, st, n, q, s->ptile.p);
+    nt = RED_BLOCKS;
+  } else {
+    launch_cg(m, s->p.p, s->w.p, s->ptile.p, st);
+  }
   const bool x1 = s->nlevels > 1 && s->prm.nu_pre > 0;   // first sweep of the V-cycle fused
   if (mixed)
     hipLaunchKernelGGL(cg_xr_fused_kernel<float>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, nt,
@@ -427,6 +489,55 @@ static void cg_iteration(dgh_solver s, hipStream_t st) {
                      int(s->prm.null_space), S, int(pnorm), int(mixed));
 }
 
+// Jacobi diagonal of level l (the Dirichlet one, 1 on boundary nodes, for the Dirichlet problem)
+static void solver_diag(dgh_solver s, int l, double* d) {
+  dgh_mesh m = s->meshes[l];
+  launch_op<double, OP_DIAG>(m, OpIO<double>{nullptr, d, nullptr, nullptr, 0.0, nullptr,
+                                             s->dirichlet ? m->bdy.p : nullptr}, s->st);
+}
+
+// Jacobian3 coarsest matrix (BUILD_FULL_JAC_TYPE3_BLOCK, examples/src/backend/omgJac.C:2341-2448)
+// as the one-level Galerkin product P^T A_f P, to which its Type2/Type1 element blocks sum:
+// every element e of the fine mesh adds (S_e P)^T K_e (S_e P), where row k of S_e P holds the
+// coarse shape-function weights at the node of vertex k (the prolongation's own weights).
+static void assemble_jac3_coarse(dgh_transfer t, const double* L, const double* M, double* A, int ld) {
+  dgh_mesh cm = t->coarse, fm = t->fine;
+  std::vector<double> W(144 * 27 * 8);
+  generate_transfer_weights(W.data());
+  const bool per = !fm->h_kappa.empty();
+  for (uint32_t e = 0; e < fm->E; ++e) {
+    const double h = fm->hFac * double(1u << (fm->maxD - fm->h_lev[e]));
+    const double k1 = per ? fm->h_kappa[e] : fm->lap_fac;
+    const double k2 = per ? (fm->h_mass.empty() ? 0.0 : fm->h_mass[e]) : fm->mass_fac;
+    const double fac1 = k1 * h / 2.0, fac2 = k2 * h * h * h / 8.0;
+    const double* Ls = L + size_t(fm->h_cs[e]) * 64;
+    const double* Ms = M + size_t(fm->h_cs[e]) * 64;
+    uint32_t cn[8][8];
+    double cw[8][8];
+    int cnt[8];
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t nf = fm->h_nl[size_t(e) * 8 + k];
+      const uint32_t ce = t->h_celem[nf];
+      const double* w = W.data() + (size_t(cm->h_cs[ce]) * 27 + t->h_pos[nf]) * 8;
+      cnt[k] = 0;
+      for (int q = 0; q < 8; ++q)
+        if (w[q] != 0.0) {
+          cn[k][cnt[k]] = cm->h_nl[size_t(ce) * 8 + q];
+          cw[k][cnt[k]++] = w[q];
+        }
+    }
+    for (int k = 0; k < 8; ++k)
+      for (int j = 0; j < 8; ++j) {
+        const double kv = fac1 * Ls[k * 8 + j] + fac2 * Ms[k * 8 + j];
+        for (int a = 0; a < cnt[k]; ++a) {
+          double* row = A + size_t(cn[k][a]) * ld;
+          const double ka = kv * cw[k][a];
+          for (int b = 0; b < cnt[j]; ++b) row[cn[j][b]] += ka * cw[j][b];
+        }
+      }
+  }
+}
+
 // Assembles the coarsest operator from its element blocks and inverts it on the device
 // (Cholesky, rocSOLVER).  With a constant null space, A + a 11^T (a = trace / n^2).
 static void setup_coarse(dgh_solver s) {
@@ -439,6 +550,9 @@ static void setup_coarse(dgh_solver s) {
   const int ld = (n + DENSE_PAD - 1) / DENSE_PAD * DENSE_PAD;
   s->ldc = ld;
   std::vector<double> A(size_t(n) * ld, 0.0);
+  if (s->prm.coarse_op == DGH_COARSE_JAC3 && s->nlevels > 1) {
+    assemble_jac3_coarse(s->transfers[0], L.data(), M.data(), A.data(), ld);
+  } else {
   const double hFac = m->hFac;
   for (uint32_t e = 0; e < m->E; ++e) {
     const double h = hFac * double(1u << (m->maxD - m->h_lev[e]));
@@ -450,6 +564,13 @@ static void setup_coarse(dgh_solver s) {
     const uint32_t* idx = m->h_nl.data() + size_t(e) * 8;
     for (int k = 0; k < 8; ++k)
       for (int j = 0; j < 8; ++j) A[size_t(idx[k]) * ld + idx[j]] += fac1 * Ls[k * 8 + j] + fac2 * Ms[k * 8 + j];
+  }
+  }
+  if (s->dirichlet) {   // ComputeDirichletJacobian (omgJac.C:382-506): identity rows and columns
+    const uint8_t* bd = m->h_bdy.data();
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        if (bd[i] || bd[j]) A[size_t(i) * ld + j] = (i == j) ? 1.0 : 0.0;
   }
   if (s->prm.null_space) {
     double tr = 0.0;
@@ -496,6 +617,39 @@ static void setup_coarse(dgh_solver s) {
     HIP_CHECK(hipGetLastError());
   }
   HIP_CHECK(hipDeviceSynchronize());
+}
+
+// After dgh_mesh_set_coefficients on a solver's mesh: the level's Jacobi diagonal (and, for
+// the coarsest level or the level a Jacobian3 coarsest matrix is built from, the coarsest
+// inverse) are rebuilt, and learned weights of that level give way to plain Jacobi.  Boundary
+// flags cannot change under a solver: the Dirichlet mode and the masked coarsest matrix were
+// fixed at dgh_solver_create.
+void refresh_operators(dgh_solver s) {
+  bool coarse = false;
+  for (int l = 0; l < s->nlevels; ++l) {
+    dgh_mesh m = s->meshes[l];
+    if (m->bdy_gen != s->bdy_gen[l])
+      fail(DGH_ERR_STATE, "boundary flags of level %d changed after dgh_solver_create", l);
+    if (m->coef_gen == s->coef_gen[l]) continue;
+    s->coef_gen[l] = m->coef_gen;
+    const uint32_t n = m->N;
+    auto& V = s->v64[l];
+    solver_diag(s, l, V.r.p);
+    hipLaunchKernelGGL(reciprocal_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, s->st, n,
+                       (const double*)V.r.p, V.invd.p);
+    if (!s->v32.empty())
+      hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n)), dim3(VBLOCK), 0, s->st, n,
+                         (const double*)V.invd.p, s->v32[l].invd.p);
+    HIP_CHECK(hipGetLastError());
+    s->om_level[l] = s->prm.omega;
+    if (s->learn && l < int(s->learn->lv.size())) {
+      s->learn->lv[l].ready = false;
+      if (l < int(s->learn->on.size())) s->learn->on[l] = 0;
+    }
+    if (l == 0 || (l == 1 && s->prm.coarse_op == DGH_COARSE_JAC3)) coarse = true;
+  }
+  HIP_CHECK(hipStreamSynchronize(s->st));
+  if (coarse) setup_coarse(s);
 }
 
 }  // namespace dgh
@@ -758,6 +912,7 @@ int dgh_mesh_set_coefficients(dgh_mesh m, const double* kappa, const double* mas
     if (mass) m->h_mass.assign(mass, mass + m->E);
     upload_coefs(m, m->op);
     if (!m->mop.plan.tiles.empty()) upload_coefs(m, m->mop);
+    ++m->coef_gen;
   });
 }
 
@@ -842,9 +997,13 @@ int dgh_mesh_set_boundary(dgh_mesh m, const uint8_t* bdy) {
   return guarded([&] {
     require_init();
     if (!m) fail(DGH_ERR_ARG, "NULL mesh");
+    ++m->bdy_gen;
+    ++m->gen;   // captured graphs hold the flags' buffer
     m->bdy.release();
+    m->h_bdy.clear();
     if (!bdy || !m->N) return;
     m->bdy.upload(bdy, m->N);
+    m->h_bdy.assign(bdy, bdy + m->N);
   });
 }
 
@@ -905,6 +1064,8 @@ int dgh_transfer_create(dgh_mesh coarse, dgh_mesh fine, const uint32_t* coarse_e
     }
     t->celem.upload(coarse_elem, nf);
     t->pos.upload(pos, nf);
+    t->h_celem.assign(coarse_elem, coarse_elem + nf);
+    t->h_pos.assign(pos, pos + nf);
     // R = P^T with P[j][cnl[k]] = W[s][pos_j][k]: rows are coarse nodes, columns fine nodes
     std::vector<double> W(144 * 27 * 8);
     generate_transfer_weights(W.data());
@@ -965,6 +1126,31 @@ int dgh_restrict(dgh_transfer t, int prec, const void* f, void* c, void* stream)
   });
 }
 
+int dgh_matvec_jac3(dgh_transfer t, int prec, const void* x, void* y, void* stream) {
+  return guarded([&] {
+    require_init();
+    if (!t || !x || !y) fail(DGH_ERR_ARG, "NULL argument");
+    if (x == y) fail(DGH_ERR_ARG, "x and y must not alias");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t nf = t->fine->N;
+    if (prec == DGH_F64) {
+      if (t->j3v64.n < nf) {
+        t->j3v64.alloc(nf);
+        t->j3w64.alloc(nf);
+      }
+      launch_jac3<double>(t, static_cast<const double*>(x), static_cast<double*>(y), t->j3v64.p, t->j3w64.p, st);
+    } else if (prec == DGH_F32) {
+      if (t->j3v32.n < nf) {
+        t->j3v32.alloc(nf);
+        t->j3w32.alloc(nf);
+      }
+      launch_jac3<float>(t, static_cast<const float*>(x), static_cast<float*>(y), t->j3v32.p, t->j3w32.p, st);
+    } else {
+      fail(DGH_ERR_ARG, "unknown precision %d", prec);
+    }
+  });
+}
+
 int dgh_prolong_add(dgh_transfer t, int prec, const void* c, const void* f_in, void* f_out, void* stream) {
   return guarded([&] {
     require_init();
@@ -994,6 +1180,7 @@ int dgh_solver_default_params(dgh_solver_params* p) {
   p->vcycle_prec = DGH_F64;
   p->smoother = DGH_SMOOTHER_JACOBI;
   p->use_graph = 0;
+  p->coarse_op = DGH_COARSE_JAC2;
   return DGH_OK;
 }
 
@@ -1008,6 +1195,10 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
     if (params->vcycle_prec != DGH_F64 && params->vcycle_prec != DGH_F32) fail(DGH_ERR_ARG, "bad vcycle_prec");
     if (params->smoother != DGH_SMOOTHER_JACOBI && params->smoother != DGH_SMOOTHER_LEARNED)
       fail(DGH_ERR_ARG, "unknown smoother %d", params->smoother);
+    if (params->coarse_op != DGH_COARSE_JAC2 && params->coarse_op != DGH_COARSE_JAC3)
+      fail(DGH_ERR_ARG, "unknown coarse operator %d", params->coarse_op);
+    if (params->coarse_op == DGH_COARSE_JAC3 && params->smoother == DGH_SMOOTHER_LEARNED)
+      fail(DGH_ERR_ARG, "the learned smoother is trained on Jacobian2 levels; use DGH_COARSE_JAC2");
     auto s = std::make_unique<dgh_solver_s>();
     s->nlevels = nlevels;
     s->prm = *params;
@@ -1018,6 +1209,20 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
       if (!meshes[l]) fail(DGH_ERR_ARG, "NULL mesh at level %d", l);
       if (l > 0 && (s->transfers[l - 1]->coarse != meshes[l - 1] || s->transfers[l - 1]->fine != meshes[l]))
         fail(DGH_ERR_ARG, "transfer %d does not connect levels %d and %d", l - 1, l - 1, l);
+    }
+    // boundary flags on every level: the Dirichlet problem (SetDirichletJacContexts,
+    // examples/src/backend/omgJac.C:508-586); on some levels only: an error
+    int nb = 0;
+    for (int l = 0; l < nlevels; ++l) nb += (meshes[l]->N == 0 || meshes[l]->bdy.p) ? 1 : 0;
+    if (nb != 0 && nb != nlevels)
+      fail(DGH_ERR_STATE, "boundary flags are set on %d of %d levels (all or none)", nb, nlevels);
+    s->dirichlet = (nb == nlevels) && meshes[nlevels - 1]->bdy.p != nullptr;
+    if (s->dirichlet && (params->null_space || params->smoother == DGH_SMOOTHER_LEARNED ||
+                         params->coarse_op != DGH_COARSE_JAC2))
+      fail(DGH_ERR_ARG, "the Dirichlet solve takes no null space, learned smoother or Jacobian3 levels");
+    for (int l = 0; l < nlevels; ++l) {
+      s->coef_gen.push_back(meshes[l]->coef_gen);
+      s->bdy_gen.push_back(meshes[l]->bdy_gen);
     }
     HIP_CHECK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s->hS), sizeof(double) * S_NSCAL, hipHostMallocDefault));
@@ -1035,7 +1240,9 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
       V.xa.alloc(n);
       V.xb.alloc(n);
       V.r.alloc(n);
-      launch_diag<double>(meshes[l], V.r.p, s->st);
+      const bool j3fine = params->coarse_op == DGH_COARSE_JAC3 && l >= 2;   // fine level of a Jacobian3 level
+      if (j3fine) V.t.alloc(n);
+      solver_diag(s.get(), l, V.r.p);
       hipLaunchKernelGGL(reciprocal_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, s->st, n,
                          (const double*)V.r.p, V.invd.p);
       if (params->vcycle_prec == DGH_F32) {
@@ -1045,6 +1252,7 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
         F.xa.alloc(n);
         F.xb.alloc(n);
         F.r.alloc(n);
+        if (j3fine) F.t.alloc(n);
         hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n)), dim3(VBLOCK), 0, s->st, n,
                            (const double*)V.invd.p, F.invd.p);
       }
@@ -1070,6 +1278,7 @@ int dgh_solver_vcycle(dgh_solver s, const double* r, double* z, void* stream) {
   return guarded([&] {
     require_init();
     if (!s || !r || !z) fail(DGH_ERR_ARG, "NULL argument");
+    refresh_operators(s);
     precondition(s, r, z, static_cast<hipStream_t>(stream));
   });
 }
@@ -1098,7 +1307,8 @@ int dgh_solver_solve(dgh_solver s, const double* b, double* x, dgh_solve_stats* 
     HIP_CHECK(hipStreamWaitEvent(st, eu, 0));
     HIP_CHECK(hipEventRecord(e0, st));
     const bool mixed = (s->prm.vcycle_prec == DGH_F32);
-    const size_t ntiles = m->op.plan.tiles.size();
+    refresh_operators(s);
+    const size_t ntiles = std::max<size_t>(m->op.plan.tiles.size(), size_t(RED_BLOCKS) * MAXQ);
     if (s->ptile.n < ntiles) s->ptile.alloc(ntiles);
     const uint64_t gen = mesh_generation(s);
     if (gen != s->graph_gen) {   // re-tiled or new coefficients since the graphs were captured
@@ -1161,7 +1371,13 @@ int dgh_solver_solve(dgh_solver s, const double* b, double* x, dgh_solve_stats* 
       // true residual ||b - A x|| / ||b|| in FP64 (b projected when constants are the null space)
       hipLaunchKernelGGL(copy_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, b, s->w.p);
       if (s->prm.null_space) project_mean<double>(s, n, s->w.p, st);
-      launch_resid<double>(m, x, s->w.p, s->z.p, st);
+      if (s->dirichlet) {
+        launch_dirmv<double>(m, x, s->z.p, st);
+        hipLaunchKernelGGL(resid_vec_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n,
+                           (const double*)s->w.p, (const double*)s->z.p, s->z.p);
+      } else {
+        launch_resid<double>(m, x, s->w.p, s->z.p, st);
+      }
       DotArgs<double> q{};
       q.a[0] = s->z.p; q.b[0] = s->z.p;
       reduce<double>(n, q, 1, s->part_p.p, S + S_PROJ, st);

@@ -72,6 +72,8 @@ struct State {
   DevBuf<float> K32, W32;
   DevBuf<double> KF64;            // child-0 full stencil rows [18][128] (Laplacian, mass)
   DevBuf<float> KF32;
+  DevBuf<double> SF;              // shape functions at the 3x3x3 Gauss nodes [144][8][27]
+  DevBuf<double> EL2, EGD;        // elasticity: full Laplacian [144][64] and grad-div [144][576]
   double lrow[NSTENCIL] = {}, mrow[NSTENCIL] = {};   // max row sums of |L|, |M| per type
   rocblas_handle blas = nullptr;
   // device address of the file-scope scratch array of dgh_matvec_dot_noalloc, per device id
@@ -115,6 +117,19 @@ struct DevTiles {
 
 }  // namespace dgh
 
+namespace dgh {
+// Elasticity operator plan of a mesh (elas_api.hip)
+struct ElasState {
+  bool ready = false;
+  double mu = 1.0, lam = 1.0;
+  int nseg = 0;
+  DevBuf<uint32_t> order, rowptr, slot;
+  DevBuf<int4> seg;
+  DevBuf<uint8_t> lev;
+  DevBuf<double> ye;            // [E][24] element outputs
+};
+}  // namespace dgh
+
 using namespace dgh;
 
 struct dgh_mesh_s {
@@ -136,8 +151,14 @@ struct dgh_mesh_s {
   uint32_t mtile = 0;           // batched tile size (0: default)
   int mvariant = 0;             // batched FP32 products: 0 vector ALUs, 1 matrix cores
   uint64_t gen = 0;             // bumped whenever device tilings or coefficients are rebuilt
+  uint64_t coef_gen = 0;        // bumped by dgh_mesh_set_coefficients
+  uint64_t bdy_gen = 0;         // bumped by dgh_mesh_set_boundary
   DevBuf<uint8_t> bdy;          // [N] boundary flags of the Dirichlet operators (empty: none)
+  std::vector<uint8_t> h_bdy;   // host copy (Dirichlet coarsest matrix)
   DevBuf<double> dotp;          // per-tile partials of dgh_matvec_dot
+  DevBuf<int4> geo;             // [E] {anchor x, y, z, level}: element points (dgh_mesh_set_geometry)
+  DevBuf<uint8_t> hn;           // [E] hanging masks (nodal values)
+  ElasState elas;               // dgh_mesh_set_elasticity
 };
 
 struct dgh_transfer_s {
@@ -147,6 +168,10 @@ struct dgh_transfer_s {
   DevBuf<uint32_t> rowptr, col; // R = P^T in CSR (rows: coarse nodes)
   DevBuf<float> w;              // exact dyadic weights
   DevBuf<uint32_t> ent;         // packed (32 w - 1) << 27 | col, when every entry fits
+  std::vector<uint32_t> h_celem; // host copies of celem and pos (Jacobian3 coarsest assembly)
+  std::vector<uint8_t> h_pos;
+  DevBuf<double> j3v64, j3w64;  // fine-level scratch of dgh_matvec_jac3
+  DevBuf<float> j3v32, j3w32;
 };
 
 namespace dgh {
@@ -154,6 +179,7 @@ namespace dgh {
 template <class T>
 struct LevelVecs {
   DevBuf<T> invd, b, xa, xb, r;
+  DevBuf<T> t;                  // Jacobian3: scratch of the next coarser level's operator
 };
 
 }  // namespace dgh
@@ -225,6 +251,9 @@ struct dgh_solver_s {
   hipGraph_t mgraph = nullptr;
   hipGraphExec_t mgexec = nullptr;
   std::vector<double> om_level;        // Richardson scale per level (1 where learned weights sit in invd)
+  bool dirichlet = false;              // every level carries boundary flags: the Dirichlet problem
+  std::vector<uint64_t> coef_gen;      // meshes' coef_gen / bdy_gen when the diagonals and the
+  std::vector<uint64_t> bdy_gen;       // coarsest inverse were built
   LearnState* learn = nullptr;
   ~dgh_solver_s() {
     delete learn;
@@ -262,4 +291,5 @@ void prolong_multi_f32(dgh_transfer t, int k, const float* c, const float* f_in,
 void project_mean_multi_f32(dgh_solver s, uint32_t n, int k, float* v, hipStream_t st);
 void column_dots_f32(uint32_t n, int k, const float* a, const float* b, double* part, double* out, hipStream_t st);
 void drop_graphs(dgh_solver s);
+void refresh_operators(dgh_solver s);
 }  // namespace dgh

@@ -505,7 +505,8 @@ __global__ void __launch_bounds__(256)
 prolong_kernel(uint32_t nf, const uint32_t* __restrict__ celem, const uint8_t* __restrict__ pos,
                const uint32_t* __restrict__ cnl, const uint8_t* __restrict__ cs,
                const T* __restrict__ W, const T* __restrict__ c, const T* __restrict__ f_in,
-               T* __restrict__ f_out) {
+               T* __restrict__ f_out, const uint8_t* __restrict__ cbdy = nullptr,
+               const uint8_t* __restrict__ fbdy = nullptr) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nf) return;
   const uint32_t ce = celem[j];
@@ -518,12 +519,17 @@ prolong_kernel(uint32_t nf, const uint32_t* __restrict__ celem, const uint8_t* _
   T cv[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) cv[k] = c[idx[k]];
+  if (cbdy) {   // Dirichlet: no entry from a boundary coarse node
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cv[k] = cbdy[idx[k]] ? T(0) : cv[k];
+  }
   T acc = T(0);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const T wk = w[k];
     acc += (wk != T(0)) ? wk * cv[k] : T(0);
   }
+  if (fbdy && fbdy[j]) acc = T(0);   // Dirichlet: no entry into a boundary fine node
   f_out[j] = (f_in ? f_in[j] : T(0)) + acc;
 }
 
@@ -541,7 +547,8 @@ template <class T, bool PACKED = false>
 __global__ void __launch_bounds__(256)
 restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
                 const float* __restrict__ w, const T* __restrict__ f, T* __restrict__ c, T omega = T(0),
-                const T* __restrict__ invd = nullptr, T* __restrict__ x1 = nullptr) {
+                const T* __restrict__ invd = nullptr, T* __restrict__ x1 = nullptr,
+                const uint8_t* __restrict__ cbdy = nullptr, const uint8_t* __restrict__ fbdy = nullptr) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t i = gid / RLANES, lane = gid % RLANES;
   T acc = T(0);
@@ -561,6 +568,11 @@ restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t
       }
 #pragma unroll
       for (int u = 0; u < RU; ++u) fv[u] = p0 + u * RLANES < p1 ? f[PACKED ? (e[u] & RP_COL) : e[u]] : T(0);
+      if (fbdy) {   // Dirichlet: no entry from a boundary fine node
+#pragma unroll
+        for (int u = 0; u < RU; ++u)
+          if (p0 + u * RLANES < p1 && fbdy[PACKED ? (e[u] & RP_COL) : e[u]]) fv[u] = T(0);
+      }
 #pragma unroll
       for (int u = 0; u < RU; ++u) {
         if (p0 + u * RLANES >= p1) break;
@@ -574,6 +586,7 @@ restrict_kernel(uint32_t nc, const uint32_t* __restrict__ rowptr, const uint32_t
 #pragma unroll
   for (int off = RLANES / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, RLANES);
   if (i < nc && lane == 0) {
+    if (cbdy && cbdy[i]) acc = T(0);   // Dirichlet: no entry into a boundary coarse node
     c[i] = acc;
     if (x1) x1[i] = T(0) + omega * (acc * invd[i]);
   }

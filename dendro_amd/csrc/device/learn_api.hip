@@ -408,6 +408,9 @@ int dgh_learned_train(dgh_solver s, const dgh_learn_params* p, dgh_learn_stats* 
     require_init();
     if (!s || !p) fail(DGH_ERR_ARG, "NULL argument");
     if (s->nlevels < 2) fail(DGH_ERR_STATE, "a single level has no smoother");
+    if (s->prm.coarse_op != DGH_COARSE_JAC2) fail(DGH_ERR_ARG, "learned damping is trained on Jacobian2 levels only");
+    if (s->dirichlet) fail(DGH_ERR_ARG, "learned damping has no Dirichlet mode");
+    refresh_operators(s);
     check_samples(p->samples);
     if (p->steps < 0 || p->nu < 1 || p->nu > 8 || p->refresh < 1 || !(p->lr > 0)) fail(DGH_ERR_ARG, "invalid training parameters");
     LearnState& L = ready_state(s);
@@ -457,6 +460,9 @@ int dgh_learned_apply(dgh_solver s, int enable, int samples, uint64_t seed, int*
   return guarded([&] {
     require_init();
     if (!s) fail(DGH_ERR_ARG, "NULL solver");
+    if (enable && (s->prm.coarse_op != DGH_COARSE_JAC2 || s->dirichlet))
+      fail(DGH_ERR_ARG, "learned damping needs Jacobian2 levels without boundary flags");
+    refresh_operators(s);
     LearnState& L = ready_state(s);
     int mask = 0;
     if (!enable) {

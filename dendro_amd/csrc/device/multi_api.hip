@@ -388,6 +388,9 @@ int dgh_solver_solve_multi(dgh_solver s, int k, const double* B, double* X, dgh_
   return guarded([&] {
     require_init();
     if (!s || !B || !X) fail(DGH_ERR_ARG, "NULL argument");
+    if (s->prm.coarse_op != DGH_COARSE_JAC2) fail(DGH_ERR_ARG, "the batched solve runs Jacobian2 levels only");
+    if (s->dirichlet) fail(DGH_ERR_ARG, "the batched solve has no Dirichlet mode");
+    refresh_operators(s);
     check_k(k);
     ensure_multi_buffers(s, k);
     const uint64_t gen = mesh_generation(s);

@@ -111,6 +111,20 @@ sum_partials_kernel(int n, const double* __restrict__ part, double* __restrict__
   }
 }
 
+// r = b - Ax, and the sweep x_out = x + omega ((b - Ax) invd) written as the operator kernel's
+// store phase writes it, for operators applied by other kernels (Jacobian3 levels)
+template <class T>
+__global__ void resid_vec_kernel(uint32_t n, const T* __restrict__ b, const T* ax, T* r) {   // r may be ax
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = b[i] - ax[i];
+}
+template <class T>
+__global__ void smooth_vec_kernel(uint32_t n, const T* __restrict__ x, const T* __restrict__ b,
+                                  const T* __restrict__ ax, const T* __restrict__ invd, T omega, T* __restrict__ xo) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xo[i] = x[i] + omega * ((b[i] - ax[i]) * invd[i]);
+}
+
 // v = v - sum[0] / n  (constant-null-space projection)
 template <class T>
 __global__ void subtract_mean_kernel(uint32_t n, const double* __restrict__ sum, T* __restrict__ v) {

@@ -1,6 +1,7 @@
 // extern "C" host entry points of libdendrohip: stencil tables, synthetic points, octrees,
 // and the multigrid hierarchy of DAs.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -129,6 +130,61 @@ struct dgh_damg_s {
   std::vector<std::vector<uint32_t>> first_fine;  // [level] coarse element -> first fine element
   std::vector<std::vector<uint8_t>> same;
 };
+
+// ---------------------------------------------------------------- hierarchy blob
+// Versioned binary of a DA hierarchy: header {magic "DGHDAMG\0", version, nlevels, byte order
+// mark}, then per level (coarsest first) the scalars {D, n_elem, n_node, n_hanging} and the
+// arrays octs (Morton anchor u64, level u8, flags u8), node_of, nlist, child, hnmask, lev,
+// and for levels >= 1 the transfer arrays celem, pos, first_fine, same; every array as
+// {u64 count, count elements}; last a 64-bit FNV-1a hash of everything before it.
+namespace {
+constexpr char kBlobMagic[8] = {'D', 'G', 'H', 'D', 'A', 'M', 'G', 0};
+constexpr uint32_t kBlobVersion = 1;
+constexpr uint32_t kBlobBom = 0x01020304u;
+
+struct BlobWriter {
+  std::FILE* f;
+  uint64_t h = 1469598103934665603ull;
+  void raw(const void* p, size_t n) {
+    const auto* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    if (n && std::fwrite(p, 1, n, f) != n) fail(DGH_ERR_ARG, "write failed");
+  }
+  template <class T> void val(T v) { raw(&v, sizeof(T)); }
+  template <class T> void vec(const std::vector<T>& v) {
+    val<uint64_t>(v.size());
+    raw(v.data(), v.size() * sizeof(T));
+  }
+};
+
+struct BlobReader {
+  std::FILE* f;
+  uint64_t h = 1469598103934665603ull;
+  void raw(void* p, size_t n) {
+    if (n && std::fread(p, 1, n, f) != n) fail(DGH_ERR_ARG, "hierarchy blob truncated");
+    const auto* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  }
+  template <class T> T val() {
+    T v;
+    raw(&v, sizeof(T));
+    return v;
+  }
+  template <class T> void vec(std::vector<T>& v, uint64_t expect = ~0ull) {
+    const uint64_t n = val<uint64_t>();
+    if (n > (uint64_t(1) << 40) || (expect != ~0ull && n != expect)) fail(DGH_ERR_ARG, "hierarchy blob: bad array length");
+    v.resize(n);
+    raw(v.data(), n * sizeof(T));
+  }
+};
+
+struct FileCloser {
+  std::FILE* f;
+  ~FileCloser() {
+    if (f) std::fclose(f);
+  }
+};
+}  // namespace
 
 extern "C" {
 
@@ -381,6 +437,107 @@ int dgh_damg_coarsen_coefficients(dgh_damg h, const double* fine_kappa, const do
     }
     if (out_kappa && !k.empty()) std::memcpy(out_kappa, k.data(), k.size() * sizeof(double));
     if (out_mass && !m.empty()) std::memcpy(out_mass, m.data(), m.size() * sizeof(double));
+  });
+}
+
+int dgh_damg_save(dgh_damg h, const char* path) {
+  return guarded([&] {
+    if (!h || !path) fail(DGH_ERR_ARG, "NULL argument");
+    FileCloser fc{std::fopen(path, "wb")};
+    if (!fc.f) fail(DGH_ERR_ARG, "cannot open %s for writing", path);
+    BlobWriter w{fc.f};
+    w.raw(kBlobMagic, sizeof(kBlobMagic));
+    w.val<uint32_t>(kBlobVersion);
+    w.val<uint32_t>(uint32_t(h->levels.size()));
+    w.val<uint32_t>(kBlobBom);
+    for (size_t l = 0; l < h->levels.size(); ++l) {
+      const DALevel& d = h->levels[l];
+      w.val<uint32_t>(d.D);
+      w.val<uint32_t>(d.n_elem);
+      w.val<uint32_t>(d.n_node);
+      w.val<uint32_t>(d.n_hanging);
+      w.val<uint64_t>(d.octs.size());
+      for (const Oct& o : d.octs) {
+        w.val<uint64_t>(o.m);
+        w.val<uint8_t>(o.lev);
+        w.val<uint8_t>(o.flags);
+      }
+      w.vec(d.node_of);
+      w.vec(d.nlist);
+      w.vec(d.child);
+      w.vec(d.hnmask);
+      w.vec(d.lev);
+      if (l > 0) {
+        w.vec(h->celem[l]);
+        w.vec(h->pos[l]);
+        w.vec(h->first_fine[l]);
+        w.vec(h->same[l]);
+      }
+    }
+    const uint64_t hash = w.h;
+    w.val<uint64_t>(hash);
+    if (std::fflush(fc.f) != 0) fail(DGH_ERR_ARG, "write failed");
+  });
+}
+
+int dgh_damg_load(const char* path, dgh_damg* out) {
+  return guarded([&] {
+    if (!path || !out) fail(DGH_ERR_ARG, "NULL argument");
+    FileCloser fc{std::fopen(path, "rb")};
+    if (!fc.f) fail(DGH_ERR_ARG, "cannot open %s", path);
+    BlobReader r{fc.f};
+    char magic[8];
+    r.raw(magic, sizeof(magic));
+    if (std::memcmp(magic, kBlobMagic, sizeof(magic)) != 0) fail(DGH_ERR_ARG, "%s is not a hierarchy blob", path);
+    const uint32_t version = r.val<uint32_t>();
+    if (version != kBlobVersion) fail(DGH_ERR_ARG, "hierarchy blob version %u (this library reads %u)", version, kBlobVersion);
+    const uint32_t L = r.val<uint32_t>();
+    if (r.val<uint32_t>() != kBlobBom) fail(DGH_ERR_ARG, "hierarchy blob of another byte order");
+    if (L < 1 || L > 64) fail(DGH_ERR_ARG, "hierarchy blob: %u levels", L);
+    auto h = std::make_unique<dgh_damg_s>();
+    h->levels.resize(L);
+    h->celem.resize(L);
+    h->pos.resize(L);
+    h->first_fine.resize(L);
+    h->same.resize(L);
+    for (uint32_t l = 0; l < L; ++l) {
+      DALevel& d = h->levels[l];
+      d.D = r.val<uint32_t>();
+      d.n_elem = r.val<uint32_t>();
+      d.n_node = r.val<uint32_t>();
+      d.n_hanging = r.val<uint32_t>();
+      if (d.D < 1 || d.D > 30) fail(DGH_ERR_ARG, "hierarchy blob: depth %u", d.D);
+      const uint64_t no = r.val<uint64_t>();
+      if (no < d.n_elem || no > (uint64_t(1) << 36)) fail(DGH_ERR_ARG, "hierarchy blob: bad octant count");
+      d.octs.resize(no);
+      for (Oct& o : d.octs) {
+        o.m = r.val<uint64_t>();
+        o.lev = r.val<uint8_t>();
+        o.flags = r.val<uint8_t>();
+      }
+      r.vec(d.node_of, no);
+      r.vec(d.nlist, uint64_t(d.n_elem) * 8);
+      r.vec(d.child, d.n_elem);
+      r.vec(d.hnmask, d.n_elem);
+      r.vec(d.lev, d.n_elem);
+      for (uint32_t v : d.nlist)
+        if (v >= d.n_node) fail(DGH_ERR_MESH, "hierarchy blob: node id out of range");
+      if (l > 0) {
+        r.vec(h->celem[l], h->levels[l].n_node);
+        r.vec(h->pos[l], h->levels[l].n_node);
+        r.vec(h->first_fine[l], h->levels[l - 1].n_elem);
+        r.vec(h->same[l], h->levels[l - 1].n_elem);
+        for (uint32_t c : h->celem[l])
+          if (c >= h->levels[l - 1].n_elem) fail(DGH_ERR_MESH, "hierarchy blob: coarse element out of range");
+        for (uint8_t p : h->pos[l])
+          if (p >= 27) fail(DGH_ERR_MESH, "hierarchy blob: position code out of range");
+        for (uint32_t f : h->first_fine[l])
+          if (f >= d.n_elem) fail(DGH_ERR_MESH, "hierarchy blob: fine element out of range");
+      }
+    }
+    const uint64_t expect = r.h;
+    if (r.val<uint64_t>() != expect) fail(DGH_ERR_ARG, "hierarchy blob checksum mismatch");
+    *out = h.release();
   });
 }
 

@@ -17,6 +17,8 @@ bool is_complete_linear(const std::vector<Oct>& v, uint32_t D);
 // ---- stencils (stencils.cpp)
 void generate_stencils(double* lap2, double* mass2, double* rmat2, double* rmat1);
 void generate_transfer_weights(double* W /* [144][27][8] */);
+void generate_shfn(double* S /* [144][8][27] */);
+void generate_graddiv(double* GD /* [144][24][24] */);
 
 // ---- one DA level (da.cpp)
 // Octants of the DA live at depth Dda = D + 1 (the octree is embedded one level deeper so

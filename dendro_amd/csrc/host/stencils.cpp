@@ -157,6 +157,63 @@ void generate_stencils(double* lap2, double* mass2, double* rmat2, double* rmat1
   }
 }
 
+// Grad-div stencils of the vector (3 dofs per node) element:
+// GD[c][t][3k+a][3j+b] = int_{[-1,1]^3} d_a phi_k d_b phi_j (Matlab/computeGradDivType2.m; the
+// reference reads them from Stencils/GDType2Stencils.inp in createGDmatType2,
+// examples/src/backend/handleType2Stencils.C:303-330), by the exact monomial moments.
+void generate_graddiv(double* GD /* [144][24][24] */) {
+  for (int c = 0; c < 8; ++c)
+    for (int t = 0; t < 18; ++t) {
+      double a[8][8];
+      shape_coeffs(c, kEtypeMask[c][t], a);
+      double* G = GD + (size_t(c) * 18 + t) * 576;
+      for (int k = 0; k < 8; ++k)
+        for (int j = 0; j < 8; ++j)
+          for (int da = 0; da < 3; ++da)
+            for (int db = 0; db < 3; ++db) {
+              double v = 0.0;
+              for (int m = 0; m < 8; ++m) {
+                if (!kExp[m][da]) continue;
+                for (int n = 0; n < 8; ++n) {
+                  if (!kExp[n][db]) continue;
+                  const double cc = a[k][m] * a[j][n];
+                  if (cc == 0.0) continue;
+                  int e[3];
+                  for (int d = 0; d < 3; ++d) e[d] = kExp[m][d] + kExp[n][d] - (d == da) - (d == db);
+                  v += cc * moment(e[0]) * moment(e[1]) * moment(e[2]);
+                }
+              }
+              G[(3 * k + da) * 24 + 3 * j + db] = v;
+            }
+    }
+}
+
+// Shape functions at the 3-point Gauss nodes of the element, in the order of
+// examples/src/backend/omgRhs.C:1141-1180 (g = {0, +sqrt(3/5), -sqrt(3/5)}, m along x, n along
+// y, p along z): S[s][j][9 m + 3 n + p] = phi_j(g_m, g_n, g_p), s = 18 c + t.  Plays the role of
+// createShFnMat (examples/src/backend/handleType2Stencils.C:197-230), which reads
+// Stencils/ShFnStencils.inp.
+void generate_shfn(double* S /* [144][8][27] */) {
+  const double g[3] = {0.0, std::sqrt(3.0 / 5.0), -std::sqrt(3.0 / 5.0)};
+  for (int c = 0; c < 8; ++c)
+    for (int t = 0; t < 18; ++t) {
+      double a[8][8];
+      shape_coeffs(c, kEtypeMask[c][t], a);
+      for (int j = 0; j < 8; ++j)
+        for (int q = 0; q < 27; ++q) {
+          const double P[3] = {g[q / 9], g[(q / 3) % 3], g[q % 3]};
+          double v = 0.0;
+          for (int m = 0; m < 8; ++m) {
+            double b = a[j][m];
+            for (int d = 0; d < 3; ++d)
+              if (kExp[m][d]) b *= P[d];
+            v += b;
+          }
+          S[((size_t(c) * 18 + t) * 8 + j) * 27 + q] = v;
+        }
+    }
+}
+
 // Weights of the 8 coarse shape functions of stencil (c,t) at the 27 positions
 // {-1,0,1}^3 of the element, pos = tx + 3 ty + 9 tz: W[s][pos][k], s = 18 c + t.
 void generate_transfer_weights(double* W /* [144][27][8] */) {

@@ -147,6 +147,13 @@ int dgh_damg_transfer_arrays(dgh_damg h, int fine_level, uint32_t* coarse_elem, 
  * (examples/src/backend/matProp.C:192-232).  out_kappa/out_mass hold n_elem of the level. */
 int dgh_damg_coarsen_coefficients(dgh_damg h, const double* fine_kappa, const double* fine_mass,
                                   int level, double* out_kappa, double* out_mass);
+/* Saves / loads a hierarchy (every level's octants, node numbering, element arrays and the
+ * transfer maps) as a versioned binary with a checksum, so that a mesh is built once and
+ * loaded by later runs (checkpoint / resume of the mesh stage).  Loading checks the magic,
+ * version, byte order, array lengths, index ranges and checksum.  No reference counterpart
+ * (the reference rebuilds its DAs from points on every run, src/omg/omg.C:1306). */
+int dgh_damg_save(dgh_damg h, const char* path);
+int dgh_damg_load(const char* path, dgh_damg* out);
 int dgh_damg_free(dgh_damg h);
 
 /* ================================================================ device: runtime */
@@ -257,6 +264,51 @@ int dgh_mesh_set_tile_multi(dgh_mesh m, uint32_t tile_elems, int variant);
  * and its time per launch.  No reference counterpart. */
 int dgh_mesh_autotune_multi(dgh_mesh m, int k, uint32_t* tile_elems, int* variant, float* us);
 
+/* ================================================================ device: linear elasticity */
+/* Linear elasticity on the mesh, 3 dofs per node (vectors [N][3] interleaved, FP64):
+ * y = sum_e fac (mu L_e (x) I3 + (mu + lambda) GD_e) x_e with fac = h/2, L the Laplacian
+ * stencils and GD[3k+a][3j+b] = int d_a phi_k d_b phi_j the grad-div stencils; with boundary
+ * flags (dgh_mesh_set_boundary) boundary rows return x and boundary columns are dropped.
+ * Replaces CreateElasticityMat / ElasticityMatMult (examples/src/backend/elasticityJac.C:129-211,
+ * stencils Stencils/GDType2Stencils.inp).  set_elasticity builds the plan once (element order
+ * by stencil type, node slot CSR, a [E][24] FP64 buffer of element outputs). */
+int dgh_mesh_set_elasticity(dgh_mesh m, double mu, double lambda);
+int dgh_matvec_elasticity(dgh_mesh m, const double* x_dev, double* y_dev, void* stream);
+
+/* ================================================================ device: right-hand sides and errors */
+/* Functions on the unit cube for right-hand sides, nodal values and errors. */
+#define DGH_FN_GAUSSIANS 0  /* three Gaussians exp(-|x - c|^2 / 0.005), c = (0.1,0.1,0.1) on        */
+                            /* [0,0.2)^3, (0.3,..) on [0.2,0.4]^3, (0.6,..) on [0.5,0.7]^3        */
+                            /* (EVAL_FN, examples/src/backend/omgRhs.C:24-47)                    */
+#define DGH_FN_COS_SUM 1    /* sum_{k=1..nfreq} (a0 + 3 a1 k^2 pi^2) cos(k pi x) cos(k pi y)        */
+                            /* cos(k pi z): a0 = 1, a1 = 0 is ComputeSol's solution, a0 = a1 = 1  */
+                            /* the Helmholtz source of ComputeRHS1 (omgRhs.C:1196-1254, 1340-1430) */
+#define DGH_FN_POLY 2       /* c0 + c1 x + c2 y + c3 z + c4 xy + c5 yz + c6 zx + c7 xyz           */
+typedef struct {
+  int kind;          /* DGH_FN_* */
+  int nfreq;         /* COS_SUM: number of frequencies (the reference uses 9) */
+  double a0, a1;     /* COS_SUM: mass and Laplacian factors of each term */
+  double poly[8];    /* POLY coefficients */
+} dgh_function;
+
+/* Element anchors [n_elem][3] (DA coordinates, dgh_damg_level_arrays), uploaded once; needed by
+ * the three calls below, which evaluate functions at element points.  The DA's levels and
+ * hanging masks come from dgh_mesh_create. */
+int dgh_mesh_set_geometry(dgh_mesh m, const uint32_t* anchor);
+/* b_i = sum_e (h_e^3 / 8) sum over the 3x3x3 Gauss nodes of w f(x) phi_i(x), phi the element's
+ * hanging-node shape functions.  Replaces ComputeRHS2 (examples/src/backend/omgRhs.C:1127-1193)
+ * and its shape-function table (Stencils/ShFnStencils.inp), for any dgh_function.  Element
+ * contributions are added with device atomics: the last bits may vary from run to run. */
+int dgh_rhs_quadrature(dgh_mesh m, const dgh_function* f, double* b_dev, void* stream);
+/* u_i = f(node i), from each element's non-hanging vertices.  Replaces ComputeSol
+ * (examples/src/backend/omgRhs.C:1196-1254) for any dgh_function. */
+int dgh_nodal_values(dgh_mesh m, const dgh_function* f, double* u_dev, void* stream);
+/* err = ||f - u_h||_L2 over the mesh by 3x3x3 Gauss quadrature of each element, u_h the finite
+ * element function of the nodal values u_dev.  Replaces ComputeError5
+ * (examples/src/backend/omgRhs.C:1832-1890, whose exact solution is one fixed cosine) for any
+ * dgh_function.  Synchronous; the element sums are reduced in a fixed order. */
+int dgh_l2_error(dgh_mesh m, const dgh_function* f, const double* u_dev, double* err);
+
 /* ================================================================ device: intergrid transfer */
 typedef struct dgh_transfer_s* dgh_transfer;
 
@@ -272,6 +324,15 @@ int dgh_restrict(dgh_transfer t, int prec, const void* f_dev, void* c_dev, void*
 int dgh_prolong_add(dgh_transfer t, int prec, const void* c_dev, const void* f_in_dev,
                     void* f_out_dev, void* stream);
 
+/* Jacobian3 product on the coarse level of t: y = A3 x, the coarse operator formed from the
+ * fine level's element coefficients.  Replaces Jacobian3MatMult
+ * (examples/src/backend/omgJac.C:2139-2339), whose Type2/Type1 element blocks with the fine
+ * coefficients sum to the one-level Galerkin product P^T A_f P of the fine mesh's operator (the
+ * coarse space is nested in the fine one): it is applied here as prolongation, the fine
+ * operator with the fine mesh's coefficients, and restriction.  Two fine-level scratch vectors
+ * are allocated for the transfer on the first call of each precision. */
+int dgh_matvec_jac3(dgh_transfer t, int prec, const void* x_dev, void* y_dev, void* stream);
+
 /* ================================================================ device: GMG-PCG */
 typedef struct dgh_solver_s* dgh_solver;
 
@@ -280,6 +341,10 @@ typedef struct dgh_solver_s* dgh_solver;
 
 #define DGH_SMOOTHER_JACOBI 0
 #define DGH_SMOOTHER_LEARNED 1
+
+/* operator of the levels below the finest (-jacType of examples/src/drivers/checkError.C) */
+#define DGH_COARSE_JAC2 0   /* each level's own (coarsened) element coefficients           */
+#define DGH_COARSE_JAC3 1   /* Jacobian3: the next finer level's element coefficients       */
 
 typedef struct {
   double rtol;          /* relative tolerance (-ksp_rtol)                              */
@@ -292,6 +357,9 @@ typedef struct {
   int vcycle_prec;      /* DGH_F64: plain FP64; DGH_F32: FP32 V-cycle, FP64 outer CG   */
   int smoother;         /* DGH_SMOOTHER_*                                              */
   int use_graph;        /* capture the V-cycle in a hipGraph and replay it             */
+  int coarse_op;        /* DGH_COARSE_*: operator of residuals and sweeps below the finest */
+                        /* level and of the assembled coarsest matrix (the Jacobi       */
+                        /* diagonals stay each level's own)                             */
 } dgh_solver_params;
 
 typedef struct {

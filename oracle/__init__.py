@@ -30,9 +30,11 @@ def lib():
         _lib.or_mg_free.argtypes = [C.c_void_p]
         for f in ["or_mg_set_level", "or_mg_setup", "or_mg_setup_ex", "or_mg_vcycle", "or_pcg",
                   "or_mg_apply", "or_mg_coarse_matrix", "or_mg_coarse_set_lu", "or_mg_coarse_size",
-                  "or_mg_set_threads", "or_matvec_mt"]:
+                  "or_mg_set_threads", "or_matvec_mt", "or_mg_set_jac3", "or_matvec_jac3", "or_mg_set_boundary"]:
             getattr(_lib, f).argtypes = None
         _lib.or_learn_loss.restype = C.c_double
+        _lib.or_l2_error.restype = C.c_double
+        _lib.or_eval_fn.restype = C.c_double
     return _lib
 
 
@@ -177,6 +179,68 @@ def matvec_jac3(coarse, fine, x, kappa_fine, mass_fine=None, tol=1e-12):
     return y
 
 
+def _fn_args(kind, nfreq, a0, a1, poly):
+    pv = np.zeros(8) if poly is None else np.ascontiguousarray(poly, np.float64)
+    return [C.c_int(kind), C.c_int(nfreq), C.c_double(a0), C.c_double(a1), _p(pv)], pv
+
+
+_GD = None
+
+
+def stencils_graddiv():
+    global _GD
+    if _GD is None:
+        _GD = np.zeros(82944)
+        lib().or_stencils_graddiv(_p(_GD))
+    return _GD.reshape(8, 18, 24, 24)
+
+
+def matvec_elasticity(level, x, mu=1.0, lam=1.0, bdy=None):
+    """ElasticityMatMult: x, y [N][3] interleaved; bdy flags or None."""
+    st = _st()
+    gd = np.ascontiguousarray(stencils_graddiv().reshape(-1))
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    y = np.zeros(3 * level.n_node)
+    b = None if bdy is None else np.ascontiguousarray(bdy, np.uint8)
+    lib().or_matvec_elasticity(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), C.c_uint32(level.max_depth),
+                               _p(level.nlist), _p(level.child), _p(level.hnmask), _p(level.lev), _p(b),
+                               C.c_double(mu), C.c_double(lam), _p(st["lap2"]), _p(gd), _p(x), _p(y))
+    return y
+
+
+def eval_fn(kind, x, y, z, nfreq=9, a0=1.0, a1=0.0, poly=None):
+    args, keep = _fn_args(kind, nfreq, a0, a1, poly)
+    return lib().or_eval_fn(*args, C.c_double(x), C.c_double(y), C.c_double(z))
+
+
+def rhs_quadrature(level, kind, nfreq=9, a0=1.0, a1=0.0, poly=None):
+    """ComputeRHS2's quadrature right-hand side of a function (kinds as dgh_function)."""
+    shfn = np.ascontiguousarray(stencils_type1()["shfn"].reshape(-1))
+    args, keep = _fn_args(kind, nfreq, a0, a1, poly)
+    b = np.zeros(level.n_node)
+    lib().or_rhs_quadrature(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), C.c_uint32(level.max_depth),
+                            _p(level.nlist), _p(level.child), _p(level.hnmask), _p(level.lev), _p(level.anchor),
+                            *args, _p(shfn), _p(b))
+    return b
+
+
+def nodal_values(level, kind, nfreq=9, a0=1.0, a1=0.0, poly=None):
+    args, keep = _fn_args(kind, nfreq, a0, a1, poly)
+    u = np.zeros(level.n_node)
+    lib().or_nodal_values(C.c_uint32(level.n_elem), C.c_uint32(level.n_node), C.c_uint32(level.max_depth),
+                          _p(level.nlist), _p(level.hnmask), _p(level.lev), _p(level.anchor), *args, _p(u))
+    return u
+
+
+def l2_error(level, u, kind, nfreq=9, a0=1.0, a1=0.0, poly=None):
+    shfn = np.ascontiguousarray(stencils_type1()["shfn"].reshape(-1))
+    args, keep = _fn_args(kind, nfreq, a0, a1, poly)
+    u = np.ascontiguousarray(u, np.float64)
+    return lib().or_l2_error(C.c_uint32(level.n_elem), C.c_uint32(level.max_depth), _p(level.nlist),
+                             _p(level.child), _p(level.hnmask), _p(level.lev), _p(level.anchor), *args,
+                             _p(shfn), _p(u))
+
+
 def restrict(coarse, fine, f):
     st = _st()
     f = np.ascontiguousarray(f, np.float64)
@@ -207,7 +271,8 @@ class MG:
     # unblocked loop; both are partial-pivoting LU of the same assembled matrix
     LAPACK_ABOVE = 2500
 
-    def __init__(self, levels, omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=False, threads=1):
+    def __init__(self, levels, omega=2.0 / 3.0, nu_pre=2, nu_post=2, null_space=False, threads=1,
+                 jac3=False, jac3_tol=1e-12, boundary=None):
         self._keep = []
         self.h = lib().or_mg_create(len(levels))
         for i, (L, kappa, mass, lf, mf) in enumerate(levels):
@@ -220,6 +285,13 @@ class MG:
             assert rc == 0
         if threads > 1:
             lib().or_mg_set_threads(C.c_void_p(self.h), C.c_int(threads))
+        if jac3:
+            lib().or_mg_set_jac3(C.c_void_p(self.h), C.c_int(1), C.c_double(jac3_tol))
+        if boundary is not None:   # one flag array per level (coarsest first)
+            for i, bd in enumerate(boundary):
+                b8 = np.ascontiguousarray(bd, np.uint8)
+                self._keep.append(b8)
+                assert lib().or_mg_set_boundary(C.c_void_p(self.h), C.c_int(i), _p(b8)) == 0
         big = levels[0][0].n_node > self.LAPACK_ABOVE
         rc = lib().or_mg_setup_ex(C.c_void_p(self.h), C.c_double(omega), C.c_int(nu_pre), C.c_int(nu_post),
                                   C.c_int(int(null_space)), C.c_int(0 if big else 1))

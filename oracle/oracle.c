@@ -6,6 +6,7 @@
  *
  * Paths cite the reference tree.
  */
+#define _GNU_SOURCE
 #include "oracle.h"
 
 #include <math.h>
@@ -493,7 +494,7 @@ static int pairs_push(pairs_t* P, uint32_t fn, uint32_t cn, double w) {
 }
 
 static int transfer_pairs(const or_level* C, const or_level* F, const double* rmat2,
-                          const double* rmat1, pairs_t* out) {
+                          const double* rmat1, const uint8_t* bdyc, const uint8_t* bdyf, pairs_t* out) {
   taken_t tk;
   memset(out, 0, sizeof(*out));
   tk.cap = (uint32_t*)calloc(F->N, sizeof(uint32_t));
@@ -543,6 +544,9 @@ static int transfer_pairs(const or_level* C, const or_level* F, const double* rm
           }
           ids[tk.cap[fn]++] = cn;
           const double Rval = R[cC * 8 + fC];
+          /* ITLB_SET_VALUE_SUPPRESSED_DOFS (RestrictMatVec.C:75-80, ProlongMatVec.C:76-80):
+           * no entry between a boundary node and anything */
+          if ((bdyf && bdyf[fn]) || (bdyc && bdyc[cn])) continue;
           if (Rval != 0.0 && pairs_push(out, fn, cn, Rval)) {
             rc = -1;
             break;
@@ -570,7 +574,7 @@ static void transfer_apply(const pairs_t* P, uint32_t n_out, const double* in, d
 static int transfer(const or_level* C, const or_level* F, const double* rmat2,
                     const double* rmat1, const double* in, double* out, int restrict_dir) {
   pairs_t P;
-  int rc = transfer_pairs(C, F, rmat2, rmat1, &P);
+  int rc = transfer_pairs(C, F, rmat2, rmat1, NULL, NULL, &P);
   if (rc) return rc;
   transfer_apply(&P, restrict_dir ? C->N : F->N, in, out, restrict_dir);
   pairs_free(&P);
@@ -584,6 +588,193 @@ int or_restrict(const or_level* C, const or_level* F, const double* rmat2, const
 int or_prolong(const or_level* C, const or_level* F, const double* rmat2, const double* rmat1,
                const double* c, double* f) {
   return transfer(C, F, rmat2, rmat1, c, f, 0);
+}
+
+/* ------------------------------------------------------------------ elasticity (3 dofs per node) */
+/* Matlab/computeGradDivType2.m: gd2[c][t][3k+a][3j+b] = int_{[-1,1]^3} d_a phi_k d_b phi_j
+ * (2-point Gauss), the grad-div stencils of Stencils/GDType2Stencils.inp */
+void or_stencils_graddiv(double* gd2) {
+  const double g[2] = {-1.0 / sqrt(3.0), 1.0 / sqrt(3.0)};
+  for (int c = 0; c < 8; ++c)
+    for (int t = 0; t < 18; ++t) {
+      double phi[8][8];
+      coarse_coeffs(c, kMask[c][t], phi);
+      double* G = gd2 + ((size_t)c * 18 + t) * 576;
+      for (int q = 0; q < 576; ++q) G[q] = 0.0;
+      for (int m = 0; m < 2; ++m)
+        for (int n = 0; n < 2; ++n)
+          for (int p = 0; p < 2; ++p)
+            for (int k = 0; k < 8; ++k)
+              for (int a = 0; a < 3; ++a) {
+                const double ga = eval_grad(phi[k], a, g[m], g[n], g[p]);
+                for (int j = 0; j < 8; ++j)
+                  for (int b = 0; b < 3; ++b) G[(3 * k + a) * 24 + 3 * j + b] += ga * eval_grad(phi[j], b, g[m], g[n], g[p]);
+              }
+    }
+}
+
+/* ELASTICITY_ELEM_MULT_BLOCK (examples/src/backend/elasticityJac.C:129-165): fac = h/2; a
+ * boundary row (bdy nonzero, when bdy is given) is out[3i+d] = in[3i+d]; other rows add, over
+ * non-boundary columns j, mu fac L[k][j] in[3j+d] and (mu+lambda) fac GD[3k+d][3j+d'] in[3j+d'];
+ * vectors interleaved [N][3]. */
+void or_matvec_elasticity(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist, const uint8_t* child,
+                          const uint8_t* hnmask, const uint8_t* lev, const uint8_t* bdy, double mu, double lambda,
+                          const double* lap2, const double* gd2, const double* x, double* y) {
+  memset(y, 0, sizeof(double) * 3 * (size_t)N);
+  const double hFac = 1.0 / ((double)(1u << (maxD - 1)));
+  for (uint32_t e = 0; e < E; ++e) {
+    const double h = hFac * (double)(1u << (maxD - lev[e]));
+    const double fac = h / 2.0;
+    const int c = child[e];
+    int t = 0;
+    if (hnmask[e]) t = or_etype(c, hnmask[e]);
+    const double* L = lap2 + ((size_t)c * 18 + t) * 64;
+    const double* G = gd2 + ((size_t)c * 18 + t) * 576;
+    const uint32_t* idx = nlist + (size_t)e * 8;
+    for (int k = 0; k < 8; ++k) {
+      if (bdy && bdy[idx[k]]) {
+        for (int d = 0; d < 3; ++d) y[3 * (size_t)idx[k] + d] = x[3 * (size_t)idx[k] + d];
+        continue;
+      }
+      for (int j = 0; j < 8; ++j) {
+        if (bdy && bdy[idx[j]]) continue;
+        for (int d = 0; d < 3; ++d)
+          y[3 * (size_t)idx[k] + d] += (mu * fac * L[k * 8 + j] * x[3 * (size_t)idx[j] + d]);
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            y[3 * (size_t)idx[k] + a] += ((mu + lambda) * fac * (G[(3 * k + a) * 24 + 3 * j + b]) * x[3 * (size_t)idx[j] + b]);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ functions on the cube */
+/* EVAL_FN of examples/src/backend/omgRhs.C:24-47 (kind 0), the cosine sums of ComputeSol /
+ * ComputeRHS1 (omgRhs.C:1236-1245, 1382-1392) with factors (a0 + 3 a1 k^2 pi^2) (kind 1), and a
+ * trilinear polynomial (kind 2, coefficients of 1, x, y, z, xy, yz, zx, xyz). */
+typedef struct {
+  int kind, nfreq;
+  double a0, a1, c[8];
+} or_fn;
+
+static double or_eval(const or_fn* f, double x, double y, double z) {
+  double r = 0.0;
+  if (f->kind == 0) {
+    if ((x >= 0.0) && (y >= 0.0) && (z >= 0.0) && (x < 0.2) && (y < 0.2) && (z < 0.2)) {
+      double dx = x - 0.1, dy = y - 0.1, dz = z - 0.1;
+      r += exp(-((dx * dx + dy * dy + dz * dz) / 0.005));
+    }
+    if ((x >= 0.2) && (y >= 0.2) && (z >= 0.2) && (x <= 0.4) && (y <= 0.4) && (z <= 0.4)) {
+      double dx = x - 0.3, dy = y - 0.3, dz = z - 0.3;
+      r += exp(-((dx * dx + dy * dy + dz * dz) / 0.005));
+    }
+    if ((x >= 0.5) && (y >= 0.5) && (z >= 0.5) && (x <= 0.7) && (y <= 0.7) && (z <= 0.7)) {
+      double dx = x - 0.6, dy = y - 0.6, dz = z - 0.6;
+      r += exp(-((dx * dx + dy * dy + dz * dz) / 0.005));
+    }
+    return r;
+  }
+  if (f->kind == 1) {
+    for (int k = 1; k <= f->nfreq; ++k) {
+      double fk = k;
+      r += (f->a0 + 3 * f->a1 * fk * fk * M_PI * M_PI) * cos(fk * M_PI * x) * cos(fk * M_PI * y) * cos(fk * M_PI * z);
+    }
+    return r;
+  }
+  return f->c[0] + f->c[1] * x + f->c[2] * y + f->c[3] * z + f->c[4] * x * y + f->c[5] * y * z + f->c[6] * z * x +
+         f->c[7] * x * y * z;
+}
+
+double or_eval_fn(int kind, int nfreq, double a0, double a1, const double* poly, double x, double y, double z) {
+  or_fn f = {kind, nfreq, a0, a1, {0}};
+  if (poly) memcpy(f.c, poly, sizeof(f.c));
+  return or_eval(&f, x, y, z);
+}
+
+static const double kGw[3] = {8.0 / 9.0, 5.0 / 9.0, 5.0 / 9.0};
+
+/* ComputeRHS2 (examples/src/backend/omgRhs.C:1127-1193) for the function f: per element,
+ * 3x3x3 Gauss quadrature of f against the 8 shape functions (shfn[c][t][j][m][n][p]). */
+void or_rhs_quadrature(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist, const uint8_t* child,
+                       const uint8_t* hnmask, const uint8_t* lev, const uint32_t* anchor, int kind, int nfreq,
+                       double a0, double a1, const double* poly, const double* shfn, double* b) {
+  or_fn f = {kind, nfreq, a0, a1, {0}};
+  if (poly) memcpy(f.c, poly, sizeof(f.c));
+  const double g[3] = {0.0, sqrt((3.0 / 5.0)), -sqrt((3.0 / 5.0))};
+  memset(b, 0, sizeof(double) * N);
+  const double unit = (double)(1u << (maxD - 1));
+  for (uint32_t e = 0; e < E; ++e) {
+    const double hx = (double)(1u << (maxD - lev[e])) / unit;
+    const double x = anchor[3 * (size_t)e] / unit, y = anchor[3 * (size_t)e + 1] / unit,
+                 z = anchor[3 * (size_t)e + 2] / unit;
+    const double fac = ((hx * hx * hx) / 8.0);
+    const int c = child[e];
+    int t = 0;
+    if (hnmask[e]) t = or_etype(c, hnmask[e]);
+    for (int j = 0; j < 8; ++j) {
+      double integral = 0.0;
+      for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 3; ++n)
+          for (int p = 0; p < 3; ++p) {
+            const double fv = or_eval(&f, (hx * (1.0 + g[m]) * 0.5) + x, (hx * (1.0 + g[n]) * 0.5) + y,
+                                      (hx * (1.0 + g[p]) * 0.5) + z);
+            integral += kGw[m] * kGw[n] * kGw[p] * fv * shfn[((((size_t)c * 18 + t) * 8 + j) * 3 + m) * 9 + n * 3 + p];
+          }
+      b[nlist[(size_t)e * 8 + j]] += fac * integral;
+    }
+  }
+}
+
+/* ComputeSol (omgRhs.C:1196-1254) for the function f: values at the non-hanging vertices. */
+void or_nodal_values(uint32_t E, uint32_t N, uint32_t maxD, const uint32_t* nlist, const uint8_t* hnmask,
+                     const uint8_t* lev, const uint32_t* anchor, int kind, int nfreq, double a0, double a1,
+                     const double* poly, double* u) {
+  or_fn f = {kind, nfreq, a0, a1, {0}};
+  if (poly) memcpy(f.c, poly, sizeof(f.c));
+  memset(u, 0, sizeof(double) * N);
+  const double unit = (double)(1u << (maxD - 1));
+  for (uint32_t e = 0; e < E; ++e) {
+    const double hx = (double)(1u << (maxD - lev[e])) / unit;
+    const double x = anchor[3 * (size_t)e] / unit, y = anchor[3 * (size_t)e + 1] / unit,
+                 z = anchor[3 * (size_t)e + 2] / unit;
+    for (int i = 0; i < 8; ++i)
+      if (!(hnmask[e] & (1 << i)))
+        u[nlist[(size_t)e * 8 + i]] = or_eval(&f, x + (i & 1) * hx, y + ((i >> 1) & 1) * hx, z + ((i >> 2) & 1) * hx);
+  }
+}
+
+/* ComputeError5 (omgRhs.C:1832-1890) for the function f: sqrt of the sum over elements of the
+ * 3x3x3 Gauss quadrature of (f - u_h)^2. */
+double or_l2_error(uint32_t E, uint32_t maxD, const uint32_t* nlist, const uint8_t* child, const uint8_t* hnmask,
+                   const uint8_t* lev, const uint32_t* anchor, int kind, int nfreq, double a0, double a1,
+                   const double* poly, const double* shfn, const double* u) {
+  or_fn f = {kind, nfreq, a0, a1, {0}};
+  if (poly) memcpy(f.c, poly, sizeof(f.c));
+  const double g[3] = {0.0, sqrt((3.0 / 5.0)), -sqrt((3.0 / 5.0))};
+  const double unit = (double)(1u << (maxD - 1));
+  double error = 0.0;
+  for (uint32_t e = 0; e < E; ++e) {
+    const double hx = (double)(1u << (maxD - lev[e])) / unit;
+    const double x = anchor[3 * (size_t)e] / unit, y = anchor[3 * (size_t)e + 1] / unit,
+                 z = anchor[3 * (size_t)e + 2] / unit;
+    const double fac = ((hx * hx * hx) / 8.0);
+    const int c = child[e];
+    int t = 0;
+    if (hnmask[e]) t = or_etype(c, hnmask[e]);
+    double integral = 0.0;
+    for (int m = 0; m < 3; ++m)
+      for (int n = 0; n < 3; ++n)
+        for (int p = 0; p < 3; ++p) {
+          const double fv = or_eval(&f, (hx * (1.0 + g[m]) * 0.5) + x, (hx * (1.0 + g[n]) * 0.5) + y,
+                                    (hx * (1.0 + g[p]) * 0.5) + z);
+          double uh = 0.0;
+          for (int j = 0; j < 8; ++j)
+            uh += u[nlist[(size_t)e * 8 + j]] * shfn[((((size_t)c * 18 + t) * 8 + j) * 3 + m) * 9 + n * 3 + p];
+          integral += kGw[m] * kGw[n] * kGw[p] * (fv - uh) * (fv - uh);
+        }
+    error += integral * fac;
+  }
+  return sqrt(error);
 }
 
 /* ------------------------------------------------------------------ Jacobian3 */
@@ -664,6 +855,7 @@ typedef struct {
   const double *kappa, *mass;
   double lap_fac, mass_fac;
   double* invdiag;
+  const uint8_t* bdy;      /* Dirichlet: boundary flags of the nodes (NULL: natural) */
   double *b, *x, *r, *z;   /* work vectors of this level */
   pairs_t tp;              /* transfer pairs from the next coarser level (level >= 1) */
 } mg_level;
@@ -680,6 +872,10 @@ struct or_mg_s {
   int* piv;
   int factored;
   int nthreads;  /* > 1: level products by or_matvec_mt */
+  int jac3;      /* Jacobian3 on the levels below the finest (operator of residuals and sweeps,
+                    and the assembled coarsest matrix); the Jacobi diagonals stay Jacobian2's */
+  double *lap1, *mass1;
+  double jac3_tol;
 };
 
 or_mg* or_mg_create(int nlevels) {
@@ -692,6 +888,9 @@ or_mg* or_mg_create(int nlevels) {
   mg->rmat2 = (double*)malloc(sizeof(double) * 9216);
   mg->rmat1 = (double*)malloc(sizeof(double) * 73728);
   or_stencils(mg->lap2, mg->mass2, mg->rmat2, mg->rmat1);
+  mg->lap1 = (double*)malloc(sizeof(double) * 73728);
+  mg->mass1 = (double*)malloc(sizeof(double) * 73728);
+  or_stencils_type1(mg->lap1, mg->mass1);
   return mg;
 }
 
@@ -710,6 +909,8 @@ void or_mg_free(or_mg* mg) {
   free(mg->mass2);
   free(mg->rmat2);
   free(mg->rmat1);
+  free(mg->lap1);
+  free(mg->mass1);
   free(mg->lu);
   free(mg->piv);
   free(mg);
@@ -738,6 +939,11 @@ int or_mg_set_level(or_mg* mg, int l, uint32_t E, uint32_t N, uint32_t maxD, con
 
 static void apply_A(or_mg* mg, int l, const double* x, double* y) {
   mg_level* L = &mg->L[l];
+  if (L->bdy) {
+    or_matvec_dirichlet(L->lv.E, L->lv.N, L->lv.max_depth, L->lv.nlist, L->lv.child, L->lv.hnmask, L->lv.lev,
+                        L->bdy, L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2, mg->mass2, x, y);
+    return;
+  }
   if (mg->nthreads > 1 &&
       or_matvec_mt(mg->nthreads, L->lv.E, L->lv.N, L->lv.max_depth, L->lv.nlist, L->lv.child,
                    L->lv.hnmask, L->lv.lev, L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2,
@@ -745,6 +951,22 @@ static void apply_A(or_mg* mg, int l, const double* x, double* y) {
     return;
   or_matvec(L->lv.E, L->lv.N, L->lv.max_depth, L->lv.nlist, L->lv.child, L->lv.hnmask, L->lv.lev,
             L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2, mg->mass2, x, y);
+}
+
+/* The operator of level l inside the V-cycle (PETSc's Jmat of the level): with Jacobian3 the
+ * levels below the finest use the next finer level's element coefficients
+ * (examples/src/drivers/checkError.C:658-667 wires them; Jacobian3MatMult, omgJac.C:2235-2339);
+ * uniform coefficients make Jacobian3 equal to the level's own operator. */
+static int j3_level(const or_mg* mg, int l) {
+  return mg->jac3 && l < mg->nlevels - 1 && mg->L[l + 1].kappa != NULL;
+}
+static void apply_J(or_mg* mg, int l, const double* x, double* y) {
+  if (!j3_level(mg, l)) {
+    apply_A(mg, l, x, y);
+    return;
+  }
+  or_matvec_jac3(&mg->L[l].lv, &mg->L[l + 1].lv, mg->L[l + 1].kappa, mg->L[l + 1].mass, mg->jac3_tol, mg->lap2,
+                 mg->mass2, mg->lap1, mg->mass1, x, y);
 }
 
 int or_mg_apply(or_mg* mg, int level, const double* x, double* y) {
@@ -765,6 +987,7 @@ static void project_mean(double* v, uint32_t n) {
  * (the reference's -mg_coarse_pc_type lu, bin/options:203-206).  With a constant null space
  * the factored matrix is A + a 11^T (a = trace/n^2) applied to mean-free right-hand sides,
  * which returns the mean-free solution of A x = b. */
+static int coarse_assemble_jac3(or_mg* mg);
 static int coarse_assemble(or_mg* mg) {
   mg_level* L = &mg->L[0];
   const int n = (int)L->lv.N;
@@ -773,6 +996,11 @@ static int coarse_assemble(or_mg* mg) {
   mg->lu = (double*)calloc((size_t)n * n, sizeof(double));
   mg->piv = (int*)malloc(sizeof(int) * n);
   if (!mg->lu || !mg->piv) return -1;
+  if (j3_level(mg, 0)) {
+    int rc = coarse_assemble_jac3(mg);
+    if (rc) return rc;
+    goto null_space;
+  }
   /* element by element: entry (idx[k], idx[j]) += fac1 L[k][j] + fac2 M[k][j], elements in
    * storage order -- the same terms in the same order as the column products A e_j */
   const double hFac = 1.0 / ((double)(1u << (L->lv.max_depth - 1)));
@@ -792,11 +1020,86 @@ static int coarse_assemble(or_mg* mg) {
       for (int j = 0; j < 8; ++j)
         mg->lu[(size_t)idx[k] * n + idx[j]] += ((fac1 * Ls[k * 8 + j]) + (fac2 * Ms[k * 8 + j])) * 1.0;
   }
+null_space:
+  if (L->bdy) { /* ComputeDirichletJacobian (omgJac.C:382-506): boundary rows and columns of the
+                   identity */
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        if (L->bdy[i] || L->bdy[j]) mg->lu[(size_t)i * n + j] = (i == j) ? 1.0 : 0.0;
+  }
   if (mg->null_space) {
     double tr = 0.0;
     for (int i = 0; i < n; ++i) tr += mg->lu[(size_t)i * n + i];
     const double a = tr / ((double)n * n);
     for (size_t q = 0; q < (size_t)n * n; ++q) mg->lu[q] += a;
+  }
+  return 0;
+}
+
+/* BUILD_FULL_JAC_TYPE3_BLOCK (examples/src/backend/omgJac.C:2341-2448): the coarsest matrix
+ * from level 1's element coefficients, element by element with the blocks of
+ * or_matvec_jac3 (Type2 for a coarse element that is also a fine one or whose 8 children
+ * agree within tol, else the sum of its children's Type1 blocks). */
+static int coarse_assemble_jac3(or_mg* mg) {
+  const or_level* C = &mg->L[0].lv;
+  const or_level* F = &mg->L[1].lv;
+  const double* kf = mg->L[1].kappa;
+  const double* mf = mg->L[1].mass;
+  const int n = (int)C->N;
+  double* A = mg->lu;
+  const uint32_t maxD = C->max_depth;
+  const double hFac = 1.0 / ((double)(1u << (maxD - 1)));
+  uint32_t f = 0;
+  for (uint32_t c = 0; c < C->E; ++c) {
+    const uint32_t* ca = C->anchor + 3 * (size_t)c;
+    while (f < F->E && (F->anchor[3 * (size_t)f] != ca[0] || F->anchor[3 * (size_t)f + 1] != ca[1] ||
+                        F->anchor[3 * (size_t)f + 2] != ca[2]))
+      ++f;
+    if (f >= F->E) return -2;
+    const double h = hFac * (double)(1u << (maxD - C->lev[c]));
+    const uint32_t* idx = C->nlist + (size_t)c * 8;
+    const int cn = C->child[c];
+    int t = 0;
+    if (C->hnmask[c]) t = or_etype(cn, C->hnmask[c]);
+    const double* L2 = mg->lap2 + ((size_t)cn * 18 + t) * 64;
+    const double* M2 = mg->mass2 + ((size_t)cn * 18 + t) * 64;
+    double K[64];
+    if (F->lev[f] == C->lev[c]) {
+      const double fac1 = kf[f] * h / 2.0, fac2 = (mf ? mf[f] : 0.0) * h * h * h / 8.0;
+      for (int q = 0; q < 64; ++q) K[q] = (fac1 * L2[q]) + (fac2 * M2[q]);
+      ++f;
+    } else {
+      if (f + 8 > F->E) return -2;
+      double p1[8], p2[8], max1, min1, max2, min2, tot1 = 0.0, tot2 = 0.0;
+      for (int q = 0; q < 8; ++q, ++f) {
+        p1[q] = kf[f];
+        p2[q] = mf ? mf[f] : 0.0;
+      }
+      max1 = min1 = p1[0];
+      max2 = min2 = p2[0];
+      for (int q = 0; q < 8; ++q) {
+        max1 = p1[q] > max1 ? p1[q] : max1;
+        min1 = p1[q] < min1 ? p1[q] : min1;
+        max2 = p2[q] > max2 ? p2[q] : max2;
+        min2 = p2[q] < min2 ? p2[q] : min2;
+        tot1 += p1[q];
+        tot2 += p2[q];
+      }
+      if (((max1 - min1) <= (mg->jac3_tol * max1)) && ((max2 - min2) <= (mg->jac3_tol * max2))) {
+        const double fac1 = tot1 * h / 16.0, fac2 = tot2 * h * h * h / 64.0;
+        for (int q = 0; q < 64; ++q) K[q] = (fac1 * L2[q]) + (fac2 * M2[q]);
+      } else {
+        for (int q = 0; q < 64; ++q) K[q] = 0.0;
+        for (int cf = 0; cf < 8; ++cf) {
+          const double fac1 = p1[cf] * h / 2.0, fac2 = p2[cf] * h * h * h / 8.0;
+          const double* L1 = mg->lap1 + (((size_t)cn * 18 + t) * 8 + cf) * 64;
+          const double* M1 = mg->mass1 + (((size_t)cn * 18 + t) * 8 + cf) * 64;
+          for (int q = 0; q < 64; ++q) K[q] += (fac1 * L1[q]) + (fac2 * M1[q]);
+        }
+      }
+    }
+    for (int k = 0; k < 8; ++k)
+      for (int j = 0; j < 8; ++j) A[(size_t)idx[k] * n + idx[j]] += K[k * 8 + j];
   }
   return 0;
 }
@@ -875,14 +1178,19 @@ int or_mg_setup_ex(or_mg* mg, double omega, int nu_pre, int nu_post, int null_sp
     L->x = (double*)calloc(N, sizeof(double));
     L->r = (double*)calloc(N, sizeof(double));
     L->z = (double*)calloc(N, sizeof(double));
-    or_diagonal(L->lv.E, N, L->lv.max_depth, L->lv.nlist, L->lv.child, L->lv.hnmask, L->lv.lev,
-                L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2, mg->mass2, L->invdiag);
+    if (L->bdy)
+      or_diagonal_dirichlet(L->lv.E, N, L->lv.max_depth, L->lv.nlist, L->lv.child, L->lv.hnmask, L->lv.lev,
+                            L->bdy, L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2, mg->mass2, L->invdiag);
+    else
+      or_diagonal(L->lv.E, N, L->lv.max_depth, L->lv.nlist, L->lv.child, L->lv.hnmask, L->lv.lev,
+                  L->kappa, L->mass, L->lap_fac, L->mass_fac, mg->lap2, mg->mass2, L->invdiag);
     /* PCSetUp_Jacobi: the reciprocal of the diagonal */
     for (uint32_t i = 0; i < N; ++i) L->invdiag[i] = 1.0 / L->invdiag[i];
   }
   for (int l = 1; l < mg->nlevels; ++l) {
     pairs_free(&mg->L[l].tp);
-    int rp = transfer_pairs(&mg->L[l - 1].lv, &mg->L[l].lv, mg->rmat2, mg->rmat1, &mg->L[l].tp);
+    int rp = transfer_pairs(&mg->L[l - 1].lv, &mg->L[l].lv, mg->rmat2, mg->rmat1, mg->L[l - 1].bdy, mg->L[l].bdy,
+                            &mg->L[l].tp);
     if (rp) return rp;
   }
   int rc = coarse_assemble(mg);
@@ -890,9 +1198,22 @@ int or_mg_setup_ex(or_mg* mg, double omega, int nu_pre, int nu_post, int null_sp
   return coarse_factor(mg);
 }
 
+int or_mg_set_boundary(or_mg* mg, int level, const uint8_t* bdy) {
+  if (!mg || level < 0 || level >= mg->nlevels) return -1;
+  mg->L[level].bdy = bdy;
+  return 0;
+}
+
 int or_mg_set_threads(or_mg* mg, int nthreads) {
   if (!mg || nthreads < 1 || nthreads > 256) return -1;
   mg->nthreads = nthreads;
+  return 0;
+}
+
+int or_mg_set_jac3(or_mg* mg, int on, double tol) {
+  if (!mg || tol < 0.0) return -1;
+  mg->jac3 = on;
+  mg->jac3_tol = tol;
   return 0;
 }
 
@@ -922,13 +1243,13 @@ static void richardson(or_mg* mg, int l, const double* b, double* x, int nu, int
   if (guess_zero) {
     memcpy(r, b, sizeof(double) * N);
   } else {
-    apply_A(mg, l, x, r);
+    apply_J(mg, l, x, r);
     for (uint32_t i = 0; i < N; ++i) r[i] = b[i] - r[i];
   }
   for (int it = 0; it < nu; ++it) {
     for (uint32_t i = 0; i < N; ++i) x[i] = x[i] + mg->omega * (r[i] * L->invdiag[i]);
     if (it + 1 < nu) {
-      apply_A(mg, l, x, r);
+      apply_J(mg, l, x, r);
       for (uint32_t i = 0; i < N; ++i) r[i] = b[i] - r[i];
     }
   }
@@ -946,7 +1267,7 @@ static int vcycle(or_mg* mg, int l, const double* b, double* x) {
   const uint32_t N = L->lv.N;
   memset(x, 0, sizeof(double) * N);
   richardson(mg, l, b, x, mg->nu_pre, 1);
-  apply_A(mg, l, x, L->r);
+  apply_J(mg, l, x, L->r);
   for (uint32_t i = 0; i < N; ++i) L->r[i] = b[i] - L->r[i];
   transfer_apply(&L->tp, Lc->lv.N, L->r, Lc->b, 1);
   memset(Lc->x, 0, sizeof(double) * Lc->lv.N);

@@ -86,6 +86,26 @@ int or_restrict(const or_level* coarse, const or_level* fine, const double* rmat
 int or_prolong(const or_level* coarse, const or_level* fine, const double* rmat2,
                const double* rmat1, const double* c, double* f);
 
+/* elasticity (3 dofs per node): grad-div stencils [8][18][24][24] and the product of
+ * ElasticityMatMult with optional Dirichlet flags (NULL: none), vectors [N][3] */
+void or_stencils_graddiv(double* gd2);
+void or_matvec_elasticity(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist, const uint8_t* child,
+                          const uint8_t* hnmask, const uint8_t* lev, const uint8_t* bdy, double mu, double lambda,
+                          const double* lap2, const double* gd2, const double* x, double* y);
+
+/* functions on the unit cube: kind 0 = ComputeRHS2's Gaussians, 1 = cosine sum
+ * sum_{k<=nfreq} (a0 + 3 a1 k^2 pi^2) cos cos cos, 2 = trilinear polynomial poly[8] */
+double or_eval_fn(int kind, int nfreq, double a0, double a1, const double* poly, double x, double y, double z);
+void or_rhs_quadrature(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist, const uint8_t* child,
+                       const uint8_t* hnmask, const uint8_t* lev, const uint32_t* anchor, int kind, int nfreq,
+                       double a0, double a1, const double* poly, const double* shfn, double* b);
+void or_nodal_values(uint32_t E, uint32_t N, uint32_t max_depth, const uint32_t* nlist, const uint8_t* hnmask,
+                     const uint8_t* lev, const uint32_t* anchor, int kind, int nfreq, double a0, double a1,
+                     const double* poly, double* u);
+double or_l2_error(uint32_t E, uint32_t max_depth, const uint32_t* nlist, const uint8_t* child,
+                   const uint8_t* hnmask, const uint8_t* lev, const uint32_t* anchor, int kind, int nfreq,
+                   double a0, double a1, const double* poly, const double* shfn, const double* u);
+
 /* Jacobian3MatMult (omgJac.C:2139-2233): coarse-level product with the fine level's element
  * coefficients kf/mf (mf NULL: zero mass); returns 0, or -2 when the meshes do not nest. */
 int or_matvec_jac3(const or_level* coarse, const or_level* fine, const double* kf, const double* mf,
@@ -108,6 +128,14 @@ int or_mg_setup(or_mg* mg, double omega, int nu_pre, int nu_post, int null_space
  * e.g. from a blocked library factorisation of a large coarsest level. */
 int or_mg_setup_ex(or_mg* mg, double omega, int nu_pre, int nu_post, int null_space, int factor);
 int or_mg_coarse_size(or_mg* mg);
+/* Jacobian3 (-jacType 3 of examples/src/drivers/checkError.C:658-667) on the levels below the
+ * finest, tolerance -tolToCoarsenMatProp; before or_mg_setup */
+int or_mg_set_jac3(or_mg* mg, int on, double tol);
+/* Dirichlet problem (SetDirichletJacContexts, examples/src/backend/omgJac.C:508-586): the
+ * level's operator and diagonal become the Dirichlet-masked ones, transfers drop every entry
+ * that touches a boundary node (the suppressed DOFs), the coarsest matrix has identity rows
+ * and columns on the boundary; bdy[N] must stay alive; before or_mg_setup */
+int or_mg_set_boundary(or_mg* mg, int level, const uint8_t* bdy);
 /* level products on nthreads threads (or_matvec_mt); 1 (default) keeps or_matvec */
 int or_mg_set_threads(or_mg* mg, int nthreads);
 int or_mg_coarse_matrix(or_mg* mg, double* A);

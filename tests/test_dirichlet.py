@@ -169,3 +169,121 @@ def test_dirichlet_needs_boundary(dev, hier_small):
     m.set_boundary(None)
     with pytest.raises(da.DendroError):
         m.diagonal_dirichlet(yt)
+
+
+# ------------------------------------------------------------------ the Dirichlet solve
+def _dirichlet_hierarchy(h, mode):
+    ms, coefs = [], var_coefficients(h, mass_base=1.0) if mode == "var" else None
+    for l in range(h.n):
+        m = da.Mesh(h.levels[l])
+        if coefs is None:
+            m.set_coefficients(None, None, **MODES[mode])
+        else:
+            m.set_coefficients(coefs[l][0], coefs[l][1])
+        m.set_boundary(h.damg.boundary(l))
+        ms.append(m)
+    ts = [da.Transfer(ms[l - 1], ms[l], *h.transfers[l - 1]) for l in range(1, h.n)]
+    lv = [(h.levels[l], None if coefs is None else coefs[l][0], None if coefs is None else coefs[l][1],
+           MODES.get(mode, {}).get("lap_fac", 1.0), MODES.get(mode, {}).get("mass_fac", 1.0)) for l in range(h.n)]
+    return ms, ts, lv
+
+
+@pytest.mark.gpu
+def test_dirichlet_transfers_gpu(dev, hier_small):
+    """With boundary flags on both meshes, restriction and prolongation drop every entry that
+    touches a boundary node (the suppressed DOFs of RestrictMatVec.C:75-80 and
+    ProlongMatVec.C:76-80): against the oracle's masked transfer pairs, through the V-cycle's
+    own kernels."""
+    h = hier_small
+    ms, ts, lv = _dirichlet_hierarchy(h, "laplacian")
+    C, F = h.levels[-2], h.levels[-1]
+    bc, bf = h.damg.boundary(h.n - 2) != 0, h.damg.boundary(h.n - 1) != 0
+    f = np.random.default_rng(5).uniform(-1, 1, F.n_node)
+    c = np.random.default_rng(6).uniform(-1, 1, C.n_node)
+    ct = torch.empty(C.n_node, dtype=torch.float64, device=dev)
+    ts[-1].restrict(torch.tensor(f, device=dev), ct)
+    ref = oracle.restrict(C, F, np.where(bf, 0.0, f))
+    ref[bc] = 0.0
+    assert rel_err(ct.cpu().numpy(), ref) <= 1e-12
+    fo = torch.empty(F.n_node, dtype=torch.float64, device=dev)
+    ts[-1].prolong_add(torch.tensor(c, device=dev), torch.tensor(f, device=dev), fo)
+    p = oracle.prolong(C, F, np.where(bc, 0.0, c))
+    p[bf] = 0.0
+    assert rel_err(fo.cpu().numpy(), f + p) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["laplacian", "var"])
+def test_dirichlet_solve_gpu(dev, hier_small, mode):
+    """The Dirichlet problem through the multigrid-preconditioned CG (SetDirichletJacContexts,
+    examples/src/backend/omgJac.C:508-586): V-cycle and PCG iterates k = 1, 3, 5 against the
+    oracle's Dirichlet hierarchy (<= 1e-11 / 1e-12), the solution returns b on boundary nodes,
+    and the FP32 V-cycle solve meets the same true residual."""
+    h = hier_small
+    ms, ts, lv = _dirichlet_hierarchy(h, mode)
+    bd = [h.damg.boundary(l) for l in range(h.n)]
+    mg = oracle.MG(lv, omega=2.0 / 3.0, nu_pre=2, nu_post=2, boundary=bd)
+    b = np.random.default_rng(51).uniform(-1, 1, h.fine.n_node)
+    bt = torch.tensor(b, dtype=torch.float64, device=dev)
+    s = da.Solver(ms, ts, da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2))
+    z = torch.empty_like(bt)
+    s.vcycle(bt, z)
+    torch.cuda.synchronize()
+    assert rel_err(z.cpu().numpy(), mg.vcycle(b)) <= 1e-11
+    for k in (1, 3, 5):
+        xo, its, _, ho = mg.pcg(b, rtol=1e-30, max_it=k, norm_type=da.NORM_UNPRECONDITIONED)
+        p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=k, rtol=1e-30,
+                              norm_type=da.NORM_UNPRECONDITIONED)
+        x = torch.zeros_like(bt)
+        st, hist = da.Solver(ms, ts, p).solve(bt, x, history_len=k + 1)
+        torch.cuda.synchronize()
+        assert st.iterations == its == k
+        assert rel_err(x.cpu().numpy(), xo) <= 1e-12, k
+        assert np.allclose(hist, ho, rtol=1e-10, atol=0)
+    out = {}
+    for prec in (da.F64, da.F32):
+        p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=200, rtol=1e-10,
+                              norm_type=da.NORM_UNPRECONDITIONED, vcycle_prec=prec, use_graph=1)
+        x = torch.zeros_like(bt)
+        st, _ = da.Solver(ms, ts, p).solve(bt, x)
+        assert st.converged and st.rel_true <= 1e-10, (prec, st.rel_true)
+        xs = x.cpu().numpy()
+        assert np.abs(xs[bd[-1] != 0] - b[bd[-1] != 0]).max() <= 1e-9
+        out[prec] = (xs, st.iterations)
+    print(f"Dirichlet {mode}: FP64 {out[da.F64][1]} its, FP32 V-cycle {out[da.F32][1]} its, "
+          f"solution difference {rel_err(out[da.F32][0], out[da.F64][0]):.2e}")
+
+
+@pytest.mark.gpu
+def test_dirichlet_solver_rules(dev, hier_small):
+    """Flags on some levels only, a null space with flags, or flags changed after the solver
+    was created are errors; new coefficients after creation are taken up (diagonals and the
+    coarsest inverse rebuilt)."""
+    h = hier_small
+    ms, ts, lv = _dirichlet_hierarchy(h, "laplacian")
+    ms[0].set_boundary(None)
+    with pytest.raises(da.DendroError):
+        da.Solver(ms, ts, da.default_params())
+    ms[0].set_boundary(h.damg.boundary(0))
+    with pytest.raises(da.DendroError):
+        da.Solver(ms, ts, da.default_params(null_space=1))
+    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=100, rtol=1e-10,
+                          norm_type=da.NORM_UNPRECONDITIONED)
+    s = da.Solver(ms, ts, p)
+    b = torch.tensor(np.random.default_rng(2).uniform(-1, 1, h.fine.n_node), device=dev)
+    x = torch.zeros_like(b)
+    s.solve(b, x)
+    # Laplacian -> Jacobian (mass added) on every level after creation: same solver, new operator
+    for m in ms:
+        m.set_coefficients(None, None, 1.0, 1.0)
+    x2 = torch.zeros_like(b)
+    st2, _ = s.solve(b, x2)
+    _, _, lvj = _dirichlet_hierarchy(h, "jacobian")
+    xo, its, conv, _ = oracle.MG(lvj, omega=2.0 / 3.0, nu_pre=2, nu_post=2,
+                                 boundary=[h.damg.boundary(l) for l in range(h.n)]).pcg(
+        b.cpu().numpy(), rtol=1e-10, max_it=100, norm_type=0)
+    assert st2.converged and st2.iterations == its
+    assert rel_err(x2.cpu().numpy(), xo) <= 1e-9
+    ms[-1].set_boundary(h.damg.boundary(h.n - 1))
+    with pytest.raises(da.DendroError):
+        s.solve(b, x)

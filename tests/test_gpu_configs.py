@@ -256,3 +256,8 @@ def test_config5_learned_4m(dev):
           f"{stl.iterations} its {stl.seconds*1e3:.1f} ms")
     assert stj.converged and stl.converged and stl.rel_true <= 1e-8
     assert tr.loss_last < tr.loss_first
+    del s
+    # Jacobian3 below the finest level (the next finer level's coefficients; Jacobi smoothing)
+    _, st3, _ = _solve(ms, ts, b, dev, null_space=1, vcycle_prec=da.F32, coarse_op=da.COARSE_JAC3)
+    print(f"config 5: Jacobian3 levels, Jacobi {st3.iterations} its {st3.seconds*1e3:.1f} ms")
+    assert st3.converged and st3.rel_true <= 1e-8

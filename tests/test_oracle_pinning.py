@@ -448,3 +448,236 @@ def test_jacobian3_is_one_level_galerkin(hier_tiny):
     assert np.abs(A2c - G).max() > 1e-3 * scale
     x = np.random.default_rng(10).uniform(-1, 1, C.n_node)
     assert rel_err(oracle.matvec_jac3(C, F, x, kf, mf, tol=1e300), A2c @ x) <= 1e-14
+
+
+class DenseMGJac3(DenseMG):
+    """DenseMG with the Jacobian3 level operators of -jacType 3: below the finest level the
+    residuals and sweeps use J_l = P_l^T A_{l+1} P_l (the next finer level's Jacobian2 with its
+    own coefficients), the Jacobi diagonals stay diag(A_l), and the coarsest solve uses J_0."""
+
+    def __init__(self, h, coefs, null_space):
+        super().__init__(h, coefs, null_space)
+        self.J = [(self.P[l].T @ self.A[l + 1] @ self.P[l]).tocsr() for l in range(h.n - 1)] + [self.A[-1]]
+        J0 = self.J[0].toarray()
+        self.A0inv = np.linalg.pinv(J0) if null_space else np.linalg.inv(J0)
+
+    def vcycle(self, B, omega, nu_pre, nu_post, l=None):
+        top = l is None
+        l = self.h.n - 1 if top else l
+        if l == 0:
+            X = self.A0inv @ ((B - B.mean(0)) if self.ns else B)
+        else:
+            J = self.J[l]
+            dinv = (1.0 / self.A[l].diagonal())[:, None]
+            X = np.zeros_like(B)
+            for _ in range(nu_pre):
+                X = X + omega * dinv * (B - J @ X)
+            Xc = self.vcycle(self.P[l - 1].T @ (B - J @ X), omega, nu_pre, nu_post, l - 1)
+            X = X + self.P[l - 1] @ Xc
+            for _ in range(nu_post):
+                X = X + omega * dinv * (B - J @ X)
+        if top and self.ns:
+            X = X - X.mean(0)
+        return X
+
+
+@pytest.mark.parametrize("ns", [False, True])
+def test_vcycle_jacobian3_equals_dense_textbook(hier_tiny, ns):
+    """The oracle's V-cycle with Jacobian3 level operators (including the element-by-element
+    coarsest Type3 assembly) against the dense V-cycle on Galerkin products; negative control:
+    the Jacobian2 V-cycle differs."""
+    h = hier_tiny
+    coefs = var_coefficients(h, mass_base=None if ns else 1.0)
+    lv = [(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)]
+    d3 = DenseMGJac3(h, coefs, ns)
+    B = _probe(h.fine.n_node, 5)
+    if ns:
+        B = B - B.mean(0)
+    mg3 = oracle.MG(lv, omega=0.6, nu_pre=1, nu_post=3, null_space=ns, jac3=True)
+    Z3 = _oracle_vcycle(mg3, B)
+    Zd = d3.vcycle(B, 0.6, 1, 3)
+    assert np.abs(Z3 - Zd).max() <= 1e-11 * np.abs(Zd).max()
+    Z2 = _oracle_vcycle(oracle.MG(lv, omega=0.6, nu_pre=1, nu_post=3, null_space=ns), B)
+    assert np.abs(Z2 - Zd).max() > 1e-4 * np.abs(Zd).max()
+
+
+# ------------------------------------------------------------------ right-hand sides and errors
+POLY = [0.3, 1.0, -2.0, 0.5, 1.5, -0.7, 0.25, 2.0]
+
+
+def test_function_special_values():
+    """The Gaussians of ComputeRHS2 are 1 at their centres, cut at their boxes (the first box
+    open at 0.2, the others closed), and the cosine sum is the closed form at chosen points."""
+    G = 0
+    assert oracle.eval_fn(G, 0.1, 0.1, 0.1) == 1.0
+    assert oracle.eval_fn(G, 0.3, 0.3, 0.3) == 1.0
+    assert oracle.eval_fn(G, 0.6, 0.6, 0.6) == 1.0
+    assert oracle.eval_fn(G, 0.2, 0.2, 0.2) == pytest.approx(np.exp(-0.03 / 0.005), rel=1e-15)
+    assert oracle.eval_fn(G, 0.25, 0.1, 0.1) == 0.0
+    assert oracle.eval_fn(G, 0.45, 0.45, 0.45) == 0.0
+    assert oracle.eval_fn(1, 0.0, 0.0, 0.0, nfreq=9, a0=1.0, a1=0.0) == pytest.approx(9.0, rel=1e-15)
+    v = sum((2 + 3 * 3 * k * k * np.pi ** 2) * np.cos(k * np.pi * 0.3) * np.cos(k * np.pi * 0.7) *
+            np.cos(k * np.pi * 0.1) for k in range(1, 5))
+    assert oracle.eval_fn(1, 0.3, 0.7, 0.1, nfreq=4, a0=2.0, a1=3.0) == pytest.approx(v, rel=1e-13)
+
+
+def test_quadrature_rhs_and_l2_error_exact_on_trilinear(tiny):
+    """On an adaptive mesh with hanging vertices: the quadrature right-hand side of a trilinear
+    g is M g (g lies in the finite element space and 3-point Gauss is exact for the products),
+    so f = 1 gives the mass matrix's row sums; nodal values of g are g at the node
+    coordinates; the L2 error of g's own nodal values is zero and that of u = 0 is
+    sqrt(g^T M g)."""
+    from tests.meshes import node_coords
+    L = tiny.fine
+    assert L.n_hanging > 0
+    xyz = node_coords(L)
+    g = oracle.nodal_values(L, 2, poly=POLY)
+    gx = (POLY[0] + POLY[1] * xyz[:, 0] + POLY[2] * xyz[:, 1] + POLY[3] * xyz[:, 2] + POLY[4] * xyz[:, 0] * xyz[:, 1]
+          + POLY[5] * xyz[:, 1] * xyz[:, 2] + POLY[6] * xyz[:, 2] * xyz[:, 0] + POLY[7] * np.prod(xyz, 1))
+    assert np.abs(g - gx).max() <= 1e-14
+    Mg = oracle.matvec(L, g, lap_fac=0.0, mass_fac=1.0)
+    b = oracle.rhs_quadrature(L, 2, poly=POLY)
+    assert np.abs(b - Mg).max() <= 1e-14 * np.abs(Mg).max()
+    one = [1.0] + [0.0] * 7
+    b1 = oracle.rhs_quadrature(L, 2, poly=one)
+    M1 = oracle.matvec(L, np.ones(L.n_node), lap_fac=0.0, mass_fac=1.0)
+    assert np.abs(b1 - M1).max() <= 1e-15 and abs(b1.sum() - 1.0) <= 1e-14
+    assert oracle.l2_error(L, g, 2, poly=POLY) <= 1e-14
+    assert oracle.l2_error(L, np.zeros(L.n_node), 2, poly=POLY) == pytest.approx(np.sqrt(g @ Mg), rel=1e-13)
+
+
+def test_cosine_sum_norm_and_convergence():
+    """||sum_{k<=3} cos(k pi x) cos(k pi y) cos(k pi z)||_L2 = sqrt(3/8) (orthogonal terms), to
+    quadrature accuracy on a uniform mesh; the Helmholtz solve with the quadrature right-hand
+    side of (1 + 3 k^2 pi^2) cos cos cos converges to it at second order in h."""
+    errs = []
+    for d in (3, 4, 5):
+        n = 1 << d
+        c = (np.arange(n) + 0.5) / n
+        X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+        damg = da.DAMG(da.Octree.from_points(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).ravel(), d), 3, 20)
+        Ls = [damg.level(l) for l in range(damg.nlevels)]
+        F = Ls[-1]
+        assert F.n_elem == n ** 3
+        if d == 5:
+            assert oracle.l2_error(F, np.zeros(F.n_node), 1, nfreq=3) == pytest.approx(np.sqrt(3 / 8), rel=1e-6)
+        b = oracle.rhs_quadrature(F, 1, nfreq=2, a0=1.0, a1=1.0)
+        x, its, conv, _ = oracle.MG([(L, None, None, 1.0, 1.0) for L in Ls]).pcg(b, rtol=1e-12, max_it=100)
+        assert conv
+        errs.append(oracle.l2_error(F, x, 1, nfreq=2))
+    rates = [errs[i] / errs[i + 1] for i in range(2)]
+    assert all(3.8 <= r <= 4.3 for r in rates), rates
+
+
+class DenseMGDirichlet(DenseMG):
+    """DenseMG for the Dirichlet problem: A_l the Dirichlet-masked operators (identity rows and
+    columns on boundary nodes), P_l with every entry that touches a boundary node removed."""
+
+    def __init__(self, h, coefs):
+        self.h, self.ns = h, False
+        self.bdy = [h.damg.boundary(l).astype(bool) for l in range(h.n)]
+        self.A = []
+        for l in range(h.n):
+            L, (k, m), bd = h.levels[l], coefs[l], self.bdy[l]
+            self.A.append(_sparse_from_columns(
+                lambda e, L=L, k=k, m=m, bd=bd: oracle.matvec_dirichlet(L, bd, e, kappa=k, mass=m), L.n_node))
+        self.P = []
+        for l in range(1, h.n):
+            C, F = h.levels[l - 1], h.levels[l]
+            P = _sparse_from_columns(lambda e, C=C, F=F: oracle.prolong(C, F, e), C.n_node, F.n_node).tolil()
+            P[np.nonzero(self.bdy[l])[0], :] = 0.0
+            P[:, np.nonzero(self.bdy[l - 1])[0]] = 0.0
+            self.P.append(P.tocsr())
+        self.A0inv = np.linalg.inv(self.A[0].toarray())
+
+
+def test_vcycle_dirichlet_equals_dense_textbook(hier_tiny):
+    """The oracle's Dirichlet V-cycle (masked operators and diagonals, suppressed-DOF transfers,
+    identity rows in the coarsest matrix) against the dense textbook V-cycle on the masked
+    matrices; negative control: unmasked transfers differ."""
+    h = hier_tiny
+    coefs = var_coefficients(h, mass_base=1.0)
+    lv = [(h.levels[l], coefs[l][0], coefs[l][1], 1.0, 1.0) for l in range(h.n)]
+    dd = DenseMGDirichlet(h, coefs)
+    B = _probe(h.fine.n_node, 6)
+    mg = oracle.MG(lv, omega=0.6, nu_pre=2, nu_post=1, boundary=dd.bdy)
+    Z = _oracle_vcycle(mg, B)
+    Zd = dd.vcycle(B, 0.6, 2, 1)
+    assert np.abs(Z - Zd).max() <= 1e-11 * np.abs(Zd).max()
+    P_full = [_sparse_from_columns(lambda e, C=h.levels[l - 1], F=h.levels[l]: oracle.prolong(C, F, e),
+                                   h.levels[l - 1].n_node, h.levels[l].n_node) for l in range(1, h.n)]
+    dd.P = P_full
+    assert np.abs(dd.vcycle(B, 0.6, 2, 1) - Zd).max() > 1e-6 * np.abs(Zd).max()
+    # a Dirichlet PCG solve returns x = b on the boundary nodes and satisfies the masked system
+    b = np.random.default_rng(3).uniform(-1, 1, h.fine.n_node)
+    x, its, conv, _ = mg.pcg(b, rtol=1e-12, max_it=100)
+    assert conv
+    bd = dd.bdy[-1]
+    assert np.abs(x[bd] - b[bd]).max() <= 1e-10
+    assert np.linalg.norm(dd.A[-1] @ x - b) <= 1e-10 * np.linalg.norm(b)
+
+
+# ------------------------------------------------------------------ elasticity
+def _q1_graddiv():
+    """Regular-element grad-div block [3k+a][3j+b] = int_{[-1,1]^3} d_a phi_k d_b phi_j in closed
+    form: the 1-D factors are K1 (derivative-derivative), M1 (value-value) and
+    D1 = int phi_p dphi_q, one derivative per direction a / b."""
+    D1 = np.array([[-0.5, 0.5], [-0.5, 0.5]])   # int phi_p phi_q' on [-1, 1]
+    G = np.zeros((24, 24))
+    for a in range(3):
+        for b in range(3):
+            f = []
+            for d in range(3):   # Kronecker order z, y, x (vertex bit 0 = x)
+                if d == a and d == b:
+                    f.append(K1)
+                elif d == a:
+                    f.append(D1.T)
+                elif d == b:
+                    f.append(D1)
+                else:
+                    f.append(M1)
+            blk = np.kron(np.kron(f[2], f[1]), f[0])
+            G[a::3, b::3] = blk
+    return G
+
+
+def test_elasticity_stencils_and_operator(tiny):
+    """or_stencils_graddiv against the reference's GDType2Stencils.inp (fixture) and the
+    closed-form regular element; the elasticity product (mu Laplacian on every component plus
+    (mu + lambda) grad-div) against the constrained vector-Q1 assembly on an adaptive mesh;
+    translations in its kernel; with Dirichlet flags, identity rows and dropped columns."""
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "stencils_ref2.npz"))
+    gd = oracle.stencils_graddiv()
+    assert np.abs(gd - ref["gd2"]).max() <= 1e-15
+    assert np.abs(gd[0, 0] - _q1_graddiv()).max() <= 1e-15
+    L = tiny.fine
+    cons = Constrained(tiny, tiny.n - 1)
+    N = L.n_node
+    mu, lam = 0.7, 1.9
+    A = np.zeros((3 * N, 3 * N))
+    unit = 1 << (L.max_depth - 1)
+    Ke0 = np.kron(Q1_LAP, np.eye(3)) * mu + _q1_graddiv() * (mu + lam)
+    for e, (a, s) in enumerate(cons.elems):
+        rows = cons.element_rows(a, s)
+        nodes = sorted({n for w in rows for n in w})
+        P = np.zeros((8, len(nodes)))
+        for k, w in enumerate(rows):
+            for n, v in w.items():
+                P[k, nodes.index(n)] += v
+        P3 = np.kron(P, np.eye(3))
+        ix = (3 * np.array(nodes)[:, None] + np.arange(3)).ravel()
+        A[np.ix_(ix, ix)] += (s / unit) / 2.0 * (P3.T @ Ke0 @ P3)
+    Ao = _sparse_from_columns(lambda v: oracle.matvec_elasticity(L, v, mu, lam), 3 * N).toarray()
+    assert np.abs(Ao - A).max() <= 1e-13 * np.abs(A).max()
+    for d in range(3):
+        t = np.zeros((N, 3))
+        t[:, d] = 1.0
+        assert np.abs(Ao @ t.ravel()).max() <= 1e-14 * np.abs(A).max()
+    bdy = tiny.damg.boundary(tiny.n - 1).astype(bool)
+    b3 = np.repeat(bdy, 3)
+    Ad = np.where(b3[:, None] | b3[None, :], 0.0, A)
+    Ad[b3, b3] = 1.0
+    x = np.random.default_rng(4).uniform(-1, 1, 3 * N)
+    yd = oracle.matvec_elasticity(L, x, mu, lam, bdy=bdy)
+    assert np.abs(yd - Ad @ x).max() <= 1e-12 * np.abs(A).max()
+    assert np.array_equal(yd[b3], x[b3])
