@@ -37,6 +37,10 @@ WORKLOADS = {
                coef="helmholtz", null_space=0, rhs_seed=12,
                desc="Helmholtz (Jacobian1, lambda=mu=1) MatVec + 8-level GMG-PCG, octree from 8M points "
                     "(4M uniform + 4M in 8 Gaussian clusters), max depth 14, FP64"),
+    "c5": dict(name="clusters4M_d13", kind="CLUSTERS", npts=4000000, seed=4, cluster_seed=51, depth=13, nlevels=8,
+               coef="contrast1e3", null_space=1, rhs_seed=22,
+               desc="Jacobian2 PCG with learned damping, octree from 4M points in 8 Gaussian clusters, max depth "
+                    "13, log-normal kappa with contrast 1e3, zero mass, constant null space, FP32 V-cycle"),
     "c2": dict(name="lognormal1M_d12", kind="LOGNORMAL", npts=1000000, seed=2, cluster_seed=0, depth=12,
                nlevels=5, coef="lognormal", null_space=1, rhs_seed=22,
                desc="Jacobian2 MatVec + 5-level GMG-PCG, octree from 1M log-normal points, max depth 12, "
@@ -60,8 +64,12 @@ class Problem:
         del pts
         self.levels = [self.damg.level(l) for l in range(self.damg.nlevels)]
         F = self.levels[-1]
-        if w["coef"] == "lognormal":
-            kf = 10.0 ** np.random.default_rng(21).uniform(-1, 1, F.n_elem)
+        if w["coef"] in ("lognormal", "contrast1e3"):
+            if w["coef"] == "lognormal":
+                kf = 10.0 ** np.random.default_rng(21).uniform(-1, 1, F.n_elem)
+            else:   # kappa = exp(sigma z), z ~ N(0,1) seed 52, clipped to [10^-1.5, 10^1.5]
+                z = np.random.default_rng(52).standard_normal(F.n_elem)
+                kf = np.clip(np.exp(np.log(10 ** 1.5) / 2.0 * z), 10 ** -1.5, 10 ** 1.5)
             self.coefs = [(self.damg.coarsen_coefficients(kf, None, l)[0], None) for l in range(self.damg.nlevels)]
         else:
             self.coefs = [(None, None)] * self.damg.nlevels
@@ -283,6 +291,35 @@ def bench_pcg(P, dev, learned=True):
     return out
 
 
+def bench_c5(dev, args):
+    """Config 5: learned damping against plain Jacobi on the 4M cluster octree (kappa contrast
+    1e3), FP32 V-cycle, PCG to 1e-8: iterations and time of both, training time."""
+    import torch
+    import dendro_amd as da
+    P = Problem(WORKLOADS["c5"])
+    choose_tilings(P, argparse.Namespace(tile=0, block=0, autotune_all=False))
+    N = P.fine.n_node
+    b = torch.tensor(np.random.default_rng(22).uniform(-1, 1, N), dtype=torch.float64, device=dev)
+    out = {"workload": P.w["desc"], "n_elem": P.fine.n_elem, "n_node": N}
+    for name, kw in (("jacobi", {}), ("learned", {"smoother": da.SMOOTHER_LEARNED})):
+        s = da.Solver(P.meshes, P.trans, P.params(vcycle_prec=da.F32, **kw))
+        extra = {}
+        if name == "learned":
+            s.learned_init(1)
+            tr = s.learned_train(steps=200, samples=8, nu=2, refresh=1, lr=3e-3, seed=5000)
+            extra = {"levels_learned_mask": s.learned_apply(True, samples=8, seed=6000),
+                     "train_seconds": round(tr.seconds, 2), "objective_first": round(tr.loss_first, 5),
+                     "objective_last": round(tr.loss_last, 5)}
+        x = torch.zeros_like(b)
+        s.solve(b, x)
+        st, _ = s.solve(b, x)
+        out[name] = {"seconds": round(st.seconds, 5), "iterations": st.iterations, "rel_true": st.rel_true,
+                     "converged": bool(st.converged), **extra}
+        log(f"c5 {name}: {st.iterations} its, {st.seconds*1e3:.2f} ms {extra}")
+        del s
+    return out
+
+
 def bench_multi(P, dev, stream, steps):
     """32 right-hand sides (seeds 400..431): the batched FP32 operator and the batched PCG
     (one FP32 V-cycle per iteration for all columns, FP64 outer CG, one hipGraph)."""
@@ -342,30 +379,38 @@ def measured_traffic(workload, kernel, tile_elems, block):
     return t.get("bytes_per_launch")
 
 
+def choose_tilings(P, args):
+    """Operator tilings of every level; returns the finest level's choice.  The autotuner
+    rebuilds the host tiling once per candidate: seconds per candidate on meshes of tens of
+    millions of elements.  Above AUTOTUNE_MAX_ELEMS (unless --autotune-all) a level takes the
+    tiling the autotuner chose for the 44.3M-element finest level of the 8M mixed octree
+    (4096 elements x 512 threads, 741 us per FP64 MatVec in its search)."""
+    import dendro_amd as da
+    mf = P.meshes[-1]
+    if args.tile:
+        for l, m in enumerate(P.meshes[:-1]):
+            m.autotune(da.F64) if P.levels[l].n_elem <= AUTOTUNE_MAX_ELEMS else m.set_tile(4096, 512)
+        mf.set_tile(args.tile, args.block)
+        return {"tile_elems": args.tile, "block": args.block or 512}
+    t0 = time.time()
+    for l, m in enumerate(P.meshes):
+        if P.levels[l].n_elem > AUTOTUNE_MAX_ELEMS and not args.autotune_all:
+            m.set_tile(4096, 512)
+            tuned = {"tile_elems": 4096, "block": 512, "fixed": True}
+        else:
+            tuned = m.autotune(da.F64)
+        log(f"  level {l} (E={P.levels[l].n_elem}): {tuned}  [{time.time() - t0:.1f}s]")
+    log(f"tilings chosen in {time.time() - t0:.1f}s: finest {tuned}")
+    return tuned
+
+
 def matvec_section(P, args, dev, stream):
     """Autotuned FP64 MatVec: K timed steps, cold-cache launches, FP32 rate, roofline."""
     import torch
     import dendro_amd as da
     F = P.fine
     mf = P.meshes[-1]
-    if args.tile:
-        mf.set_tile(args.tile, args.block)
-        tuned = {"tile_elems": args.tile, "block": args.block or 512}
-    else:
-        # The autotuner rebuilds the host tiling once per candidate: seconds per candidate on
-        # meshes of tens of millions of elements.  Above AUTOTUNE_MAX_ELEMS (unless
-        # --autotune-all) a level takes the tiling the autotuner chose for the 44.3M-element
-        # finest level of the 8M mixed octree (4096 elements x 512 threads, 741 us per FP64
-        # MatVec in its search).
-        t0 = time.time()
-        for l, m in enumerate(P.meshes):
-            if P.levels[l].n_elem > AUTOTUNE_MAX_ELEMS and not args.autotune_all:
-                m.set_tile(4096, 512)
-                tuned = {"tile_elems": 4096, "block": 512, "fixed": True}
-            else:
-                tuned = m.autotune(da.F64)
-            log(f"  level {l} (E={P.levels[l].n_elem}): {tuned}  [{time.time() - t0:.1f}s]")
-        log(f"tilings chosen in {time.time() - t0:.1f}s: finest {tuned}")
+    tuned = choose_tilings(P, args)
     ti = mf.tile_info()
     log(f"tiling: {ti}")
     x = torch.tensor(np.random.default_rng(11).uniform(-1, 1, F.n_node), dtype=torch.float64, device=dev)
@@ -416,6 +461,7 @@ def main():
     ap.add_argument("--no-pcg", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-damping PCG")
     ap.add_argument("--no-c2", action="store_true", help="skip the secondary 1M log-normal workload")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 4M cluster learned-damping comparison")
     ap.add_argument("--no-multi", action="store_true", help="skip the 32 right-hand-side batch (secondary)")
     ap.add_argument("--no-dirichlet", action="store_true", help="skip the Dirichlet-masked MatVec (secondary)")
     ap.add_argument("--matvec-only", action="store_true",
@@ -436,7 +482,7 @@ def main():
               "not sharded across GPUs in this build.", file=sys.stderr)
         sys.exit(2)
     if args.matvec_only:
-        args.no_cpu = args.no_pcg = args.no_c2 = True
+        args.no_cpu = args.no_pcg = args.no_c2 = args.no_c5 = True
     if args.cpu_table:
         cpu_table()
         return
@@ -451,8 +497,7 @@ def main():
     F = P.fine
 
     if args.pcg_trace:
-        for m in P.meshes:
-            m.autotune(da.F64)
+        choose_tilings(P, args)
         log("levels: " + ", ".join(f"{l}: E={L.n_elem} {P.meshes[l].tile_info()}" for l, L in enumerate(P.levels)))
         b = torch.tensor(np.random.default_rng(P.w["rhs_seed"]).uniform(-1, 1, F.n_node), dtype=torch.float64,
                          device=dev)
@@ -520,6 +565,9 @@ def main():
             c2["multi_rhs"] = bench_multi(Q, dev, stream, args.steps)
         out["c2"] = c2
         del Q
+
+    if not args.no_c5 and not args.no_pcg:
+        out["c5"] = bench_c5(dev, args)
 
     if not args.no_cpu:
         log("CPU baseline ...")

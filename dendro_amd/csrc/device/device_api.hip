@@ -453,8 +453,8 @@ static void cg_iteration(dgh_solver s, hipStream_t st) {
   else
     hipLaunchKernelGGL(cg_p_fused_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
                        static_cast<const double*>(s->zres), s->p.p);
-  // alpha from the partials of p.w is formed inside the update kernel: the operator's tile
-  // partials, or (Dirichlet problem) the block partials of a separate dot pass
+  // alpha from the partials of p.w: the operator's tile partials, or (Dirichlet problem) the
+  // block partials of a separate dot pass
   int nt = int(m->op.plan.tiles.size());
   if (s->dirichlet) {
     launch_dirmv<double>(m, s->p.p, s->w.p, st);
@@ -467,16 +467,16 @@ static void cg_iteration(dgh_solver s, hipStream_t st) {
     launch_cg(m, s->p.p, s->w.p, s->ptile.p, st);
   }
   const bool x1 = s->nlevels > 1 && s->prm.nu_pre > 0;   // first sweep of the V-cycle fused
+  hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED2_THREADS), 0, st, nt, (const double*)s->ptile.p, S);
   if (mixed)
-    hipLaunchKernelGGL(cg_xr_fused_kernel<float>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, nt,
-                       (const double*)s->ptile.p, S, (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p,
-                       s->v32[top].b.p, float(s->om_level[top]), (const float*)s->v32[top].invd.p,
+    hipLaunchKernelGGL(cg_xr_fused_kernel<float>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
+                       (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p, s->v32[top].b.p,
+                       float(s->om_level[top]), (const float*)s->v32[top].invd.p,
                        x1 ? s->v32[top].xa.p : s->v32[top].xb.p);
   else
-    hipLaunchKernelGGL(cg_xr_fused_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, nt,
-                       (const double*)s->ptile.p, S, (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p,
-                       (float*)nullptr, s->om_level[top], (const double*)s->v64[top].invd.p,
-                       x1 ? s->v64[top].xa.p : s->v64[top].xb.p);
+    hipLaunchKernelGGL(cg_xr_fused_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, st, n, (const double*)S,
+                       (const double*)s->p.p, (const double*)s->w.p, s->x.p, s->r.p, (float*)nullptr,
+                       s->om_level[top], (const double*)s->v64[top].invd.p, x1 ? s->v64[top].xa.p : s->v64[top].xb.p);
   const void* z = precondition_raw(s, s->r.p, st, x1);
   if (z != s->zres) fail(DGH_ERR_STATE, "V-cycle result buffer changed");
   if (mixed)

@@ -3,9 +3,14 @@
 //   y_k = sum_j fac (mu L[k][j] I + (mu + lambda) GD[3k..3k+2][3j..3j+2]) x_j,  fac = h/2,
 // with optional Dirichlet flags (boundary rows y = x, boundary columns dropped).
 // Two passes, no atomics:
-//   1. one thread per element, elements sorted by stencil type so that every wavefront holds
-//      one type: the 8x8 Laplacian and 24x24 grad-div rows are wave-uniform loads (scalar
-//      cache); the 24 element outputs go to ye[i][24] (i = sorted position);
+//   1. one thread per element.  Child c's stencils are child 0's of the mirrored hanging type
+//      t' with the vertex order k^c and, for grad-div, the signs s_a s_b (s_a = -1 when bit a
+//      of c is set: the mirror flips axis a), so the element takes its inputs as
+//      xe[3k+a] = s_a x[3 idx[k^c] + a], applies child 0's stencils of type t' and returns
+//      s_a out[3k+a] to vertex k^c.  Elements are sorted by t' inside chunks of ELAS_CHUNK
+//      Morton-consecutive elements (the chunk's nodes stay in L2), and every wavefront holds
+//      one t': the 8x8 Laplacian and 24x24 grad-div rows are wave-uniform loads.  The 24
+//      outputs go to ye[i][24] (i = sorted position, vertex order of the element);
 //   2. one thread per node: y[3n+d] = sum over the node's (element, vertex) slots of ye, in a
 //      fixed order (a CSR built on the host).
 #pragma once
@@ -17,13 +22,16 @@
 namespace dgh {
 
 constexpr int ELAS_WAVE = 64;
+constexpr uint32_t ELAS_CHUNK = 16384;
 
-// seg[w] = {type, first sorted position, count} of wavefront w (count <= 64)
+// seg[w] = {mirrored type t', first sorted position, count} of wavefront w (count <= 64);
+// L0 [18][64] and GD0 [18][576]: child 0's stencils
 __global__ void __launch_bounds__(256)
 elas_elem_kernel(int nseg, const int4* __restrict__ seg, const uint32_t* __restrict__ order,
-                 const uint32_t* __restrict__ nl, const uint8_t* __restrict__ lev, uint32_t maxD, double hFac,
-                 double mu, double lam, const double* __restrict__ L2, const double* __restrict__ GD,
-                 const uint8_t* __restrict__ bdy, const double* __restrict__ x, double* __restrict__ ye) {
+                 const uint32_t* __restrict__ nl, const uint8_t* __restrict__ lev, const uint8_t* __restrict__ child,
+                 uint32_t maxD, double hFac, double mu, double lam, const double* __restrict__ L0,
+                 const double* __restrict__ GD0, const uint8_t* __restrict__ bdy, const double* __restrict__ x,
+                 double* __restrict__ ye) {
   const int w = int(blockIdx.x) * (256 / ELAS_WAVE) + int(threadIdx.x) / ELAS_WAVE;
   if (w >= nseg) return;
   const int lane = threadIdx.x % ELAS_WAVE;
@@ -32,20 +40,26 @@ elas_elem_kernel(int nseg, const int4* __restrict__ seg, const uint32_t* __restr
   const int type = __builtin_amdgcn_readfirstlane(sg.x);
   const uint32_t i = uint32_t(sg.y + lane);
   const uint32_t e = order[i];
-  const uint4 a = reinterpret_cast<const uint4*>(nl)[2 * e];
-  const uint4 b = reinterpret_cast<const uint4*>(nl)[2 * e + 1];
-  const uint32_t idx[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  const uint32_t c = child[e];
+  const uint4 a4 = reinterpret_cast<const uint4*>(nl)[2 * e];
+  const uint4 b4 = reinterpret_cast<const uint4*>(nl)[2 * e + 1];
+  const uint32_t idx[8] = {a4.x, a4.y, a4.z, a4.w, b4.x, b4.y, b4.z, b4.w};
+  const double sg0 = (c & 1u) ? -1.0 : 1.0, sg1 = (c & 2u) ? -1.0 : 1.0, sg2 = (c & 4u) ? -1.0 : 1.0;
   double xe[24];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const bool keep = !(bdy && bdy[idx[k]]);
+    uint32_t v = idx[0];   // idx[k ^ c] without dynamic register indexing
 #pragma unroll
-    for (int d = 0; d < 3; ++d) xe[3 * k + d] = keep ? x[3 * size_t(idx[k]) + d] : 0.0;
+    for (int q = 1; q < 8; ++q) v = (uint32_t(q) == (uint32_t(k) ^ c)) ? idx[q] : v;
+    const bool keep = !(bdy && bdy[v]);
+    xe[3 * k] = keep ? sg0 * x[3 * size_t(v)] : 0.0;
+    xe[3 * k + 1] = keep ? sg1 * x[3 * size_t(v) + 1] : 0.0;
+    xe[3 * k + 2] = keep ? sg2 * x[3 * size_t(v) + 2] : 0.0;
   }
   const double fac = hFac * double(1u << (maxD - lev[e])) / 2.0;
   const double f1 = mu * fac, f2 = (mu + lam) * fac;
-  const double* __restrict__ Lt = L2 + size_t(type) * 64;
-  const double* __restrict__ Gt = GD + size_t(type) * 576;
+  const double* __restrict__ Lt = L0 + size_t(type) * 64;
+  const double* __restrict__ Gt = GD0 + size_t(type) * 576;
   double* __restrict__ out = ye + size_t(i) * 24;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -64,9 +78,10 @@ elas_elem_kernel(int nseg, const int4* __restrict__ seg, const uint32_t* __restr
       g1 += Gt[(3 * k + 1) * 24 + q] * xe[q];
       g2 += Gt[(3 * k + 2) * 24 + q] * xe[q];
     }
-    out[3 * k] = f1 * s0 + f2 * g0;
-    out[3 * k + 1] = f1 * s1 + f2 * g1;
-    out[3 * k + 2] = f1 * s2 + f2 * g2;
+    const uint32_t ko = uint32_t(k) ^ c;   // the element's own vertex of mirrored vertex k
+    out[3 * ko] = sg0 * (f1 * s0 + f2 * g0);
+    out[3 * ko + 1] = sg1 * (f1 * s1 + f2 * g1);
+    out[3 * ko + 2] = sg2 * (f1 * s2 + f2 * g2);
   }
 }
 

@@ -125,7 +125,7 @@ struct ElasState {
   int nseg = 0;
   DevBuf<uint32_t> order, rowptr, slot;
   DevBuf<int4> seg;
-  DevBuf<uint8_t> lev;
+  DevBuf<uint8_t> lev, child;
   DevBuf<double> ye;            // [E][24] element outputs
 };
 }  // namespace dgh

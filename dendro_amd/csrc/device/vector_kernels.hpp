@@ -94,7 +94,7 @@ reduce2_kernel(int nq, const double* __restrict__ part, double* __restrict__ out
   }
 }
 
-// out[0] = part[0] + ... + part[n-1] in the order cg_xr_fused_kernel sums the operator's tile
+// out[0] = part[0] + ... + part[n-1] in the order cg_alpha_kernel sums the operator's tile
 // partials (one workgroup of RED2_THREADS): the same value CG forms for p.w.
 static __global__ void __launch_bounds__(RED2_THREADS)
 sum_partials_kernel(int n, const double* __restrict__ part, double* __restrict__ out) {
@@ -291,33 +291,34 @@ __global__ void cg_p_fused_kernel(uint32_t n, const double* __restrict__ S, cons
   if (i < n) p[i] = (double(z[i]) - S[S_MU]) + S[S_RATIO] * p[i];
 }
 
-// alpha = rz / p.w with p.w the sum of the operator's tile partials, formed by every
-// workgroup in the same fixed order (one RED2_THREADS-thread pass);
-// workgroup 0 stores p.w and alpha.  Then x = x + alpha p;  r = r + (-alpha) w;  and the
-// V-cycle's first smoothing sweep from zero, x1 = omega (r invd), in the V-cycle's precision
-// TV (mixed: also the FP32 copy of r, r32).
-static_assert(VBLOCK == RED2_THREADS, "one partial sum pass per update workgroup");
-template <class TV>
-__global__ void __launch_bounds__(VBLOCK)
-cg_xr_fused_kernel(uint32_t n, int ntiles, const double* __restrict__ part, double* __restrict__ S,
-                   const double* __restrict__ p, const double* __restrict__ w, double* __restrict__ x,
-                   double* __restrict__ r, float* __restrict__ r32, TV omega, const TV* __restrict__ invd,
-                   TV* __restrict__ x1) {
+// p.w = the sum of the operator's tile partials (the order of sum_partials_kernel), and
+// alpha = rz / p.w, into S (grid: one workgroup of RED2_THREADS)
+static __global__ void __launch_bounds__(RED2_THREADS)
+cg_alpha_kernel(int ntiles, const double* __restrict__ part, double* __restrict__ S) {
   __shared__ double red[RED2_THREADS / 64];
   double v = 0.0;
   for (int k = threadIdx.x; k < ntiles; k += RED2_THREADS) v += part[k];
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  double pw = 0.0;
-  for (int k = 0; k < RED2_THREADS / 64; ++k) pw += red[k];
-  const double a = S[S_RZ] / pw;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (threadIdx.x == 0) {
+    double pw = 0.0;
+    for (int k = 0; k < RED2_THREADS / 64; ++k) pw += red[k];
     S[S_PW] = pw;
-    S[S_ALPHA] = a;
+    S[S_ALPHA] = S[S_RZ] / pw;
   }
+}
+
+// x = x + alpha p;  r = r + (-alpha) w;  and the V-cycle's first smoothing sweep from zero,
+// x1 = omega (r invd), in the V-cycle's precision TV (mixed: also the FP32 copy of r, r32)
+template <class TV>
+__global__ void __launch_bounds__(VBLOCK)
+cg_xr_fused_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ p,
+                   const double* __restrict__ w, double* __restrict__ x, double* __restrict__ r,
+                   float* __restrict__ r32, TV omega, const TV* __restrict__ invd, TV* __restrict__ x1) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const double a = S[S_ALPHA];
   x[i] = x[i] + a * p[i];
   const double ri = r[i] + (-a) * w[i];
   r[i] = ri;

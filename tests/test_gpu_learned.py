@@ -132,6 +132,9 @@ def test_training_gain_and_fallback(setup, dev):
     print(f"PCG to 1e-8: Jacobi {st_j.iterations} its {st_j.seconds*1e3:.2f} ms, learned (levels {mask:#x}) "
           f"{st_l.iterations} its {st_l.seconds*1e3:.2f} ms, rel_true {st_l.rel_true:.2e}")
     assert st_l.converged and st_l.rel_true <= 1.05e-8
+    # the gain: fewer PCG iterations than plain damped Jacobi on this high-contrast mesh
+    # (training and sampling are seeded and every reduction runs in a fixed order)
+    assert st_l.iterations < st_j.iterations
     # fallback: disabling restores the Jacobi solve exactly (same iterations)
     s.learned_apply(False)
     x.zero_()
