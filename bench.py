@@ -101,10 +101,13 @@ class Problem:
         F = self.fine
         return F.n_elem * (32 + 2) + F.n_node * 2 * b_v + (0 if self.uniform else F.n_elem * b_v)
 
-    def kernel_name(self, block, lane):
+    def kernel_name(self, block, rows):
+        """The FP64 MatVec kernel instantiation for a stencil-row source (da.ROWS_*)."""
         mass = "true" if self.uniform else "false"
         uni = "true" if self.uniform else "false"
-        return f"dgh::tile_op_kernel<double, double, {mass}, 0, {block}, {uni}, {'true' if lane else 'false'}>"
+        lane = "true" if rows == 1 else "false"
+        comb = "true" if rows == 2 and self.uniform else "false"
+        return f"dgh::tile_op_kernel<double, double, {mass}, 0, {block}, {uni}, {lane}, {comb}>"
 
     def params(self, **kw):
         import dendro_amd as da

@@ -17,6 +17,7 @@ F64, F32 = 0, 1
 NORM_UNPRECONDITIONED, NORM_PRECONDITIONED = 0, 1
 SMOOTHER_JACOBI, SMOOTHER_LEARNED = 0, 1
 COARSE_JAC2, COARSE_JAC3 = 0, 1
+ROWS_TYPE, ROWS_LANE, ROWS_CLASS = 0, 1, 2   # DGH_ROWS_*: stencil-row source of the operator
 FN_GAUSSIANS, FN_COS_SUM, FN_POLY = 0, 1, 2
 PTS_GAUSS, PTS_LOGNORMAL, PTS_MIXED, PTS_CLUSTERS, PTS_UNIFORM = range(5)
 NOALLOC_MAX_TILES = 16384     # DGH_NOALLOC_MAX_TILES
@@ -353,8 +354,9 @@ class Mesh:
                 "n_node_entries": nn.value, "lds_bytes_f64": lds.value, "lane_stencils": on.value}
 
     def set_lane_stencils(self, on):
-        """1: operator kernels read stencil rows per lane from an LDS copy of the table;
-        0: per stencil type of a wavefront through the scalar cache (default)."""
+        """Stencil-row source: ROWS_LANE (1) per lane from an LDS copy of the table;
+        ROWS_TYPE (0) per stencil type of a wavefront through the scalar cache; ROWS_CLASS (2)
+        the element matrix of each (type, level) class (uniform coefficients only)."""
         check(lib().dgh_mesh_set_lane_stencils(self.h, int(on)))
 
     def autotune(self, prec=F64):

@@ -27,8 +27,24 @@ namespace dgh {
 
 template <class T>
 static size_t op_lds_bytes(const dgh_mesh m) {
-  return m->lane_rows ? tile_lds_bytes<T>(m->op.block, m->op.plan.max_touch, true)
-                      : (sizeof(T) == 8 ? m->op.lds64 : m->op.lds32);
+  return m->lane_rows == DGH_ROWS_LANE ? tile_lds_bytes<T>(m->op.block, m->op.plan.max_touch, true)
+                                       : (sizeof(T) == 8 ? m->op.lds64 : m->op.lds32);
+}
+
+// Stencil-row source of a launch: class matrices need uniform coefficients (per-element ones
+// fall back to the per-type rows).
+static int op_rows_mode(const dgh_mesh m) {
+  return (m->lane_rows == DGH_ROWS_CLASS && m->op.rk64.p) ? DGH_ROWS_TYPE : m->lane_rows;
+}
+
+// stencil rows of an operator launch: the class matrices for uniform coefficients (unless
+// rows are read per lane from LDS), else the child-0 Laplacian and mass rows
+template <class T> const T* kctab(const DevTiles& d);
+template <> const double* kctab<double>(const DevTiles& d) { return d.kc64.p; }
+template <> const float* kctab<float>(const DevTiles& d) { return d.kc32.p; }
+template <class T>
+static const T* op_rows(dgh_mesh m, bool comb) {
+  return comb ? kctab<T>(m->op) : KFtab<T>();
 }
 
 template <class T> const T* rk(dgh_mesh m);
@@ -54,24 +70,27 @@ static void launch_op_b(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
   const size_t lds = op_lds_bytes<T>(m);
   const T* kap = rk<T>(m);
   const T* mas = rm<T>(m);
-#define DGH_OP_LAUNCH(MASS, UNI, LANE)                                                                        \
-  hipLaunchKernelGGL((tile_op_kernel<T, T, MASS, MODE, BLOCK, UNI, LANE>), dim3(grid), dim3(BLOCK), lds, st,      \
-                     m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,     \
-                     m->hFac, KFtab<T>(), m->op.node_tab.p, m->op.texp.p, io)
-  if (m->lane_rows) {
+#define DGH_OP_LAUNCH(MASS, UNI, LANE, COMB)                                                                   \
+  hipLaunchKernelGGL((tile_op_kernel<T, T, MASS, MODE, BLOCK, UNI, LANE, COMB>), dim3(grid), dim3(BLOCK), lds, st, \
+                     m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,      \
+                     m->hFac, op_rows<T>(m, COMB), m->op.node_tab.p, m->op.texp.p, io)
+  const int rows = op_rows_mode(m);
+  if (rows == DGH_ROWS_LANE) {
     if (!kap)
-      DGH_OP_LAUNCH(true, true, true);
+      DGH_OP_LAUNCH(true, true, true, false);
     else if (!mas)
-      DGH_OP_LAUNCH(false, false, true);
+      DGH_OP_LAUNCH(false, false, true, false);
     else
-      DGH_OP_LAUNCH(true, false, true);
+      DGH_OP_LAUNCH(true, false, true, false);
+  } else if (rows == DGH_ROWS_CLASS) {
+    DGH_OP_LAUNCH(true, true, false, true);
   } else {
     if (!kap)
-      DGH_OP_LAUNCH(true, true, false);
+      DGH_OP_LAUNCH(true, true, false, false);
     else if (!mas)
-      DGH_OP_LAUNCH(false, false, false);
+      DGH_OP_LAUNCH(false, false, false, false);
     else
-      DGH_OP_LAUNCH(true, false, false);
+      DGH_OP_LAUNCH(true, false, false, false);
   }
 #undef DGH_OP_LAUNCH
   HIP_CHECK(hipGetLastError());
@@ -200,7 +219,28 @@ void upload_coefs(dgh_mesh m, DevTiles& d) {
     te[t] = make_int2(std::max(e, -1022), v);
   }
   d.texp.upload(te.data(), te.size());
-  if (!per_elem) return;
+  d.kc64.release();
+  d.kc32.release();
+  if (!per_elem) {
+    // element matrices per record class, K = fac1 L + fac2 M with the factors elem_factors
+    // forms (each entry rounded once)
+    std::vector<double> kc(size_t(NCLASS) * 64, 0.0);
+    std::vector<uint8_t> seen(NCLASS, 0);
+    for (size_t r = 0; r < R; ++r) {
+      const uint32_t c = d.plan.meta[r];
+      if (c >= uint32_t(NCLASS)) fail(DGH_ERR_MESH, "record class %u out of range", c);
+      if (seen[c]) continue;
+      seen[c] = 1;
+      const double h = m->hFac * double(1u << (m->maxD - (c >> 5)));
+      const double f1 = m->lap_fac * h / 2.0, f2 = m->mass_fac * h * h * h / 8.0;
+      const double* KF = g.hKF.data() + size_t(c & 31u) * KFSTRIDE;
+      for (int q = 0; q < 64; ++q) kc[size_t(c) * 64 + q] = f1 * KF[q] + f2 * KF[64 + q];
+    }
+    d.kc64.upload(kc.data(), kc.size());
+    std::vector<float> kcf(kc.begin(), kc.end());
+    d.kc32.upload(kcf.data(), kcf.size());
+    return;
+  }
   std::vector<double> k(R);
   std::vector<float> kf(R);
   // per-record element factors, formed as elem_factors forms them
@@ -262,8 +302,10 @@ static uint32_t op_budget(int block) {
 // block size's node budget.
 static void plan_mesh(dgh_mesh m, uint32_t T) {
   build_devtiles(m, m->op, T, op_lds_f64, op_budget(m->block), m->block);
-  if (m->lane_rows && tile_lds_bytes<double>(m->block, m->op.plan.max_touch, true) > size_t(g.max_lds))
-    m->lane_rows = 0;   // no LDS room left for the stencil table
+  if (m->rows_auto)   // class matrices once the tiles fill the device several times over
+    m->lane_rows = m->op.plan.tiles.size() >= CLASS_ROWS_MIN_TILES ? DGH_ROWS_CLASS : DGH_ROWS_TYPE;
+  if (m->lane_rows == DGH_ROWS_LANE && tile_lds_bytes<double>(m->block, m->op.plan.max_touch, true) > size_t(g.max_lds))
+    m->lane_rows = DGH_ROWS_TYPE;   // no LDS room left for the stencil table
 }
 
 }  // namespace dgh
@@ -404,24 +446,27 @@ static void launch_cg_b(dgh_mesh m, const double* p, double* w, double* part, hi
   const OpIO<double> io{p, w, nullptr, nullptr, 0.0, part};
   const double* kap = m->op.rk64.p;
   const double* mas = m->op.rm64.p;
-#define DGH_CG_LAUNCH(MASS, UNI, LANE)                                                                         \
-  hipLaunchKernelGGL((tile_op_kernel<double, double, MASS, OP_CG, BLOCK, UNI, LANE>), dim3(grid), dim3(BLOCK), lds, \
-                     st, m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD, \
-                     m->hFac, KFtab<double>(), m->op.node_tab.p, m->op.texp.p, io)
-  if (m->lane_rows) {
+#define DGH_CG_LAUNCH(MASS, UNI, LANE, COMB)                                                                    \
+  hipLaunchKernelGGL((tile_op_kernel<double, double, MASS, OP_CG, BLOCK, UNI, LANE, COMB>), dim3(grid), dim3(BLOCK), \
+                     lds, st, m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac,      \
+                     m->maxD, m->hFac, op_rows<double>(m, COMB), m->op.node_tab.p, m->op.texp.p, io)
+  const int rows = op_rows_mode(m);
+  if (rows == DGH_ROWS_LANE) {
     if (!kap)
-      DGH_CG_LAUNCH(true, true, true);
+      DGH_CG_LAUNCH(true, true, true, false);
     else if (!mas)
-      DGH_CG_LAUNCH(false, false, true);
+      DGH_CG_LAUNCH(false, false, true, false);
     else
-      DGH_CG_LAUNCH(true, false, true);
+      DGH_CG_LAUNCH(true, false, true, false);
+  } else if (rows == DGH_ROWS_CLASS) {
+    DGH_CG_LAUNCH(true, true, false, true);
   } else {
     if (!kap)
-      DGH_CG_LAUNCH(true, true, false);
+      DGH_CG_LAUNCH(true, true, false, false);
     else if (!mas)
-      DGH_CG_LAUNCH(false, false, false);
+      DGH_CG_LAUNCH(false, false, false, false);
     else
-      DGH_CG_LAUNCH(true, false, false);
+      DGH_CG_LAUNCH(true, false, false, false);
   }
 #undef DGH_CG_LAUNCH
   HIP_CHECK(hipGetLastError());
@@ -702,6 +747,7 @@ int dgh_init(int device) {
       }
     }
     g.KF64.upload(KF.data(), KF.size());
+    g.hKF = KF;
     std::vector<float> KFf(KF.begin(), KF.end());
     g.KF32.upload(KFf.data(), KFf.size());
     g.ready = true;
@@ -810,11 +856,13 @@ int dgh_mesh_set_lane_stencils(dgh_mesh m, int on) {
   return guarded([&] {
     require_init();
     if (!m) fail(DGH_ERR_ARG, "NULL mesh");
-    if (on != 0 && on != 1) fail(DGH_ERR_ARG, "lane stencils: 0 or 1");
-    if (on && tile_lds_bytes<double>(m->block, m->op.plan.max_touch, true) > size_t(g.max_lds))
+    if (on != DGH_ROWS_TYPE && on != DGH_ROWS_LANE && on != DGH_ROWS_CLASS)
+      fail(DGH_ERR_ARG, "stencil rows: DGH_ROWS_TYPE, DGH_ROWS_LANE or DGH_ROWS_CLASS");
+    if (on == DGH_ROWS_LANE && tile_lds_bytes<double>(m->block, m->op.plan.max_touch, true) > size_t(g.max_lds))
       fail(DGH_ERR_ARG, "the tiling leaves no LDS room for the stencil table");
     ++m->gen;   // captured graphs hold the kernel choice
     m->lane_rows = on;
+    m->rows_auto = false;
   });
 }
 
@@ -861,7 +909,9 @@ int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* bloc
           continue;
         }
         if (m->op.plan.tile_elems != T) continue;   // reduced to fit LDS: tried already
-        for (int L : {0, 1}) {   // stencil rows per wavefront type, or per lane from LDS
+        // stencil rows per wavefront type, per lane from LDS, per (type, level) class
+        for (int L : {DGH_ROWS_TYPE, DGH_ROWS_LANE, DGH_ROWS_CLASS}) {
+          if (L == DGH_ROWS_CLASS && !m->h_kappa.empty()) continue;   // uniform coefficients only
           m->lane_rows = L;
           if (op_lds_bytes<double>(m) > size_t(g.max_lds)) continue;
           auto run = [&] {
@@ -891,6 +941,7 @@ int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* bloc
     if (!bt) fail(DGH_ERR_STATE, "no tiling candidate fits");
     m->block = int(bb);
     m->lane_rows = bl;
+    m->rows_auto = false;
     plan_mesh(m, bt);
     if (tile_elems) *tile_elems = bt;
     if (block) *block = bb;

@@ -63,6 +63,9 @@ static inline unsigned blocks_for(size_t n, int bs = VBLOCK) { return unsigned((
 
 // ---------------------------------------------------------------- library state
 constexpr int OP_LDS_PER_CU = 160 * 1024;   // LDS of one CU (gfx950)
+// Default stencil-row source: the class matrices from this many tiles up (four workgroups per
+// CU); on fewer tiles the launch is latency-bound and the fewer passes of per-type rows win.
+constexpr size_t CLASS_ROWS_MIN_TILES = 1024;
 
 struct State {
   bool ready = false;
@@ -72,6 +75,7 @@ struct State {
   DevBuf<float> K32, W32;
   DevBuf<double> KF64;            // child-0 full stencil rows [18][128] (Laplacian, mass)
   DevBuf<float> KF32;
+  std::vector<double> hKF;        // host copy of KF64
   DevBuf<double> SF;              // shape functions at the 3x3x3 Gauss nodes [144][8][27]
   DevBuf<double> EL2, EGD;        // elasticity: full Laplacian [144][64] and grad-div [144][576]
   double lrow[NSTENCIL] = {}, mrow[NSTENCIL] = {};   // max row sums of |L|, |M| per type
@@ -113,6 +117,10 @@ struct DevTiles {
   DevBuf<double> rk64, rm64;    // per-record element factors (kappa h / 2, mass h^3 / 8)
   DevBuf<float> rk32, rm32;
   DevBuf<int2> texp;            // per tile: exponent bound of the contributions, log2 bound of the terms per node
+  // uniform coefficients: the element matrices fac1 L + fac2 M of every record class
+  // (child-0 type | level << 5), [NCLASS][64], rows of the classes the tiling holds
+  DevBuf<double> kc64;
+  DevBuf<float> kc32;
 };
 
 }  // namespace dgh
@@ -142,7 +150,8 @@ struct dgh_mesh_s {
   DevBuf<uint32_t> nl;          // [E][8]
   DevBuf<uint8_t> cs;           // [E] stencil index 18c+t
   double lap_fac = 1.0, mass_fac = 1.0;
-  int lane_rows = 0;            // operator kernels read stencil rows per lane from LDS
+  int lane_rows = 0;            // stencil-row source of the operator kernels (DGH_ROWS_*)
+  bool rows_auto = true;        // choose it at each re-tiling (none set explicitly)
   // owner-computes tilings: single right-hand side, and batched right-hand sides (smaller
   // tiles: the batched kernel stages several values per node in LDS)
   DevTiles op;

@@ -19,6 +19,7 @@ namespace dgh {
 
 constexpr int NXCD = 8;          // XCDs of an MI355X; workgroups are dealt round-robin
 constexpr int NSTENCIL = 18;     // child-0 stencil types
+constexpr int NCLASS = 1024;     // record classes: child-0 type | level << 5 (levels <= 31)
 
 // Maps the hardware workgroup id to a tile so that each XCD works on one contiguous
 // range of tiles: neighbouring tiles share halo nodes, whose x values then meet in one L2.
@@ -80,7 +81,8 @@ struct Rec {
 // (lap_fac, mass_fac scaled by the element size here).  LSC scales the vertex ids (the
 // single right-hand-side kernel takes them as LDS byte offsets).  UNI: 1 uniform, 0 per
 // record, -1 decided by kappa at run time.
-template <class TC, bool MASS, uint32_t LSC = 1, int UNI = -1>
+// FAC false: the factors are not formed (class matrices carry them).
+template <class TC, bool MASS, uint32_t LSC = 1, int UNI = -1, bool FAC = true>
 __device__ __forceinline__ void load_rec(Rec& R, uint32_t g, const uint16_t* __restrict__ lid,
                                          const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
                                          const TC* __restrict__ mass, double lap_fac, double mass_fac,
@@ -94,7 +96,7 @@ __device__ __forceinline__ void load_rec(Rec& R, uint32_t g, const uint16_t* __r
   if (UNI < 0 ? kappa != nullptr : UNI == 0) {
     R.k1 = double(kappa[g]);
     R.k2 = MASS ? double(mass[g]) : 0.0;
-  } else {
+  } else if (FAC) {
     elem_factors(R.mt >> 5, maxD, hFac, lap_fac, mass_fac, R.k1, R.k2);
   }
 }
@@ -239,8 +241,47 @@ __device__ __forceinline__ void eval_rec_tri(const Rec& R, uint32_t n_own, uint3
 // lanes (one in class-pure wavefronts, two at the class boundaries of a tile) with a
 // wave-uniform row pointer.  The compiler computes a row only when some lane owns it.
 // R.l holds LDS byte offsets (ids times sizeof(T)); own_bytes = n_own sizeof(T).
-template <class T, bool MASS, int MODE, uint32_t XS>
+// COMB (uniform coefficients): KF holds the element matrix fac1 L + fac2 M of each record
+// class (type | level << 5, [NCLASS][64]); the walk is over classes and a row costs 8
+// products instead of 16 plus the two factor products.
+template <class T, bool MASS, int MODE, uint32_t XS, bool COMB = false>
 __device__ __forceinline__ void eval_rows(const Rec& R, uint32_t own_bytes, int es, const T* __restrict__ KF) {
+  if (COMB) {
+    const T sc = T(ldexp(1.0, es));
+    T xv[8];
+    if (MODE != OP_DIAG) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xv[j] = *lds_at<T>(XS + R.l[j]);
+    }
+    const uint32_t cl = R.mt;
+    bool done = false;
+    for (;;) {
+      const uint64_t pend = __ballot(!done);
+      if (pend == 0) break;
+      const uint32_t cu = __builtin_amdgcn_readlane(cl, __builtin_ctzll(pend));
+      const bool mine = !done && cl == cu;
+      const T* Kt = KF + cu * 64u;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        T kr[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) kr[j] = Kt[k * 8 + j];
+        if (mine && R.l[k] < own_bytes) {
+          T a;
+          if (MODE == OP_DIAG) {
+            a = kr[k];
+          } else {
+            a = kr[0] * xv[0];
+#pragma unroll
+            for (int j = 1; j < 8; ++j) a += kr[j] * xv[j];
+          }
+          lds_add(lds_at<fx_acc_t<T>>(R.l[k]), fx_term(a, sc, T(0), T(0), false));
+        }
+      }
+      done = done || mine;
+    }
+    return;
+  }
   const T fac1 = T(R.k1), fac2 = T(R.k2);
   const T f1s = T(ldexp(double(fac1), es));
   const T f2s = MASS ? T(ldexp(double(fac2), es)) : T(0);
@@ -340,7 +381,9 @@ constexpr int XCHUNK = 8;   // x gathers per thread in flight together
 // separate instantiations so that neither keeps the other's arguments in scalar registers.
 // LANE: per-lane stencil rows from an LDS copy of KF (eval_rows_lane) instead of per-type
 // passes through the scalar cache.
-template <class T, class TC, bool MASS, int MODE, int BLOCK, bool UNI = false, bool LANE = false>
+// COMB (uniform coefficients, not LANE): KF is the tiling's class matrix table (fac1 L +
+// fac2 M per record class, eval_rows' COMB form) and the records' factors are not formed.
+template <class T, class TC, bool MASS, int MODE, int BLOCK, bool UNI = false, bool LANE = false, bool COMB = false>
 __global__ void __launch_bounds__(BLOCK)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
                const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
@@ -348,6 +391,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
                double hFac, const T* __restrict__ KF, const uint32_t* __restrict__ node_tab,
                const int2* __restrict__ texp, OpIO<T> io) {
   constexpr int NW = BLOCK / 64;
+  static_assert(!COMB || (UNI && !LANE), "class matrices need uniform coefficients and scalar rows");
   const T* __restrict__ x = io.x;
   static_assert(NW <= 16, "one max |x| word per wavefront in 64 LDS bytes");
   static_assert(sizeof(fx_acc_t<T>) == sizeof(T), "sums and staged x share the byte offsets of a record");
@@ -364,8 +408,8 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   // first record: its loads are issued before the x gather so both chains overlap
   Rec R;
   if (threadIdx.x < rec.n_elem)
-    load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0>(R, rec.elem_off + threadIdx.x, lid, meta, kappa, mass, lap_fac, mass_fac,
-                                               maxD, hFac);
+    load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0, !COMB>(R, rec.elem_off + threadIdx.x, lid, meta, kappa, mass, lap_fac,
+                                                      mass_fac, maxD, hFac);
   const uint32_t kf = tile_kf_off<T>(XS, rec.n_touch);
   if (LANE) {   // the stencil table, copied while the gather runs (visible after the barrier)
     for (uint32_t i = threadIdx.x; i < KF_ELEMS; i += BLOCK) lds_at<T>(kf)[i] = KF[i];
@@ -436,13 +480,13 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
       if (LANE)
         eval_rows_lane<T, MASS, MODE, XS>(R, n_own * uint32_t(sizeof(T)), es, kf);
       else
-        eval_rows<T, MASS, MODE, XS>(R, n_own * uint32_t(sizeof(T)), es, KF);
+        eval_rows<T, MASS, MODE, XS, COMB>(R, n_own * uint32_t(sizeof(T)), es, KF);
     }
     base += BLOCK;
     if (base >= rec.n_elem) break;
     if (base + threadIdx.x < rec.n_elem) {   // wave-uniform bases, lane offsets fixed
       const uint32_t r0 = rec.elem_off + base;
-      load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0>(R, threadIdx.x, lid + size_t(r0) * 8, meta + r0,
+      load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0, !COMB>(R, threadIdx.x, lid + size_t(r0) * 8, meta + r0,
                                                  UNI ? kappa : kappa + r0, UNI || !MASS ? mass : mass + r0, lap_fac,
                                                  mass_fac, maxD, hFac);
     }

@@ -5,7 +5,8 @@
 // as a vertex).  A tile's record list holds its own elements plus the "halo" elements of
 // other tiles that touch a node it owns; the kernel evaluates those records, accumulates
 // the rows of owned nodes in LDS and stores each owned node once.  Inside a tile the
-// records are sorted by stencil class so that the wavefronts read mostly one stencil.
+// records are sorted by class (child-0 type, then level) so that the wavefronts read mostly
+// one stencil.
 //
 // Stencils are mirror images across child numbers: reflecting the axes on which child c
 // has its bit set maps the element to child 0 and vertex k to vertex k^c, so
@@ -168,14 +169,17 @@ void build_tiles(uint32_t E, uint32_t N, const uint32_t* nl, const uint8_t* chil
         recs.resize(e1 - e0);
         std::iota(recs.begin(), recs.end(), e0);
         recs.insert(recs.end(), halo[t].begin(), halo[t].end());
-        std::stable_sort(recs.begin(), recs.end(), [&](uint32_t a, uint32_t b) { return tp[a] < tp[b]; });
+        // classes: child-0 type, then level (uniform coefficients read one element matrix
+        // per (type, level) class)
+        auto cls = [&](uint32_t e) { return (uint32_t(tp[e]) << 8) | lev[e]; };
+        std::stable_sort(recs.begin(), recs.end(), [&](uint32_t a, uint32_t b) { return cls(a) < cls(b); });
         // Inside each class, deal the Morton-ordered records round-robin over the class's
         // wavefronts: the 64 lanes of a wavefront then hold elements about n/64 apart, so
         // siblings (which share the parent-centre node after the mirror permutation) do
         // not collide on one LDS address in the same atomic instruction.
         for (size_t a = 0; a < recs.size();) {
           size_t b = a;
-          while (b < recs.size() && tp[recs[b]] == tp[recs[a]]) ++b;
+          while (b < recs.size() && cls(recs[b]) == cls(recs[a])) ++b;
           const size_t n = b - a, W = (n + 63) / 64;
           if (W > 1) {
             std::vector<uint32_t> seg(recs.begin() + a, recs.begin() + b);

@@ -197,11 +197,19 @@ int dgh_mesh_set_tile(dgh_mesh m, uint32_t tile_elems, uint32_t block);
  * LDS bytes per FP64 workgroup.  No reference counterpart. */
 int dgh_mesh_tile_info(dgh_mesh m, uint32_t* tile_elems, uint32_t* n_tiles, uint64_t* n_records,
                        uint64_t* n_node_entries, uint32_t* lds_bytes_f64);
-/* Stencil rows of the operator kernels: 0 = read through the scalar cache once per stencil
- * type present in a wavefront (default), 1 = read per lane from a copy of the stencil table
- * in LDS (one pass per wavefront however many types its lanes hold; pays on small levels,
- * whose small tiles mix many types).  Fails if the tiling leaves no LDS room for the table.
- * No reference counterpart: a kernel choice, also made by dgh_mesh_autotune. */
+/* Stencil rows of the operator kernels: DGH_ROWS_TYPE = the Laplacian and mass rows read
+ * through the scalar cache once per stencil type present in a wavefront; DGH_ROWS_LANE =
+ * read per lane from a copy of the stencil table in LDS (one pass per wavefront however many
+ * types its lanes hold; pays on small levels, whose small tiles mix many types);
+ * DGH_ROWS_CLASS = uniform coefficients only (per-element ones use DGH_ROWS_TYPE): the
+ * element matrix lap_fac h/2 L + mass_fac h^3/8 M of each (type, level) class, read through
+ * the scalar cache once per class present in a wavefront (half the products per row).
+ * Until one is set, each re-tiling picks DGH_ROWS_CLASS for tilings of at least 1024 tiles
+ * and DGH_ROWS_TYPE otherwise.  DGH_ROWS_LANE fails if the tiling leaves no LDS room for the
+ * table.  No reference counterpart: a kernel choice, also made by dgh_mesh_autotune. */
+#define DGH_ROWS_TYPE 0
+#define DGH_ROWS_LANE 1
+#define DGH_ROWS_CLASS 2
 int dgh_mesh_set_lane_stencils(dgh_mesh m, int on);
 int dgh_mesh_lane_stencils(dgh_mesh m, int* on);
 /* Autotuner: times the operator kernel of precision prec on this mesh for every candidate

@@ -110,9 +110,11 @@ def _mesh(h, mode):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode,lane,block", [("laplacian", 0, 0), ("jacobian", 0, 0), ("var", 0, 0),
-                                             ("var", 1, 0), ("jacobian", 1, 256), ("var", 0, 1024)])
+                                             ("var", 1, 0), ("jacobian", 1, 256), ("var", 0, 1024),
+                                             ("laplacian", 2, 0), ("jacobian", 2, 1024)])
 def test_matvec_dirichlet_gpu(dev, hier_small, mode, lane, block):
-    """lane: stencil rows per lane from LDS; block: 256 / 1024 threads per tile (0: default)"""
+    """lane: stencil-row source (1 per lane from LDS, 2 per (type, level) class matrix);
+    block: 256 / 1024 threads per tile (0: default)"""
     h = hier_small
     F = h.fine
     b = h.damg.boundary(h.n - 1)

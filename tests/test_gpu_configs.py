@@ -207,11 +207,18 @@ def test_config3_mixed_8m(dev):
     damg, levels = _build(da.PTS_MIXED, 8000000, 14, 8, 3, 31)
     F = levels[-1]
     ms, ts = _device_hierarchy(damg, levels, None, 1.0, 1.0)
+    # uniform coefficients on a tiling of >= 1024 tiles: the class matrices by default
+    assert ms[-1].tile_info()["lane_stencils"] == da.ROWS_CLASS
     x = np.random.default_rng(11).uniform(-1, 1, F.n_node)
     xt = torch.tensor(x, device=dev)
     yt = torch.empty_like(xt)
     ms[-1].matvec(xt, yt)
-    assert rel_err(yt.cpu().numpy(), oracle.matvec(F, x)) <= 1e-12
+    y_ref = oracle.matvec(F, x)
+    assert rel_err(yt.cpu().numpy(), y_ref) <= 1e-12
+    ms[-1].set_lane_stencils(da.ROWS_TYPE)    # and the per-type rows at full size
+    ms[-1].matvec(xt, yt)
+    assert rel_err(yt.cpu().numpy(), y_ref) <= 1e-12
+    ms[-1].set_lane_stencils(da.ROWS_CLASS)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(100):                      # the configuration's 100 MatVecs

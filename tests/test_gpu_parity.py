@@ -49,10 +49,12 @@ def _meshes(h, coefs=None, lap_fac=1.0, mass_fac=1.0, lane=0):
     return ms, ts
 
 
-@pytest.mark.parametrize("lane", [0, 1])
+@pytest.mark.parametrize("lane", [da.ROWS_TYPE, da.ROWS_LANE, da.ROWS_CLASS])
 @pytest.mark.parametrize("mode", ["jac1", "jac2", "jac2_nomass"])
 def test_matvec_fp64(dev, hier_small, mode, lane):
-    """lane = 1: stencil rows read per lane from the LDS copy of the table."""
+    """lane: the stencil-row source (per wavefront type, per lane from the LDS copy of the
+    table, per (type, level) class matrix; per-element coefficients use the per-type rows
+    for the last)."""
     h = hier_small
     F = h.fine
     rng = np.random.default_rng(11)
@@ -146,17 +148,19 @@ def test_matvec_reproducible_and_scale_exact(dev, hier_small, prec):
     assert torch.isnan(yn[F.n_node // 2])
 
 
-@pytest.mark.parametrize("lane", [0, 1])
+@pytest.mark.parametrize("lane", [da.ROWS_TYPE, da.ROWS_LANE, da.ROWS_CLASS])
 def test_matvec_dot(dev, hier_small, lane):
     """y = A x with x.y from the operator's tile partials: y bit-identical to dgh_matvec,
     both against the oracle (<= 1e-12), and x.y against the FP64 dot of the oracle's y."""
     h = hier_small
     F = h.fine
-    c = var_coefficients(h, mass_base=1.0)[-1]
+    # the class matrices serve uniform coefficients (lambda = 1.5, mu = 0.5)
+    c = var_coefficients(h, mass_base=1.0)[-1] if lane != da.ROWS_CLASS else (None, None)
+    kw = dict(kappa=c[0], mass=c[1]) if c[0] is not None else dict(lap_fac=1.5, mass_fac=0.5)
     x = np.random.default_rng(17).uniform(-1, 1, F.n_node)
-    y_ref = oracle.matvec(F, x, kappa=c[0], mass=c[1])
+    y_ref = oracle.matvec(F, x, **kw)
     m = da.Mesh(F)
-    m.set_coefficients(c[0], c[1])
+    m.set_coefficients(c[0], c[1], kw.get("lap_fac", 1.0), kw.get("mass_fac", 1.0))
     m.set_lane_stencils(lane)
     xt = _t(x, dev)
     y1, y2 = torch.empty_like(xt), torch.empty_like(xt)
@@ -170,16 +174,17 @@ def test_matvec_dot(dev, hier_small, lane):
     assert abs(float(xy.item()) - d_ref) <= 1e-12 * abs(d_ref)
 
 
-@pytest.mark.parametrize("lane", [0, 1])
+@pytest.mark.parametrize("lane", [da.ROWS_TYPE, da.ROWS_LANE, da.ROWS_CLASS])
 def test_diagonal_fp64(dev, hier_small, lane):
     h = hier_small
-    c = var_coefficients(h, mass_base=1.0)
+    c = var_coefficients(h, mass_base=1.0)[-1] if lane != da.ROWS_CLASS else (None, None)
+    kw = dict(kappa=c[0], mass=c[1]) if c[0] is not None else dict(lap_fac=1.5, mass_fac=0.5)
     m = da.Mesh(h.fine)
-    m.set_coefficients(c[-1][0], c[-1][1])
+    m.set_coefficients(c[0], c[1], kw.get("lap_fac", 1.0), kw.get("mass_fac", 1.0))
     m.set_lane_stencils(lane)
     d = torch.empty(h.fine.n_node, dtype=torch.float64, device=dev)
     m.diagonal(d)
-    assert rel_err(_np(d), oracle.diagonal(h.fine, c[-1][0], c[-1][1])) <= TOL64
+    assert rel_err(_np(d), oracle.diagonal(h.fine, **kw)) <= TOL64
 
 
 def test_transfer_fp64(dev, hier_small):
@@ -368,6 +373,41 @@ def test_pcg_graph_recaptured_after_retiling(dev, hier_small):
     # the operator sums are order-independent: the same operator gives the same iterates
     assert st2.converged and st2.iterations == st1.iterations
     assert torch.equal(x2, x1)
+
+
+@pytest.mark.parametrize("prec", [da.F64, da.F32])
+def test_class_rows_agree(dev, hier_small, prec):
+    """Uniform coefficients: the (type, level) class matrices and the per-type rows agree to
+    rounding, the default source of a small tiling is the per-type rows, and a PCG solve takes
+    the same iterations with every source."""
+    h = hier_small
+    dt = torch.float64 if prec == da.F64 else torch.float32
+    x = _t(np.random.default_rng(43).uniform(-1, 1, h.fine.n_node), dev, dt)
+    ms, ts = _meshes(h, lap_fac=1.0, mass_fac=1.0)
+    assert ms[-1].tile_info()["lane_stencils"] == da.ROWS_TYPE   # few tiles: per-type rows
+    ys = []
+    for rows in (da.ROWS_TYPE, da.ROWS_CLASS, da.ROWS_LANE):
+        ms[-1].set_lane_stencils(rows)
+        y = torch.empty_like(x)
+        ms[-1].matvec(x, y, prec=prec)
+        ys.append(_np(y))
+    for y in ys[1:]:
+        err = rel_err(y, ys[0])
+        print(f"class / lane vs type stencil rows: {err:.2e}")
+        assert err <= (1e-14 if prec == da.F64 else 1e-6)
+    b = _t(np.random.default_rng(44).uniform(-1, 1, h.fine.n_node), dev)
+    p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=50, rtol=1e-10,
+                          norm_type=da.NORM_UNPRECONDITIONED, use_graph=1, vcycle_prec=prec)
+    s = da.Solver(ms, ts, p)
+    its = []
+    for rows in (da.ROWS_TYPE, da.ROWS_CLASS):
+        for m in ms:
+            m.set_lane_stencils(rows)
+        xo = torch.zeros_like(b)
+        st, _ = s.solve(b, xo)
+        assert st.converged and st.rel_true <= 1e-10
+        its.append(st.iterations)
+    assert its[0] == its[1]
 
 
 @pytest.mark.parametrize("prec", [da.F64, da.F32])
