@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 METRIC = "PCG s to 1e-8 rel resid; MatVec GDoF/s + % HBM peak; vs plain FP64 GPU"
 HBM_PEAK_GBS = 8000.0
 AUTOTUNE_MAX_ELEMS = 8_000_000
+LARGE_TILING = (8192, 1024)   # elements per tile (cap), threads per tile, above AUTOTUNE_MAX_ELEMS
 
 WORKLOADS = {
     "c3": dict(name="mixed8M_d14", kind="MIXED", npts=8000000, seed=3, cluster_seed=31, depth=14, nlevels=8,
@@ -386,20 +387,23 @@ def choose_tilings(P, args):
     """Operator tilings of every level; returns the finest level's choice.  The autotuner
     rebuilds the host tiling once per candidate: seconds per candidate on meshes of tens of
     millions of elements.  Above AUTOTUNE_MAX_ELEMS (unless --autotune-all) a level takes the
-    tiling the autotuner chose for the 44.3M-element finest level of the 8M mixed octree
-    (4096 elements x 512 threads, 741 us per FP64 MatVec in its search)."""
+    fastest tiling of a sweep on the 44.3M-element finest level of the 8M mixed octree
+    (at most 8192 elements x 1024 threads per tile, 1.276 records per element: 570 us per
+    FP64 MatVec there, against 624 us for 4096 x 512)."""
     import dendro_amd as da
     mf = P.meshes[-1]
     if args.tile:
         for l, m in enumerate(P.meshes[:-1]):
-            m.autotune(da.F64) if P.levels[l].n_elem <= AUTOTUNE_MAX_ELEMS else m.set_tile(4096, 512)
+            if getattr(args, "matvec_only", False):
+                break   # profiler runs: the finest level only
+            m.autotune(da.F64) if P.levels[l].n_elem <= AUTOTUNE_MAX_ELEMS else m.set_tile(*LARGE_TILING)
         mf.set_tile(args.tile, args.block)
         return {"tile_elems": args.tile, "block": args.block or 512}
     t0 = time.time()
     for l, m in enumerate(P.meshes):
         if P.levels[l].n_elem > AUTOTUNE_MAX_ELEMS and not args.autotune_all:
-            m.set_tile(4096, 512)
-            tuned = {"tile_elems": 4096, "block": 512, "fixed": True}
+            m.set_tile(*LARGE_TILING)
+            tuned = {"tile_elems": LARGE_TILING[0], "block": LARGE_TILING[1], "fixed": True}
         else:
             tuned = m.autotune(da.F64)
         log(f"  level {l} (E={P.levels[l].n_elem}): {tuned}  [{time.time() - t0:.1f}s]")

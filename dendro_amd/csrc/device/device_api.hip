@@ -837,7 +837,13 @@ int dgh_mesh_create(uint32_t n_elem, uint32_t n_node, uint32_t max_depth, const 
     m->h_lev.assign(lev, lev + n_elem);
     m->nl.upload(m->h_nl.data(), m->h_nl.size());
     m->cs.upload(m->h_cs.data(), n_elem);
-    plan_mesh(m.get(), 2048);
+    // default tiling: larger tiles (fewer halo records) where the tiles still fill the device
+    if (n_elem >= LARGE_MESH_ELEMS) {
+      m->block = 1024;
+      plan_mesh(m.get(), 8192);
+    } else {
+      plan_mesh(m.get(), 2048);
+    }
     *out = m.release();
   });
 }
@@ -900,7 +906,7 @@ int dgh_mesh_autotune(dgh_mesh m, int prec, uint32_t* tile_elems, uint32_t* bloc
     uint32_t bt = 0, bb = 0;
     int bl = 0;
     // small levels need small tiles: enough workgroups to fill the device
-    for (uint32_t T : {128u, 256u, 512u, 1024u, 2048u, 4096u}) {
+    for (uint32_t T : {128u, 256u, 512u, 1024u, 2048u, 4096u, 8192u}) {
       for (int B : {256, 512, 1024}) {
         m->block = B;
         try {

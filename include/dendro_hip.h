@@ -189,7 +189,8 @@ int dgh_mesh_set_coefficients(dgh_mesh m, const double* kappa, const double* mas
                               double lap_fac, double mass_fac);
 int dgh_mesh_info(dgh_mesh m, uint32_t* n_elem, uint32_t* n_node);
 /* Re-tiles the operator kernel with tiles of at most tile_elems Morton-consecutive elements
- * (default 2048; reduced automatically until the tile's LDS fits) and block threads per
+ * (default 2048 with 512 threads, or 8192 with 1024 threads on meshes of at least 4M
+ * elements; reduced automatically until the tile's LDS fits) and block threads per
  * workgroup (256, 512 or 1024; 0 keeps the current value).  No reference counterpart: the
  * tuning hook of the tile autotuner. */
 int dgh_mesh_set_tile(dgh_mesh m, uint32_t tile_elems, uint32_t block);
