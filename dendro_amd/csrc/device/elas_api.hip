@@ -4,7 +4,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <numeric>
 #include <vector>
 
 #include "elas_kernels.hpp"
@@ -12,63 +11,88 @@
 
 namespace dgh {
 
-static void elas_tables() {
-  if (g.EL2.p) return;
-  std::vector<double> L(9216), G(size_t(144) * 576);
-  generate_stencils(L.data(), nullptr, nullptr, nullptr);
-  generate_graddiv(G.data());
-  g.EL2.upload(L.data(), L.size());
-  g.EGD.upload(G.data(), G.size());
-}
+constexpr uint32_t ELAS_TILE = 2048;   // element cap of an elasticity tile (the node budget cuts first)
 
-// Host plan of the elasticity passes: inside chunks of ELAS_CHUNK Morton-consecutive elements,
-// elements sorted by their mirrored (child-0) stencil type, cut into wavefront segments of one
-// type; the node CSR of (sorted position, vertex) slots.
-static uint8_t mirrored_type(uint8_t hn, uint8_t c) {
-  uint32_t mm = 0;
-  for (int k = 0; k < 8; ++k)
-    if (hn & (1u << k)) mm |= 1u << (k ^ c);
-  const int t = etype_of(0, int(mm));
-  if (t < 0) fail(DGH_ERR_MESH, "mirrored hanging mask is not a child-0 type");
-  return uint8_t(t);
-}
-
+// Owner-computes tiling of the elasticity operator (the scalar operator's plan, with 24-byte
+// node slots: a smaller owned-node capacity and node budget) and the records' meta with the
+// child number.
 static void elas_plan(dgh_mesh m) {
   ElasState& S = m->elas;
   if (S.ready) return;
-  const uint32_t E = m->E, N = m->N;
-  if (E > (1u << 29)) fail(DGH_ERR_ARG, "too many elements for the elasticity slot encoding");
-  std::vector<uint8_t> tp(E);
-  for (uint32_t e = 0; e < E; ++e) tp[e] = mirrored_type(m->h_hn[e], m->h_child[e]);
-  std::vector<uint32_t> order(E);
-  std::iota(order.begin(), order.end(), 0u);
-  std::vector<int4> seg;
-  for (uint32_t c0 = 0; c0 < E; c0 += ELAS_CHUNK) {
-    const uint32_t c1 = std::min<uint32_t>(E, c0 + ELAS_CHUNK);
-    std::stable_sort(order.begin() + c0, order.begin() + c1, [&](uint32_t a, uint32_t b) { return tp[a] < tp[b]; });
-    for (uint32_t i = c0; i < c1;) {
-      const uint8_t t = tp[order[i]];
-      uint32_t j = i;
-      while (j < c1 && j - i < uint32_t(ELAS_WAVE) && tp[order[j]] == t) ++j;
-      seg.push_back(make_int4(int(t), int(i), int(j - i), 0));
-      i = j;
+  build_tiles(m->E, m->N, m->h_nl.data(), m->h_child.data(), m->h_hn.data(), m->h_lev.data(), ELAS_TILE, S.plan,
+              ELAS_BUDGET, ELAS_OWN_CAP);
+  if (elas_lds_bytes(S.plan.max_touch) > size_t(g.max_lds))
+    fail(DGH_ERR_ARG, "elasticity tiles need %zu bytes of LDS (limit %d)", elas_lds_bytes(S.plan.max_touch), g.max_lds);
+  const size_t R = S.plan.rec_elem.size();
+  std::vector<uint16_t> meta(R);
+  for (size_t r = 0; r < R; ++r) {   // child-0 hanging mask | child << 8 | level << 11
+    const uint32_t t = S.plan.meta[r] & 31u, lev = S.plan.meta[r] >> 5;
+    if (t >= uint32_t(NSTENCIL) || lev > 31u) fail(DGH_ERR_MESH, "record class out of range");
+    meta[r] = uint16_t(kEtypeMask[0][t] | (uint32_t(m->h_child[S.plan.rec_elem[r]]) << 8) | (lev << 11));
+  }
+  S.tiles.upload(S.plan.tiles.data(), S.plan.tiles.size());
+  S.lid.upload(S.plan.lid.data(), S.plan.lid.size());
+  S.meta.upload(meta.data(), R);
+  S.node_tab.upload(S.plan.node_tab.data(), S.plan.node_tab.size());
+  S.ready = true;
+}
+
+// The unit element's matrix K_0 = mu L_0 (x) I + (mu + lambda) GD_0 (child 0, no hanging
+// vertex; elasticityJac.C:171-206 applies fac = h/2 per element), and the per-tile
+// fixed-point bounds from the element matrices fac (mu L (x) I + (mu + lambda) GD) of the
+// (type, level) classes the tiling holds: texp[t].x = e with every row sum of |fac K_t| over
+// the tile's records below 2^e, texp[t].y = log2 bound of the terms per owned node.
+static void elas_matrices(dgh_mesh m) {
+  ElasState& S = m->elas;
+  std::vector<double> L(9216), G(size_t(144) * 576);
+  generate_stencils(L.data(), nullptr, nullptr, nullptr);
+  generate_graddiv(G.data());
+  std::vector<double> k0(576), ke(576), rowmax(NCLASS, 0.0);
+  for (int r = 0; r < 24; ++r)
+    for (int q = 0; q < 24; ++q)
+      k0[r * 24 + q] = S.mu * ((r % 3 == q % 3) ? L[(r / 3) * 8 + q / 3] : 0.0) + (S.mu + S.lam) * G[r * 24 + q];
+  std::vector<uint8_t> seen(NCLASS, 0);
+  for (uint16_t c : S.plan.meta) {
+    if (seen[c]) continue;
+    seen[c] = 1;
+    const uint32_t t = c & 31u, lev = c >> 5;
+    const double fac = m->hFac * double(1u << (m->maxD - lev)) / 2.0;
+    const double f1 = S.mu * fac, f2 = (S.mu + S.lam) * fac;
+    double* K = ke.data();
+    for (int r = 0; r < 24; ++r) {
+      double rs = 0.0;
+      for (int q = 0; q < 24; ++q) {
+        const double lap = (r % 3 == q % 3) ? L[size_t(t) * 64 + (r / 3) * 8 + q / 3] : 0.0;
+        K[r * 24 + q] = f1 * lap + f2 * G[size_t(t) * 576 + r * 24 + q];
+        rs += std::fabs(K[r * 24 + q]);
+      }
+      rowmax[c] = std::max(rowmax[c], rs);
     }
   }
-  std::vector<uint32_t> rowptr(size_t(N) + 1, 0);
-  for (size_t q = 0; q < size_t(E) * 8; ++q) rowptr[m->h_nl[q] + 1]++;
-  for (uint32_t v = 0; v < N; ++v) rowptr[v + 1] += rowptr[v];
-  std::vector<uint32_t> slot(size_t(E) * 8), fill(rowptr.begin(), rowptr.end() - 1);
-  for (uint32_t i = 0; i < E; ++i)
-    for (int k = 0; k < 8; ++k) slot[fill[m->h_nl[size_t(order[i]) * 8 + k]]++] = i * 8 + uint32_t(k);
-  S.order.upload(order.data(), E);
-  S.seg.upload(seg.data(), seg.size());
-  S.nseg = int(seg.size());
-  S.rowptr.upload(rowptr.data(), rowptr.size());
-  S.slot.upload(slot.data(), slot.size());
-  S.lev.upload(m->h_lev.data(), E);
-  S.child.upload(m->h_child.data(), E);
-  S.ye.alloc(size_t(E) * 24);
-  S.ready = true;
+  std::vector<int2> te(S.plan.tiles.size());
+  std::vector<uint32_t> terms;
+  for (size_t t = 0; t < S.plan.tiles.size(); ++t) {
+    const TileRec& tr = S.plan.tiles[t];
+    terms.assign(tr.n_own, 0u);
+    double mx = 0.0;
+    for (uint32_t r = tr.elem_off; r < tr.elem_off + tr.n_elem; ++r) {
+      mx = std::max(mx, rowmax[S.plan.meta[r]]);
+      for (int k = 0; k < 8; ++k) {
+        const uint16_t l = S.plan.lid[size_t(r) * 8 + k];
+        if (l < tr.n_own) terms[l]++;
+      }
+    }
+    uint32_t vmax = 1;
+    for (uint32_t c : terms) vmax = std::max(vmax, c);
+    int v = 0;
+    while ((1u << v) < vmax) ++v;
+    if (!std::isfinite(mx)) fail(DGH_ERR_ARG, "non-finite elasticity element matrices");
+    int e = -1022;
+    if (mx > 0.0) std::frexp(mx, &e);
+    te[t] = make_int2(std::max(e, -1022), v);
+  }
+  S.ke.upload(k0.data(), k0.size());
+  S.texp.upload(te.data(), te.size());
 }
 
 }  // namespace dgh
@@ -81,10 +105,10 @@ int dgh_mesh_set_elasticity(dgh_mesh m, double mu, double lambda) {
     if (!m) fail(DGH_ERR_ARG, "NULL mesh");
     if (!(mu > 0.0) || !(mu + lambda > 0.0) || !std::isfinite(lambda))
       fail(DGH_ERR_ARG, "Lame parameters need mu > 0 and mu + lambda > 0");
-    elas_tables();
     elas_plan(m);
     m->elas.mu = mu;
     m->elas.lam = lambda;
+    elas_matrices(m);
   });
 }
 
@@ -96,15 +120,15 @@ int dgh_matvec_elasticity(dgh_mesh m, const double* x, double* y, void* stream) 
     if (!m->elas.ready) fail(DGH_ERR_STATE, "dgh_mesh_set_elasticity has not been called on this mesh");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const ElasState& S = m->elas;
-    const uint8_t* bdy = m->bdy.p;
-    if (S.nseg)
-      hipLaunchKernelGGL(elas_elem_kernel, dim3((S.nseg + 3) / 4), dim3(256), 0, st, S.nseg, (const int4*)S.seg.p,
-                         (const uint32_t*)S.order.p, (const uint32_t*)m->nl.p, (const uint8_t*)S.lev.p,
-                         (const uint8_t*)S.child.p, m->maxD, m->hFac, S.mu, S.lam, (const double*)g.EL2.p,
-                         (const double*)g.EGD.p, bdy, x, S.ye.p);
-    if (m->N)
-      hipLaunchKernelGGL(elas_node_kernel, dim3(blocks_for(m->N)), dim3(VBLOCK), 0, st, m->N,
-                         (const uint32_t*)S.rowptr.p, (const uint32_t*)S.slot.p, (const double*)S.ye.p, bdy, x, y);
+    const int nt = int(S.plan.tiles.size());
+    if (nt) {
+      const unsigned grid = unsigned(NXCD * ((nt + NXCD - 1) / NXCD));
+      const size_t lds = elas_lds_bytes(S.plan.max_touch);
+      hipLaunchKernelGGL(elas_tile_kernel, dim3(grid), dim3(ELAS_BLOCK), lds, st, (const TileRec*)S.tiles.p, nt,
+                         (const uint16_t*)S.lid.p, (const uint16_t*)S.meta.p, (const uint32_t*)S.node_tab.p,
+                         (const int2*)S.texp.p, (const double*)S.ke.p, 0u, m->maxD, m->hFac, (const uint8_t*)m->bdy.p,
+                         x, y);
+    }
     HIP_CHECK(hipGetLastError());
   });
 }

@@ -2,113 +2,228 @@
 // ElasticityMatMult (examples/src/backend/elasticityJac.C:129-211),
 //   y_k = sum_j fac (mu L[k][j] I + (mu + lambda) GD[3k..3k+2][3j..3j+2]) x_j,  fac = h/2,
 // with optional Dirichlet flags (boundary rows y = x, boundary columns dropped).
-// Two passes, no atomics:
-//   1. one thread per element.  Child c's stencils are child 0's of the mirrored hanging type
-//      t' with the vertex order k^c and, for grad-div, the signs s_a s_b (s_a = -1 when bit a
-//      of c is set: the mirror flips axis a), so the element takes its inputs as
-//      xe[3k+a] = s_a x[3 idx[k^c] + a], applies child 0's stencils of type t' and returns
-//      s_a out[3k+a] to vertex k^c.  Elements are sorted by t' inside chunks of ELAS_CHUNK
-//      Morton-consecutive elements (the chunk's nodes stay in L2), and every wavefront holds
-//      one t': the 8x8 Laplacian and 24x24 grad-div rows are wave-uniform loads.  The 24
-//      outputs go to ye[i][24] (i = sorted position, vertex order of the element);
-//   2. one thread per node: y[3n+d] = sum over the node's (element, vertex) slots of ye, in a
-//      fixed order (a CSR built on the host).
+//
+// Owner-computes tiles as for the scalar operator (tiles.cpp plan; one workgroup per tile):
+// the tile's touched nodes' three components are staged in LDS, every record (own and halo
+// elements) is evaluated, the rows of the nodes the tile owns are accumulated with 64-bit
+// fixed-point LDS additions, and each owned node is stored once (no global atomics, no
+// zeroing of y, bit-reproducible).
+// Element matrices: child c's element is child 0's with the vertex order k^c (the plan's
+// vertex permutation) and the signs s_a s_b on the (a, b) blocks (s_a = -1 when bit a of c is
+// set: the mirror flips axis a).  Child 0's element of hanging mask H is the regular one
+// constrained by the interpolation P_H of its hanging vertices, K_H = P_H^T K_0 P_H, where
+// row k of P_H is e_k for a regular vertex, (e_0 + e_k)/2 for a hanging edge midpoint
+// (k = 1, 2, 4) and (e_0 + e_a + e_b + e_k)/4 for a hanging face centre (k = a | b =
+// 3, 5, 6); the stencils of the 18 child-0 types are exactly these products (the shape
+// functions interpolate the parent's vertices).  So a lane applies P_H to its (sign-flipped)
+// inputs, the one 24x24 matrix K_0 = mu L_0 (x) I + (mu + lambda) GD_0 of the unit element
+// (every lane the same: wave-uniform rows through the scalar cache, no per-type passes), and
+// P_H^T to the result, and scales by its element's fac = h/2.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include "kernels.hpp"
 #include "vector_kernels.hpp"
 
 namespace dgh {
 
-constexpr int ELAS_WAVE = 64;
-constexpr uint32_t ELAS_CHUNK = 16384;
+constexpr int ELAS_ROWS = 24;
+constexpr int ELAS_NSTRIDE = 24;   // LDS bytes per node: three doubles (sums or staged x)
+constexpr int ELAS_XCHUNK = 4;     // node gathers per thread in flight together
 
-// seg[w] = {mirrored type t', first sorted position, count} of wavefront w (count <= 64);
-// L0 [18][64] and GD0 [18][576]: child 0's stencils
-__global__ void __launch_bounds__(256)
-elas_elem_kernel(int nseg, const int4* __restrict__ seg, const uint32_t* __restrict__ order,
-                 const uint32_t* __restrict__ nl, const uint8_t* __restrict__ lev, const uint8_t* __restrict__ child,
-                 uint32_t maxD, double hFac, double mu, double lam, const double* __restrict__ L0,
-                 const double* __restrict__ GD0, const uint8_t* __restrict__ bdy, const double* __restrict__ x,
-                 double* __restrict__ ye) {
-  const int w = int(blockIdx.x) * (256 / ELAS_WAVE) + int(threadIdx.x) / ELAS_WAVE;
-  if (w >= nseg) return;
-  const int lane = threadIdx.x % ELAS_WAVE;
-  const int4 sg = seg[w];
-  if (lane >= sg.z) return;
-  const int type = __builtin_amdgcn_readfirstlane(sg.x);
-  const uint32_t i = uint32_t(sg.y + lane);
-  const uint32_t e = order[i];
-  const uint32_t c = child[e];
-  const uint4 a4 = reinterpret_cast<const uint4*>(nl)[2 * e];
-  const uint4 b4 = reinterpret_cast<const uint4*>(nl)[2 * e + 1];
-  const uint32_t idx[8] = {a4.x, a4.y, a4.z, a4.w, b4.x, b4.y, b4.z, b4.w};
-  const double sg0 = (c & 1u) ? -1.0 : 1.0, sg1 = (c & 2u) ? -1.0 : 1.0, sg2 = (c & 4u) ? -1.0 : 1.0;
-  double xe[24];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    uint32_t v = idx[0];   // idx[k ^ c] without dynamic register indexing
-#pragma unroll
-    for (int q = 1; q < 8; ++q) v = (uint32_t(q) == (uint32_t(k) ^ c)) ? idx[q] : v;
-    const bool keep = !(bdy && bdy[v]);
-    xe[3 * k] = keep ? sg0 * x[3 * size_t(v)] : 0.0;
-    xe[3 * k + 1] = keep ? sg1 * x[3 * size_t(v) + 1] : 0.0;
-    xe[3 * k + 2] = keep ? sg2 * x[3 * size_t(v) + 2] : 0.0;
-  }
-  const double fac = hFac * double(1u << (maxD - lev[e])) / 2.0;
-  const double f1 = mu * fac, f2 = (mu + lam) * fac;
-  const double* __restrict__ Lt = L0 + size_t(type) * 64;
-  const double* __restrict__ Gt = GD0 + size_t(type) * 576;
-  double* __restrict__ out = ye + size_t(i) * 24;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const double l = Lt[k * 8 + j];
-      s0 += l * xe[3 * j];
-      s1 += l * xe[3 * j + 1];
-      s2 += l * xe[3 * j + 2];
-    }
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-#pragma unroll
-    for (int q = 0; q < 24; ++q) {
-      g0 += Gt[(3 * k) * 24 + q] * xe[q];
-      g1 += Gt[(3 * k + 1) * 24 + q] * xe[q];
-      g2 += Gt[(3 * k + 2) * 24 + q] * xe[q];
-    }
-    const uint32_t ko = uint32_t(k) ^ c;   // the element's own vertex of mirrored vertex k
-    out[3 * ko] = sg0 * (f1 * s0 + f2 * g0);
-    out[3 * ko + 1] = sg1 * (f1 * s1 + f2 * g1);
-    out[3 * ko + 2] = sg2 * (f1 * s2 + f2 * g2);
-  }
+// Threads per tile, owned-node capacity and node budget (owned capacity + touched nodes) of an
+// elasticity tile: the kernel's registers (about 160 VGPRs) allow three 256-thread workgroups
+// per CU, and the budget gives each the LDS of a third of a CU (24 B per node slot).  On the
+// 2M-element log-normal octree: 659 us per product for a two-pass (element outputs to HBM,
+// node gather) version, 387 us for per-(type, level) 24x24 class matrices, 234 us for the
+// P_H^T K_0 P_H form with 512-thread tiles of 1680 node slots, 182 us with 256 threads and
+// 170 us with this budget.
+constexpr int ELAS_BLOCK = 256;
+constexpr uint32_t ELAS_OWN_CAP = 640u;
+constexpr uint32_t ELAS_BUDGET = 2200u;
+constexpr uint32_t ELAS_XS_OFF = ELAS_NSTRIDE * ELAS_OWN_CAP + 64u;
+__host__ constexpr size_t elas_lds_bytes(uint32_t max_touch) {
+  return size_t(ELAS_XS_OFF) + size_t(ELAS_NSTRIDE) * max_touch;
 }
 
-// y[3n+d] = bdy[n] ? x[3n+d] : sum over slots p of node n of ye[slot[p] / 8][3 (slot[p] % 8) + d]
-__global__ void __launch_bounds__(VBLOCK)
-elas_node_kernel(uint32_t n, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ slot,
-                 const double* __restrict__ ye, const uint8_t* __restrict__ bdy, const double* __restrict__ x,
-                 double* __restrict__ y) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  if (bdy && bdy[v]) {
-    y[3 * size_t(v)] = x[3 * size_t(v)];
-    y[3 * size_t(v) + 1] = x[3 * size_t(v) + 1];
-    y[3 * size_t(v) + 2] = x[3 * size_t(v) + 2];
-    return;
+// The unit matrix is read through the constant address space: uniform loads from it are
+// scalar loads whatever the address arithmetic.
+template <class U>
+using const_ptr = const __attribute__((address_space(4))) U*;
+
+__device__ __forceinline__ double flip_sign(double v, uint32_t neg) {
+  return __longlong_as_double(__double_as_longlong(v) ^ (static_cast<long long>(neg) << 63));
+}
+
+// meta: child-0 hanging mask | child << 8 | level << 11.  K0: the unit element's [24][24]
+// (rows 3k + a); kzero: 0.  texp[t]: exponent bound of the tile's row sums of |fac K_H| and log2 bound of
+// the terms per node (as for the scalar operator).
+__global__ void __launch_bounds__(ELAS_BLOCK)
+elas_tile_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
+                 const uint16_t* __restrict__ meta, const uint32_t* __restrict__ node_tab,
+                 const int2* __restrict__ texp, const double* __restrict__ K0, uint32_t kzero, uint32_t maxD,
+                 double hFac, const uint8_t* __restrict__ bdy, const double* __restrict__ x, double* __restrict__ y) {
+  constexpr int BLOCK = ELAS_BLOCK;
+  constexpr int NW = BLOCK / 64;
+  constexpr uint32_t XS = ELAS_XS_OFF;
+  static_assert(NW <= 16, "one max |x| word per wavefront in 64 LDS bytes");
+  static_assert(XS + 16u <= 65535u, "staged x must start within the ds immediate offset range");
+  const lds_ptr<uint32_t> s_xhi = lds_at<uint32_t>(ELAS_NSTRIDE * ELAS_OWN_CAP);
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t >= ntiles) return;
+  const TileRec rec = tiles[t];
+  const uint32_t* nodes = node_tab + rec.node_off;
+  const uint32_t n_own = rec.n_own;
+  uint4 lp = make_uint4(0, 0, 0, 0);
+  uint32_t mt = 0;
+  if (threadIdx.x < rec.n_elem) {
+    lp = reinterpret_cast<const uint4*>(lid)[rec.elem_off + threadIdx.x];
+    mt = meta[rec.elem_off + threadIdx.x];
   }
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (uint32_t p = rowptr[v]; p < rowptr[v + 1]; ++p) {
-    const uint32_t q = slot[p];
-    const double* r = ye + size_t(q >> 3) * 24 + 3 * (q & 7);
-    s0 += r[0];
-    s1 += r[1];
-    s2 += r[2];
+  // stage the touched nodes' x (boundary nodes as 0: their columns drop out)
+  uint32_t xhi = 0;
+  for (uint32_t base = 0; base < rec.n_touch; base += ELAS_XCHUNK * BLOCK) {
+    uint32_t nid[ELAS_XCHUNK];
+#pragma unroll
+    for (int q = 0; q < ELAS_XCHUNK; ++q) {
+      const uint32_t i = base + threadIdx.x + q * BLOCK;
+      nid[q] = (i < rec.n_touch) ? nodes[i] : 0u;
+    }
+    double v[ELAS_XCHUNK][3];
+#pragma unroll
+    for (int q = 0; q < ELAS_XCHUNK; ++q) {
+      const uint32_t i = base + threadIdx.x + q * BLOCK;
+      const bool in = i < rec.n_touch;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[q][a] = in ? x[3 * size_t(nid[q]) + a] : 0.0;
+    }
+    if (bdy) {
+#pragma unroll
+      for (int q = 0; q < ELAS_XCHUNK; ++q) {
+        const uint32_t i = base + threadIdx.x + q * BLOCK;
+        if (i < rec.n_touch && bdy[nid[q]]) v[q][0] = v[q][1] = v[q][2] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < ELAS_XCHUNK; ++q) {
+      const uint32_t i = base + threadIdx.x + q * BLOCK;
+      if (i < rec.n_touch) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) *lds_at<double>(XS + ELAS_NSTRIDE * i + 8u * a) = v[q][a];
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        xhi = max(xhi, uint32_t(uint64_t(__double_as_longlong(v[q][a])) >> 32) & 0x7fffffffu);
+    }
   }
-  y[3 * size_t(v)] = s0;
-  y[3 * size_t(v) + 1] = s1;
-  y[3 * size_t(v) + 2] = s2;
+  for (uint32_t i = threadIdx.x; i < 3 * n_own; i += BLOCK) lds_at<unsigned long long>(0)[i] = 0ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) xhi = max(xhi, uint32_t(__shfl_xor(int(xhi), off)));
+  if ((threadIdx.x & 63) == 0) s_xhi[threadIdx.x >> 6] = xhi;
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < NW; ++w) xhi = max(xhi, s_xhi[w]);
+  const bool finite = (xhi >> 20) < 0x7ffu;
+  const int ex = max(int(xhi >> 20) - 1022, FX_XMIN);
+  const int es = fx_scale<double>(texp[t], ex);
+  const uint32_t own_bytes = n_own * ELAS_NSTRIDE;
+  for (uint32_t base = 0;;) {
+    if (base + threadIdx.x < rec.n_elem) {
+      uint32_t l[8];
+      l[0] = (lp.x & 0xffffu) * ELAS_NSTRIDE; l[1] = (lp.x >> 16) * ELAS_NSTRIDE;
+      l[2] = (lp.y & 0xffffu) * ELAS_NSTRIDE; l[3] = (lp.y >> 16) * ELAS_NSTRIDE;
+      l[4] = (lp.z & 0xffffu) * ELAS_NSTRIDE; l[5] = (lp.z >> 16) * ELAS_NSTRIDE;
+      l[6] = (lp.w & 0xffffu) * ELAS_NSTRIDE; l[7] = (lp.w >> 16) * ELAS_NSTRIDE;
+      const uint32_t hm = mt & 0xffu, c = (mt >> 8) & 7u, lev = mt >> 11;
+      const uint32_t neg[3] = {c & 1u, (c >> 1) & 1u, (c >> 2) & 1u};
+      // the element's fac = h/2 and 2^es in one scale per component (with the mirror sign)
+      const double sc = ldexp(hFac * double(1u << (maxD - lev)) / 2.0, es);
+      const double sa[3] = {flip_sign(sc, neg[0]), flip_sign(sc, neg[1]), flip_sign(sc, neg[2])};
+      double xe[ELAS_ROWS];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) xe[3 * k + a] = flip_sign(*lds_at<double>(XS + l[k] + 8u * a), neg[a]);
+      // per hanging vertex k: c_k = the weight of x_k in x'_k, d_k = the weight of each other
+      // value in it (edge k = 1, 2, 4: (x_0 + x_k)/2; face k = 3, 5, 6: (x_0 + x_a + x_b + x_k)/4)
+      double ck[7], dk[7];
+#pragma unroll
+      for (int k = 1; k < 7; ++k) {
+        const bool h = (hm >> k) & 1u;
+        const double w = (k == 3 || k == 5 || k == 6) ? 0.25 : 0.5;
+        ck[k] = h ? w : 1.0;
+        dk[k] = h ? w : 0.0;
+      }
+      // x' = P_H x (faces first: they read the raw values of the edge vertices)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double x0 = xe[a], x1 = xe[3 + a], x2 = xe[6 + a], x4 = xe[12 + a];
+        xe[9 + a] = ck[3] * xe[9 + a] + dk[3] * ((x0 + x1) + x2);
+        xe[15 + a] = ck[5] * xe[15 + a] + dk[5] * ((x0 + x1) + x4);
+        xe[18 + a] = ck[6] * xe[18 + a] + dk[6] * ((x0 + x2) + x4);
+        xe[3 + a] = ck[1] * x1 + dk[1] * x0;
+        xe[6 + a] = ck[2] * x2 + dk[2] * x0;
+        xe[12 + a] = ck[4] * x4 + dk[4] * x0;
+      }
+      // y = P_H^T K_0 x', one row of K_0 at a time: row (k, a) of z = K_0 x' goes to y_k with
+      // weight c_k and to vertex 0 (and, for a face, its two edge vertices) with weight d_k.
+      // kzero is 0: each row's address formally depends on the round and on the previous
+      // row's first entry, so the compiler neither hoists the (loop-invariant) row loads out
+      // of the record loop nor issues them all at once (576 values, far more than the scalar
+      // registers would hold): the rows are read one after the other through the scalar cache.
+      const const_ptr<double> Kr0 = (const_ptr<double>)(K0 + size_t(kzero * __builtin_amdgcn_readfirstlane(base)));
+      const_ptr<double> Kr = Kr0;
+      double yv[ELAS_ROWS];
+#pragma unroll
+      for (int r = 0; r < ELAS_ROWS; ++r) yv[r] = 0.0;
+#pragma unroll
+      for (int r = 0; r < ELAS_ROWS; ++r) {
+        double kr[ELAS_ROWS];
+#pragma unroll
+        for (int q = 0; q < ELAS_ROWS; ++q) kr[q] = Kr[q];
+        Kr += ELAS_ROWS + size_t(kzero * uint32_t(__builtin_amdgcn_readfirstlane(int(__double_as_longlong(kr[0])))));
+        double zr = kr[0] * xe[0];
+#pragma unroll
+        for (int q = 1; q < ELAS_ROWS; ++q) zr += kr[q] * xe[q];
+        const int k = r / 3, a = r % 3;
+        if (k == 0 || k == 7) {
+          yv[r] += zr;
+        } else {
+          yv[r] += ck[k] * zr;
+          yv[a] += dk[k] * zr;
+          if (k == 3 || k == 5 || k == 6) {
+            const int e1 = k & -k, e2 = k & (k - 1);
+            yv[3 * e1 + a] += dk[k] * zr;
+            yv[3 * e2 + a] += dk[k] * zr;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (l[k] < own_bytes) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            lds_add(lds_at<unsigned long long>(l[k] + 8u * a), fx_term(yv[3 * k + a], sa[a], 0.0, 0.0, false));
+        }
+    }
+    base += BLOCK;
+    if (base >= rec.n_elem) break;
+    if (base + threadIdx.x < rec.n_elem) {
+      lp = reinterpret_cast<const uint4*>(lid)[rec.elem_off + base + threadIdx.x];
+      mt = meta[rec.elem_off + base + threadIdx.x];
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) {
+    const uint32_t n = nodes[i];
+    const bool b = bdy && bdy[n];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double s = fx_value<double>(lds_at<unsigned long long>(0)[3 * i + a], es);
+      y[3 * size_t(n) + a] = b ? x[3 * size_t(n) + a] : (finite ? s : __builtin_nan(""));
+    }
+  }
 }
 
 }  // namespace dgh

@@ -81,7 +81,6 @@ struct State {
   DevBuf<float> KF32;
   std::vector<double> hKF;        // host copy of KF64
   DevBuf<double> SF;              // shape functions at the 3x3x3 Gauss nodes [144][8][27]
-  DevBuf<double> EL2, EGD;        // elasticity: full Laplacian [144][64] and grad-div [144][576]
   double lrow[NSTENCIL] = {}, mrow[NSTENCIL] = {};   // max row sums of |L|, |M| per type
   rocblas_handle blas = nullptr;
   // device address of the file-scope scratch array of dgh_matvec_dot_noalloc, per device id
@@ -130,15 +129,18 @@ struct DevTiles {
 }  // namespace dgh
 
 namespace dgh {
-// Elasticity operator plan of a mesh (elas_api.hip)
+// Elasticity operator plan of a mesh (elas_api.hip): owner-computes tiles with 24-byte node
+// slots, records' meta (child-0 hanging mask, child, level), per-tile fixed-point bounds and
+// the unit element's 24x24 matrix of the current Lame parameters.
 struct ElasState {
   bool ready = false;
   double mu = 1.0, lam = 1.0;
-  int nseg = 0;
-  DevBuf<uint32_t> order, rowptr, slot;
-  DevBuf<int4> seg;
-  DevBuf<uint8_t> lev, child;
-  DevBuf<double> ye;            // [E][24] element outputs
+  TilePlan plan;
+  DevBuf<TileRec> tiles;
+  DevBuf<uint16_t> lid, meta;   // meta: child-0 hanging mask | child << 8 | level << 11
+  DevBuf<uint32_t> node_tab;
+  DevBuf<int2> texp;
+  DevBuf<double> ke;            // K_0 [24][24]
 };
 }  // namespace dgh
 
