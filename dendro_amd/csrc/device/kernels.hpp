@@ -368,7 +368,11 @@ __device__ __forceinline__ void eval_rows_lane(const Rec& R, uint32_t own_bytes,
   }
 }
 
-constexpr int XCHUNK = 8;   // x gathers per thread in flight together
+// x gathers per thread in flight together: 12 (the 44M-element Helmholtz MatVec 507 -> 482 us
+// against 8), 8 in the FP32 residual and smoothing kernels that also hold the owned nodes' b
+// and invd (12 would take them past the registers of two workgroups per CU)
+constexpr int XCHUNK = 12;
+constexpr int XCHUNK_PREF = 8;
 
 // KF: [18][128] child-0 full stencil rows of precision T (Laplacian, then mass).
 // kappa/mass: per record (nullptr: uniform lap_fac / mass_fac).  MASS: mass term present.
@@ -415,26 +419,29 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
     for (uint32_t i = threadIdx.x; i < KF_ELEMS; i += BLOCK) lds_at<T>(kf)[i] = KF[i];
   }
   uint32_t xhi = 0;   // high word of max |x| over the values this thread staged
-  // FP32 residual and smoothing: the owned nodes' ids, b and invd are loaded with the first
-  // gather batch, so the store phase waits on no global load (tiles of at most
-  // XCHUNK * BLOCK owned nodes; larger ones reload them).  In FP64 the held values would
-  // take the kernel past 64 VGPRs and cost a workgroup per CU.
-  constexpr bool PREF = sizeof(T) == 4 && (MODE == OP_RESID || MODE == OP_SMOOTH);
-  const bool pref = PREF && n_own <= uint32_t(XCHUNK * BLOCK);
-  uint32_t own_n[XCHUNK];
-  T own_b[XCHUNK], own_d[XCHUNK];
+  // FP32 residual and smoothing with up to 512 threads: the owned nodes' ids, b and invd are
+  // loaded with the first gather batch, so the store phase waits on no global load (tiles of
+  // at most XC * BLOCK owned nodes; larger ones reload them).  In FP64, and with 1024-thread
+  // tiles (two workgroups per CU need at most 64 VGPRs), the held values cost more than the
+  // reloads (8M-point octree, 1024-thread finest tiles: mixed PCG 36.4 ms holding 8 per
+  // thread, 47.9 ms holding 12, 36.0 ms holding none).
+  constexpr bool PREF = sizeof(T) == 4 && (MODE == OP_RESID || MODE == OP_SMOOTH) && BLOCK < 1024;
+  constexpr int XC = PREF ? XCHUNK_PREF : XCHUNK;
+  const bool pref = PREF && n_own <= uint32_t(XC * BLOCK);
+  uint32_t own_n[XC];
+  T own_b[XC], own_d[XC];
   if (MODE != OP_DIAG) {
-    for (uint32_t base = 0; base < rec.n_touch; base += XCHUNK * BLOCK) {
-      uint32_t nid[XCHUNK];
-      T v[XCHUNK];
+    for (uint32_t base = 0; base < rec.n_touch; base += XC * BLOCK) {
+      uint32_t nid[XC];
+      T v[XC];
 #pragma unroll
-      for (int q = 0; q < XCHUNK; ++q) {
+      for (int q = 0; q < XC; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         nid[q] = (i < rec.n_touch) ? nodes[i] : 0u;
       }
       if (PREF && base == 0) {
 #pragma unroll
-        for (int q = 0; q < XCHUNK; ++q) {
+        for (int q = 0; q < XC; ++q) {
           const uint32_t i = threadIdx.x + q * BLOCK;
           own_n[q] = nid[q];
           own_b[q] = (pref && i < n_own) ? io.b[nid[q]] : T(0);
@@ -442,22 +449,22 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
         }
       }
 #pragma unroll
-      for (int q = 0; q < XCHUNK; ++q) {
+      for (int q = 0; q < XC; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         v[q] = (i < rec.n_touch) ? x[nid[q]] : T(0);
       }
       if (MODE == OP_DIRMV) {   // flags loaded beside x (both wait on the node ids only)
-        uint8_t bf[XCHUNK];
+        uint8_t bf[XC];
 #pragma unroll
-        for (int q = 0; q < XCHUNK; ++q) {
+        for (int q = 0; q < XC; ++q) {
           const uint32_t i = base + threadIdx.x + q * BLOCK;
           bf[q] = (i < rec.n_touch) ? io.bdy[nid[q]] : uint8_t(0);
         }
 #pragma unroll
-        for (int q = 0; q < XCHUNK; ++q) v[q] = bf[q] ? T(0) : v[q];
+        for (int q = 0; q < XC; ++q) v[q] = bf[q] ? T(0) : v[q];
       }
 #pragma unroll
-      for (int q = 0; q < XCHUNK; ++q) {
+      for (int q = 0; q < XC; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         if (i < rec.n_touch) xs[i] = v[q];
         xhi = max(xhi, uint32_t(uint64_t(__double_as_longlong(double(v[q]))) >> 32) & 0x7fffffffu);
@@ -494,7 +501,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   __syncthreads();
   if (PREF && pref) {
 #pragma unroll
-    for (int q = 0; q < XCHUNK; ++q) {
+    for (int q = 0; q < XC; ++q) {
       const uint32_t i = threadIdx.x + q * BLOCK;
       if (i < n_own) {
         const T v = finite ? T(fx_value<T>(acc[i], es)) : T(__builtin_nan(""));
