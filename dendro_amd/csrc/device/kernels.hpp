@@ -430,6 +430,11 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   const bool pref = PREF && n_own <= uint32_t(XC * BLOCK);
   uint32_t own_n[XC];
   T own_b[XC], own_d[XC];
+  // the other kernels keep the owned nodes' ids from the gather for the store phase (tiles
+  // whose touched nodes fit one gather batch; larger ones reload them), except the Dirichlet
+  // product, whose boundary flags already take it past 64 VGPRs with them
+  constexpr bool KEEP = !PREF && MODE != OP_DIAG && MODE != OP_DIRMV;
+  const bool keep = KEEP && rec.n_touch <= uint32_t(XC * BLOCK);
   if (MODE != OP_DIAG) {
     for (uint32_t base = 0; base < rec.n_touch; base += XC * BLOCK) {
       uint32_t nid[XC];
@@ -438,6 +443,10 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
       for (int q = 0; q < XC; ++q) {
         const uint32_t i = base + threadIdx.x + q * BLOCK;
         nid[q] = (i < rec.n_touch) ? nodes[i] : 0u;
+      }
+      if (KEEP && base == 0) {
+#pragma unroll
+        for (int q = 0; q < XC; ++q) own_n[q] = nid[q];
       }
       if (PREF && base == 0) {
 #pragma unroll
@@ -514,8 +523,14 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
     return;
   }
   double dot = 0.0;
-  for (uint32_t i = threadIdx.x; i < n_own; i += BLOCK) {
-    const uint32_t n = nodes[i];
+  for (uint32_t i = threadIdx.x, q = 0; i < n_own; i += BLOCK, ++q) {
+    uint32_t n = 0;
+    if (KEEP && keep) {
+#pragma unroll
+      for (int u = 0; u < XC; ++u) n = (uint32_t(u) == q) ? own_n[u] : n;
+    } else {
+      n = nodes[i];
+    }
     const double s = fx_value<T>(acc[i], es);
     const T v = finite ? T(s) : T(__builtin_nan(""));
     if (MODE == OP_MATVEC) {
