@@ -52,13 +52,24 @@ def test_elasticity_matches_oracle(dev, hier_small, dirichlet):
 
 
 def test_elasticity_throughput_lognormal(dev):
-    """The 1M log-normal octree (2.0M elements): parity against the oracle and the time per
-    product (printed)."""
+    """The 1M log-normal octree (2.0M elements): parity against the oracle, with and without
+    Dirichlet flags, and the time per product (printed)."""
     pts = da.points(da.PTS_LOGNORMAL, 1000000, 2)
     damg = da.DAMG(da.Octree.from_points(pts, 12), 1, 500)
     L = damg.level(damg.nlevels - 1)
     m = da.Mesh(L)
     m.set_elasticity(1.0, 1.0)
+    bdy = damg.boundary(damg.nlevels - 1)
+    xb = np.random.default_rng(72).uniform(-1, 1, (L.n_node, 3))
+    mb = da.Mesh(L)
+    mb.set_boundary(bdy)
+    mb.set_elasticity(0.7, 1.9)
+    xbt = torch.tensor(xb, dtype=torch.float64, device=dev)
+    ybt = torch.empty_like(xbt)
+    mb.matvec_elasticity(xbt, ybt)
+    torch.cuda.synchronize()
+    assert rel_err(ybt.cpu().numpy().ravel(), oracle.matvec_elasticity(L, xb, 0.7, 1.9, bdy=bdy)) <= 1e-12
+    del mb, xbt, ybt
     x = np.random.default_rng(71).uniform(-1, 1, (L.n_node, 3))
     xt = torch.tensor(x, dtype=torch.float64, device=dev)
     yt = torch.empty_like(xt)
