@@ -383,8 +383,13 @@ def test_class_rows_agree(dev, hier_small, prec):
     h = hier_small
     dt = torch.float64 if prec == da.F64 else torch.float32
     x = _t(np.random.default_rng(43).uniform(-1, 1, h.fine.n_node), dev, dt)
+    m0 = da.Mesh(h.fine)   # no source set: a tiling of few tiles takes the per-type rows
+    m0.set_coefficients(None, None, 1.0, 1.0)
+    assert m0.tile_info()["lane_stencils"] == da.ROWS_TYPE
+    with pytest.raises(da.DendroError):
+        m0.set_lane_stencils(3)
+    del m0
     ms, ts = _meshes(h, lap_fac=1.0, mass_fac=1.0)
-    assert ms[-1].tile_info()["lane_stencils"] == da.ROWS_TYPE   # few tiles: per-type rows
     ys = []
     for rows in (da.ROWS_TYPE, da.ROWS_CLASS, da.ROWS_LANE):
         ms[-1].set_lane_stencils(rows)
