@@ -405,7 +405,9 @@ class Mesh:
         check(lib().dgh_matvec_multi(self.h, int(prec), int(k), ptr(x), ptr(y), _stream(stream)))
 
     def set_tile_multi(self, tile_elems, variant=0):
-        """variant: 0 vector-ALU element products, 1 matrix-core (MFMA) products (FP32)"""
+        """FP32 product variant: 0 vector ALUs (8 columns per slice, 512 threads, one column
+        per lane), 1 matrix cores (MFMA), 2 as 0 with 1024 threads, 3 as 0 with 4 columns per
+        slice, 4 and 5 as 0 with 2 and 4 columns per lane, 6 as 3 with 2 columns per lane"""
         check(lib().dgh_mesh_set_tile_multi(self.h, int(tile_elems), int(variant)))
 
     def autotune_multi(self, k):

@@ -195,44 +195,77 @@ __device__ __forceinline__ void lds_add(lds_ptr<U> p, U v) {
   __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
 }
 
-// Batched kernels: evaluates one record with stencil triangles Ls/Ms held in scalar
-// registers (one pass over the 36 triangle entries: each stencil value feeds both rows it
-// belongs to) and adds the rows of owned nodes, scaled by 2^es, to the fixed-point sums.
-template <class T, bool MASS, int MODE>
-__device__ __forceinline__ void eval_rec_tri(const Rec& R, uint32_t n_own, uint32_t maxD, double hFac, int es,
-                                             const T* Ls, const T* Ms, const T* __restrict__ xs,
-                                             fx_acc_t<T>* __restrict__ acc) {
+// Batched kernels: evaluates one record for CPL adjacent columns with stencil triangles
+// Ls/Ms held in scalar registers (one pass over the 36 triangle entries: each stencil value
+// feeds both rows it belongs to) and adds the rows of owned nodes, column q scaled by
+// 2^es[q], to the fixed-point sums.  R.l holds the nodes' tile-local ids; node l's values
+// of the lane's columns sit at LDS byte address xb + l * RS * sizeof(T) (x) and
+// yb + l * RS * sizeof(acc) (sums), one 24-bit multiply-add per address (written out: the
+// compiler shares l * stride between the two addresses and adds twice; the strides are at
+// most 64, inline constants).
+template <class T, bool MASS, int MODE, int CPL, int RS>
+__device__ __forceinline__ void eval_rec_tri(const Rec& R, uint32_t n_own, const int* es, const T* Ls,
+                                             const T* Ms, uint32_t xb, uint32_t yb) {
+  using TA = fx_acc_t<T>;
   const T fac1 = T(R.k1), fac2 = T(R.k2);
-  const T f1s = T(ldexp(double(fac1), es));
-  const T f2s = MASS ? T(ldexp(double(fac2), es)) : T(0);
-  T xv[8];
+  T f1s[CPL], f2s[CPL];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) xv[j] = xs[R.l[j]];
-  T a[8], bm[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    a[k] = T(0);
-    bm[k] = T(0);
+  for (int q = 0; q < CPL; ++q) {
+    if constexpr (sizeof(T) == 4) {
+      f1s[q] = ldexpf(fac1, es[q]);
+      f2s[q] = MASS ? ldexpf(fac2, es[q]) : T(0);
+    } else {
+      f1s[q] = ldexp(fac1, es[q]);
+      f2s[q] = MASS ? ldexp(fac2, es[q]) : T(0);
+    }
   }
+  T xv[CPL][8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    uint32_t a;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(a) : "v"(R.l[j]), "n"(int(RS * sizeof(T))), "v"(xb));
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) xv[q][j] = *lds_at<T>(a + q * uint32_t(sizeof(T)));
+  }
+  T a[CPL][8], bm[CPL][8];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      a[q][k] = T(0);
+      bm[q][k] = T(0);
+    }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    a[k] += Ls[tri(k, k)] * xv[k];
-    if (MASS) bm[k] += Ms[tri(k, k)] * xv[k];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      a[q][k] += Ls[tri(k, k)] * xv[q][k];
+      if (MASS) bm[q][k] += Ms[tri(k, k)] * xv[q][k];
+    }
 #pragma unroll
     for (int j = k + 1; j < 8; ++j) {
       const T l = Ls[tri(k, j)];
-      a[k] += l * xv[j];
-      a[j] += l * xv[k];
-      if (MASS) {
-        const T m = Ms[tri(k, j)];
-        bm[k] += m * xv[j];
-        bm[j] += m * xv[k];
+      const T m = MASS ? Ms[tri(k, j)] : T(0);
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) {
+        a[q][k] += l * xv[q][j];
+        a[q][j] += l * xv[q][k];
+        if (MASS) {
+          bm[q][k] += m * xv[q][j];
+          bm[q][j] += m * xv[q][k];
+        }
       }
     }
   }
 #pragma unroll
   for (int k = 0; k < 8; ++k)
-    if (R.l[k] < n_own) atomicAdd(&acc[R.l[k]], fx_term(a[k], f1s, bm[k], f2s, MASS));
+    if (R.l[k] < n_own) {
+      uint32_t ya;
+      asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(ya) : "v"(R.l[k]), "n"(int(RS * sizeof(TA))), "v"(yb));
+#pragma unroll
+      for (int q = 0; q < CPL; ++q)
+        lds_add(lds_at<TA>(ya + q * uint32_t(sizeof(TA))), fx_term(a[q][k], f1s[q], bm[q][k], f2s[q], MASS));
+    }
 }
 
 // Evaluates the rows of the owned vertices of one record and adds them, scaled by 2^es, to

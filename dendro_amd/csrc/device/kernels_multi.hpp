@@ -37,19 +37,27 @@ struct MultiIO {
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
-// Vector-ALU batched operator.  Lane = (element of the round, right-hand side c of the slice):
-// the KSL lanes of an element share its record, read the element's 8 node values of their
-// column from LDS and accumulate its 8 rows; the stencil comes through the scalar cache
-// (per-type wavefront loop of eval_rows), so LDS carries only the minimum per element and
-// column: 8 reads and at most 8 additions.  Sums accumulate in fixed point as in the single
-// right-hand-side kernel (64-bit integers for FP64, 32-bit for FP32), with one scale per
-// column of the slice (a thread works on one column throughout: gather, products, store).
-// LDS rows are padded to KSL + 1 values so the lanes of different elements spread over the
-// banks.  LDS: owned sums [ys_len][KSL + 1] (fixed point), touched x [max_touch][KSL + 1] (T).
+// 16-byte vectors of the gathers and stores: 4 FP32 or 2 FP64 values
+template <class T> struct Vec16;
+template <> struct Vec16<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <class T> using vec16_t = typename Vec16<T>::type;
+
+// Vector-ALU batched operator.  Lane = (element of the round, CPL adjacent right-hand sides
+// c .. c + CPL - 1 of the slice): the KSL / CPL lanes of an element share its record, read
+// the element's 8 node values of their columns from LDS and accumulate its 8 rows; the
+// stencil comes through the scalar cache (per-type wavefront loop), so LDS carries only the
+// minimum per element and column: 8 reads and at most 8 additions, while the record, the
+// stencil walk and the LDS addresses are shared by the CPL columns of a lane.  Sums
+// accumulate in fixed point as in the single right-hand-side kernel (64-bit integers for
+// FP64, 32-bit for FP32), with one scale per column of the slice.  Gathers and stores move
+// 16-byte vectors of a node's slice.  LDS rows are padded to KSL + 1 values so the lanes of
+// different elements spread over the banks.  LDS: owned sums [ys_len][KSL + 1] (fixed
+// point), touched x [max_touch][KSL + 1] (T).
 template <class T>
 __host__ __device__ constexpr int multi_row(int ksl) { return ksl + 1; }
 
-template <class T, bool MASS, int MODE, int KSL, int MB>
+template <class T, bool MASS, int MODE, int KSL, int MB, int CPL>
 __global__ void __launch_bounds__(MB)
 tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
                   const uint16_t* __restrict__ lid, const uint16_t* __restrict__ meta,
@@ -58,7 +66,8 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
                   const uint32_t* __restrict__ node_tab, const int2* __restrict__ texp, uint32_t ys_len,
                   MultiIO<T> io) {
   static_assert(MODE != OP_DIAG, "the diagonal does not depend on the right-hand side");
-  static_assert(64 % KSL == 0 && MB % KSL == 0, "a wavefront holds whole elements");
+  constexpr int LPE = KSL / CPL;              // lanes per element
+  static_assert(KSL % CPL == 0 && 64 % LPE == 0, "a wavefront holds whole elements");
   using TA = fx_acc_t<T>;
   constexpr int RS = multi_row<T>(KSL);      // padded LDS row
   constexpr int NW = MB / 64;
@@ -74,45 +83,76 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   const uint32_t* nodes = node_tab + rec.node_off;
   const uint32_t n_own = rec.n_own;
   const size_t k = size_t(io.k);
-  uint32_t xhi = 0;   // high word of max |x| of this thread's column (threadIdx.x % KSL)
-  // MGU gathers per thread in flight together: node ids, then their x values
+  // Gathers and stores move 16-byte vectors: LPN lanes cover the KSL columns of a node's
+  // slice, VEC columns each (a thread keeps columns cv .. cv + VEC - 1 throughout, since MB
+  // is a multiple of LPN).  Indexes past the end are clamped to the last node, so the MGU
+  // node-id loads, then the MGU vector loads of a thread go out together with no branch.
+  constexpr int VEC = 16 / int(sizeof(T));
+  constexpr int LPN = KSL / VEC;
+  static_assert(KSL % VEC == 0 && MB % LPN == 0, "whole vectors per node slice");
+  using V = vec16_t<T>;
+  const int cv = int(threadIdx.x % LPN) * VEC;
+  uint32_t xhv[VEC];   // high words of max |x| of this thread's gather columns
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) xhv[e] = 0u;
+  // The owned nodes lead the tile's node list, so when they fit in the first batch of
+  // gathers (keep), the thread's store nodes are the node ids of that batch: own_n.
   constexpr int MGU = 4;
-  const uint32_t nx = rec.n_touch * KSL;
-  for (uint32_t i0 = threadIdx.x; i0 < nx; i0 += MGU * MB) {
+  const uint32_t nv = rec.n_touch * LPN;
+  const uint32_t nov = n_own * LPN;
+  const bool keep = nov <= uint32_t(MGU * MB);
+  uint32_t own_n[MGU];
+#pragma unroll
+  for (int u = 0; u < MGU; ++u) own_n[u] = 0u;
+  const T* xsl = io.x + size_t(sl) * KSL + cv;
+  for (uint32_t i0 = threadIdx.x; i0 < nv; i0 += MGU * MB) {
     uint32_t nid[MGU];
-    T v[MGU];
+    V v[MGU];
+#pragma unroll
+    for (int u = 0; u < MGU; ++u) nid[u] = nodes[min(i0 + u * MB, nv - 1) / LPN];
+    if (i0 == threadIdx.x)
+#pragma unroll
+      for (int u = 0; u < MGU; ++u) own_n[u] = nid[u];
+#pragma unroll
+    for (int u = 0; u < MGU; ++u) v[u] = *reinterpret_cast<const V*>(xsl + size_t(nid[u]) * k);
 #pragma unroll
     for (int u = 0; u < MGU; ++u) {
       const uint32_t i = i0 + u * MB;
-      nid[u] = i < nx ? nodes[i / KSL] : 0u;
-    }
+      if (i < nv) {
+        T* d = xs + (i / LPN) * RS + cv;
 #pragma unroll
-    for (int u = 0; u < MGU; ++u) {
-      const uint32_t i = i0 + u * MB;
-      v[u] = i < nx ? io.x[size_t(nid[u]) * k + size_t(sl) * KSL + i % KSL] : T(0);
-    }
-#pragma unroll
-    for (int u = 0; u < MGU; ++u) {
-      const uint32_t i = i0 + u * MB;
-      if (i < nx) {
-        xs[(i / KSL) * RS + i % KSL] = v[u];
-        xhi = max(xhi, uint32_t(uint64_t(__double_as_longlong(double(v[u]))) >> 32) & 0x7fffffffu);
+        for (int e = 0; e < VEC; ++e) {
+          d[e] = v[u][e];
+          xhv[e] = max(xhv[e], uint32_t(uint64_t(__double_as_longlong(double(v[u][e]))) >> 32) & 0x7fffffffu);
+        }
       }
     }
   }
   for (uint32_t i = threadIdx.x; i < n_own * RS; i += MB) ys[i] = TA(0);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int es = lane / KSL, c = lane % KSL;
+  const int es = lane / LPE, c = (lane % LPE) * CPL;   // element of the round, first column
 #pragma unroll
-  for (int off = 32; off >= KSL; off >>= 1) xhi = max(xhi, uint32_t(__shfl_xor(int(xhi), off)));
-  if (lane < KSL) s_xhi[wv][lane] = xhi;
+  for (int off = 32; off >= LPN; off >>= 1)
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) xhv[e] = max(xhv[e], uint32_t(__shfl_xor(int(xhv[e]), off)));
+  if (lane < LPN)
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) s_xhi[wv][cv + e] = xhv[e];
   __syncthreads();
+  // per column: max |x| < 2^ex, fixed-point scale 2^esc; a non-finite column gives NaN.
+  // (esc: columns c .. c + CPL - 1 of the products)
+  int esc[CPL];
 #pragma unroll
-  for (int w = 0; w < NW; ++w) xhi = max(xhi, s_xhi[w][c]);
-  // per column: max |x| < 2^ex, fixed-point scale 2^esc; a non-finite column gives NaN
-  const bool finite = (xhi >> 20) < 0x7ffu;
-  const int esc = fx_scale<T>(texp[t], max(int(xhi >> 20) - 1022, FX_XMIN));
-  constexpr int EPW = 64 / KSL;               // elements per wavefront round
+  for (int q = 0; q < CPL; ++q) {
+    uint32_t xhi = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) xhi = max(xhi, s_xhi[w][c + q]);
+    esc[q] = fx_scale<T>(texp[t], max(int(xhi >> 20) - 1022, FX_XMIN));
+  }
+  // LDS byte addresses of the lane's first column in row 0 of the staged x and of the sums
+  const uint32_t xbase = uint32_t(reinterpret_cast<uintptr_t>((lds_ptr<T>)(xs + c)));
+  const uint32_t ybase = uint32_t(reinterpret_cast<uintptr_t>((lds_ptr<TA>)(ys + c)));
+  constexpr int EPW = 64 / LPE;               // elements per wavefront round
   constexpr int STRIDE = (MB / 64) * EPW;
   // the records of the next round are loaded while the current round is evaluated; the
   // stencil of the wavefront's current type stays in scalar registers across rounds
@@ -144,49 +184,52 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
         cur = tu;
       }
       if (!done && tp == tu) {
-        Rec Q = R;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) Q.l[q] *= RS;
-        eval_rec_tri<T, MASS, MODE>(Q, n_own * RS, maxD, hFac, esc, Lr, Mr, xs + c, ys + c);
+        eval_rec_tri<T, MASS, MODE, CPL, RS>(R, n_own, esc, Lr, Mr, xbase, ybase);
         done = true;
       }
     }
   }
   __syncthreads();
-  // stores, MSU per thread with their loads in flight together (i % KSL == c: MB is a
-  // multiple of KSL)
-  constexpr int MSU = 4;
-  const uint32_t no = n_own * KSL;
-  for (uint32_t i0 = threadIdx.x; i0 < no; i0 += MSU * MB) {
+  // scales of the thread's store columns cv + e
+  int ess[VEC];
+  bool fins[VEC];
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) {
+    uint32_t h = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) h = max(h, s_xhi[w][cv + e]);
+    fins[e] = (h >> 20) < 0x7ffu;
+    ess[e] = fx_scale<T>(texp[t], max(int(h >> 20) - 1022, FX_XMIN));
+  }
+  // stores: 16-byte vectors as in the gather, MSU (= MGU, the batch of own_n) per thread
+  // with their loads in flight together
+  constexpr int MSU = MGU;
+  const size_t so = size_t(sl) * KSL + cv;
+  for (uint32_t i0 = threadIdx.x; i0 < nov; i0 += MSU * MB) {
     uint32_t nid[MSU];
-    T bv[MSU], dv[MSU];
+    V bv[MSU];
+    T dv[MSU];
+#pragma unroll
+    for (int u = 0; u < MSU; ++u) nid[u] = keep ? own_n[u] : nodes[min(i0 + u * MB, nov - 1) / LPN];
 #pragma unroll
     for (int u = 0; u < MSU; ++u) {
-      const uint32_t i = i0 + u * MB;
-      nid[u] = i < no ? nodes[i / KSL] : 0u;
+      if (MODE != OP_MATVEC) bv[u] = *reinterpret_cast<const V*>(io.b + size_t(nid[u]) * k + so);
+      dv[u] = MODE == OP_SMOOTH ? io.invd[nid[u]] : T(0);
     }
 #pragma unroll
     for (int u = 0; u < MSU; ++u) {
       const uint32_t i = i0 + u * MB;
-      const size_t gi = size_t(nid[u]) * k + size_t(sl) * KSL + c;
-      bv[u] = (MODE != OP_MATVEC && i < no) ? io.b[gi] : T(0);
-      dv[u] = (MODE == OP_SMOOTH && i < no) ? io.invd[nid[u]] : T(0);
-    }
+      if (i >= nov) break;
+      const uint32_t nd = i / LPN;
+      V out;
 #pragma unroll
-    for (int u = 0; u < MSU; ++u) {
-      const uint32_t i = i0 + u * MB;
-      if (i >= no) break;
-      const uint32_t nd = i / KSL;
-      const size_t gi = size_t(nid[u]) * k + size_t(sl) * KSL + c;
-      const T v = finite ? T(fx_value<T>(ys[nd * RS + c], esc)) : T(__builtin_nan(""));
-      if (MODE == OP_MATVEC) {
-        io.y[gi] = v;
-      } else if (MODE == OP_RESID) {
-        io.y[gi] = bv[u] - v;
-      } else {
-        const T rr = bv[u] - v;
-        io.y[gi] = xs[nd * RS + c] + io.omega * (rr * dv[u]);
+      for (int e = 0; e < VEC; ++e) {
+        const T v = fins[e] ? T(fx_value<T>(ys[nd * RS + cv + e], ess[e])) : T(__builtin_nan(""));
+        if (MODE == OP_MATVEC) out[e] = v;
+        else if (MODE == OP_RESID) out[e] = bv[u][e] - v;
+        else out[e] = xs[nd * RS + cv + e] + io.omega * ((bv[u][e] - v) * dv[u]);
       }
+      *reinterpret_cast<V*>(io.y + size_t(nid[u]) * k + so) = out;
     }
   }
 }

@@ -36,9 +36,10 @@ static size_t multi_lds_worst(const DevTiles& d) {
   return std::max({multi_lds_valu<float, KS>(d), multi_lds_valu<double, KS64>(d), multi_lds_mfma(d, true)});
 }
 
-// FP32 product variants: 0 vector ALUs (8 columns per slice, 512 threads), 1 matrix cores,
-// 2 vector ALUs with 1024 threads, 3 vector ALUs with 4 columns per slice (half the LDS)
-constexpr int NVARIANT = 4;
+// FP32 product variants: 0 vector ALUs (8 columns per slice, 512 threads, one column per
+// lane), 1 matrix cores, 2 as 0 with 1024 threads, 3 as 0 with 4 columns per slice (half the
+// LDS), 4 and 5 as 0 with 2 and 4 columns per lane, 6 as 3 with 2 columns per lane
+constexpr int NVARIANT = 7;
 
 static void plan_multi(dgh_mesh m, uint32_t T) {
   build_devtiles(m, m->mop, T, multi_lds_worst);
@@ -53,14 +54,14 @@ static void check_k(int k) {
   if (k < KS || k > MAXK || k % KS) fail(DGH_ERR_ARG, "k = %d right-hand sides: must be a multiple of %d in [%d, %d]", k, KS, KS, MAXK);
 }
 
-template <class T, bool MASS, int MODE, int KSL, int MB>
+template <class T, bool MASS, int MODE, int KSL, int MB, int CPL = 1>
 static void launch_valu(dgh_mesh m, const DevTiles& d, const T* kap, const T* mas, const MultiIO<T>& io,
                         hipStream_t st) {
   const int nt = int(d.plan.tiles.size());
   const int np = nt * (io.k / KSL);
   const unsigned grid = unsigned(NXCD * ((np + NXCD - 1) / NXCD));
   const size_t lds = multi_lds_valu<T, KSL>(d);
-  hipLaunchKernelGGL((tile_multi_kernel<T, MASS, MODE, KSL, MB>), dim3(grid), dim3(MB), lds, st,
+  hipLaunchKernelGGL((tile_multi_kernel<T, MASS, MODE, KSL, MB, CPL>), dim3(grid), dim3(MB), lds, st,
                      d.tiles.p, nt, io.k / KSL, d.lid.p, d.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
                      m->hFac, Ktab<T>(), d.node_tab.p, (const int2*)d.texp.p, d.ys_len, io);
 }
@@ -76,8 +77,15 @@ static void launch_mfma(dgh_mesh m, const DevTiles& d, const float* kap, const f
                      m->hFac, Ktab<float>(), d.node_tab.p, d.ys_len, io);
 }
 
+// The batched kernels move 16-byte vectors of a node's row.
+static void check_aligned(const void* a, const void* b = nullptr, const void* c = nullptr) {
+  if ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(c)) & 15u)
+    fail(DGH_ERR_ARG, "batched vectors must be 16-byte aligned");
+}
+
 template <class T, int MODE>
 static void launch_multi(dgh_mesh m, const MultiIO<T>& io, hipStream_t st) {
+  check_aligned(io.x, io.y, io.b);
   ensure_multi(m);
   const DevTiles& d = m->mop;
   if (d.plan.tiles.empty()) return;
@@ -96,6 +104,15 @@ static void launch_multi(dgh_mesh m, const MultiIO<T>& io, hipStream_t st) {
   } else if (m->mvariant == 3) {
     if (mass) launch_valu<float, true, MODE, 4, MBLOCK>(m, d, kap, mas, io, st);
     else launch_valu<float, false, MODE, 4, MBLOCK>(m, d, kap, mas, io, st);
+  } else if (m->mvariant == 4) {
+    if (mass) launch_valu<float, true, MODE, KS, MBLOCK, 2>(m, d, kap, mas, io, st);
+    else launch_valu<float, false, MODE, KS, MBLOCK, 2>(m, d, kap, mas, io, st);
+  } else if (m->mvariant == 5) {
+    if (mass) launch_valu<float, true, MODE, KS, MBLOCK, 4>(m, d, kap, mas, io, st);
+    else launch_valu<float, false, MODE, KS, MBLOCK, 4>(m, d, kap, mas, io, st);
+  } else if (m->mvariant == 6) {
+    if (mass) launch_valu<float, true, MODE, 4, MBLOCK, 2>(m, d, kap, mas, io, st);
+    else launch_valu<float, false, MODE, 4, MBLOCK, 2>(m, d, kap, mas, io, st);
   } else {
     if (mass) launch_valu<float, true, MODE, KS, MBLOCK>(m, d, kap, mas, io, st);
     else launch_valu<float, false, MODE, KS, MBLOCK>(m, d, kap, mas, io, st);
@@ -392,6 +409,7 @@ int dgh_solver_solve_multi(dgh_solver s, int k, const double* B, double* X, dgh_
     if (s->dirichlet) fail(DGH_ERR_ARG, "the batched solve has no Dirichlet mode");
     refresh_operators(s);
     check_k(k);
+    check_aligned(B, X);
     ensure_multi_buffers(s, k);
     const uint64_t gen = mesh_generation(s);
     if (gen != s->graph_gen) {   // re-tiled or new coefficients since the graphs were captured

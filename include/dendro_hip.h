@@ -258,15 +258,16 @@ int dgh_matvec_dirichlet(dgh_mesh m, int prec, const void* x_dev, void* y_dev, v
  * (examples/src/backend/omgJac.C:1058-1160). */
 int dgh_diagonal_dirichlet(dgh_mesh m, int prec, void* d_dev, void* stream);
 /* Y = A X for k right-hand sides at once (k a multiple of 8, at most 32): X and Y are
- * [N][k] row-major (node-major, right-hand side fastest).  The reference applies
+ * [N][k] row-major (node-major, right-hand side fastest), 16-byte aligned.  The reference applies
  * Jacobian1MatMult / Jacobian2MatMult (odaJac.C:290-374, omgJac.C:1687-1775) to one vector
  * per call; each column here equals that product.  FP32 runs the element products on the
- * matrix cores. */
+ * vector ALUs or the matrix cores (the variant of dgh_mesh_set_tile_multi). */
 int dgh_matvec_multi(dgh_mesh m, int prec, int k, const void* x_dev, void* y_dev, void* stream);
 /* Tile size (elements per tile) of the batched operator, rounded down until its LDS fits,
  * and the FP32 product variant (0: vector ALUs, 8 columns per workgroup slice, 512
- * threads; 1: matrix cores; 2: as 0 with 1024 threads; 3: as 0 with 4 columns per slice).
- * No reference counterpart (tuning hook). */
+ * threads, one column per lane; 1: matrix cores; 2: as 0 with 1024 threads; 3: as 0 with 4
+ * columns per slice; 4, 5: as 0 with 2, 4 columns per lane; 6: as 3 with 2 columns per
+ * lane).  No reference counterpart (tuning hook). */
 int dgh_mesh_set_tile_multi(dgh_mesh m, uint32_t tile_elems, int variant);
 /* Times the FP32 batched operator for k columns over the candidate tile sizes and all
  * product variants and keeps the fastest; outputs (may be NULL) the tile size, the variant
@@ -396,7 +397,7 @@ int dgh_solver_destroy(dgh_solver s);
 int dgh_solver_solve(dgh_solver s, const double* b_dev, double* x_dev, dgh_solve_stats* stats,
                      double* history, int history_len, void* stream);
 /* k independent solves at once (k a multiple of 8, at most 32): B and X are FP64 [N][k]
- * row-major on the device.  Each column runs the KSPSolve CG of dgh_solver_solve with its
+ * row-major on the device, 16-byte aligned.  Each column runs the KSPSolve CG of dgh_solver_solve with its
  * own scalars and stopping test; all columns share one batched FP32 V-cycle per iteration
  * (FP64 outer CG, flexible update), captured as one hipGraph when params.use_graph is set.
  * A converged column is frozen.  stats: array of k (may be NULL).  Replaces k calls of

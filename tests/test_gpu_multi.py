@@ -41,7 +41,10 @@ def _meshes(h, coefs):
 
 
 @pytest.mark.parametrize("k,tile,mass,variant", [(8, 256, True, 0), (16, 512, False, 0), (32, 1024, True, 0),
-                                                  (8, 512, True, 1), (32, 1024, False, 1)])
+                                                  (8, 512, True, 1), (32, 1024, False, 1), (16, 512, True, 2),
+                                                  (32, 512, False, 3), (32, 512, True, 4), (16, 256, False, 4),
+                                                  (32, 512, True, 5), (8, 1024, False, 5), (32, 512, False, 6),
+                                                  (8, 256, True, 6)])
 def test_matvec_multi_matches_oracle(dev, hier_small, k, tile, mass, variant):
     h = hier_small
     F = h.fine
@@ -67,8 +70,8 @@ def test_matvec_multi_matches_oracle(dev, hier_small, k, tile, mass, variant):
     assert worst32 <= 1e-5
 
 
-@pytest.mark.parametrize("prec", [da.F64, da.F32])
-def test_matvec_multi_column_scales(dev, hier_small, prec):
+@pytest.mark.parametrize("prec,variant", [(da.F64, 0), (da.F32, 0), (da.F32, 4), (da.F32, 5), (da.F32, 6)])
+def test_matvec_multi_column_scales(dev, hier_small, prec, variant):
     """The vector-ALU batched operator scales its fixed-point sums per column: columns of
     very different magnitude in one slice keep their own precision (each column equals the
     single-column product bit for bit up to its power-of-two scale), and a NaN in one column
@@ -78,7 +81,7 @@ def test_matvec_multi_column_scales(dev, hier_small, prec):
     c = var_coefficients(h, mass_base=1.0)
     m = da.Mesh(F)
     m.set_coefficients(c[-1][0], c[-1][1])
-    m.set_tile_multi(512, 0)
+    m.set_tile_multi(512, variant)
     k = 8
     dt = torch.float64 if prec == da.F64 else torch.float32
     base = np.random.default_rng(5).uniform(-1, 1, F.n_node)
@@ -114,7 +117,7 @@ def test_solve_multi_matches_single(dev, hier_small, graph):
     coefs = var_coefficients(h, mass_base=None)
     ms, ts = _meshes(h, coefs)
     for i, m in enumerate(ms[1:]):
-        m.set_tile_multi(512, i % 2)
+        m.set_tile_multi(512, (0, 1, 5, 6)[i % 4])
     p = da.default_params(omega=2.0 / 3.0, nu_pre=2, nu_post=2, max_it=200, rtol=1e-8,
                           norm_type=da.NORM_UNPRECONDITIONED, null_space=1, vcycle_prec=da.F32,
                           use_graph=graph)
