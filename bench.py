@@ -324,14 +324,33 @@ def bench_c5(dev, args):
     return out
 
 
-def bench_multi(P, dev, stream, steps):
+def calib_copy(dev):
+    """Three 256 MiB device-to-device copies: the reference kernel of known bytes for scaling
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+    c = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    del a, c
+
+
+def bench_multi(P, dev, stream, steps, solve=True, tile=0, variant=0):
     """32 right-hand sides (seeds 400..431): the batched FP32 operator and the batched PCG
-    (one FP32 V-cycle per iteration for all columns, FP64 outer CG, one hipGraph)."""
+    (one FP32 V-cycle per iteration for all columns, FP64 outer CG, one hipGraph).  tile > 0
+    fixes the finest level's batched tiling and product variant instead of autotuning
+    (profiler runs); solve=False times the operator only."""
     import torch
     import dendro_amd as da
     k = 32
     N = P.fine.n_node
-    tuned = [m.autotune_multi(k) for m in P.meshes[1:]]
+    if tile:
+        P.meshes[-1].set_tile_multi(tile, variant)
+        tuned = [{"tile_elems": tile, "variant": variant}]
+    else:
+        tuned = [m.autotune_multi(k) for m in (P.meshes[1:] if solve else P.meshes[-1:])]
     mf = P.meshes[-1]
     X = torch.tensor(np.random.default_rng(11).uniform(-1, 1, (N, k)), dtype=torch.float32, device=dev)
     Y = torch.empty_like(X)
@@ -346,6 +365,15 @@ def bench_multi(P, dev, stream, steps):
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / reps
     del X, Y
+    F = P.fine
+    alg = F.n_elem * 34 + k * N * 2 * 4 + (0 if P.uniform else F.n_elem * 4)
+    out = {"k": k, "matvec_f32_us": round(ms * 1e3, 2), "matvec_f32_gdofs": round(N * k / (ms * 1e-3) / 1e9, 3),
+           "matvec_f32_hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "tile_elems": [t["tile_elems"] for t in tuned], "variants": [t["variant"] for t in tuned]}
+    log(f"batched k={k}: MatVec FP32 {ms*1e3:.1f} us ({out['matvec_f32_gdofs']} GDoF/s, "
+        f"{100*out['matvec_f32_hbm_frac']:.1f}% HBM), tiling {out['tile_elems']} variants {out['variants']}")
+    if not solve:
+        return out
     B = torch.tensor(np.stack([np.random.default_rng(400 + j).uniform(-1, 1, N) for j in range(k)], 1),
                      dtype=torch.float64, device=dev)
     Xs = torch.zeros_like(B)
@@ -353,16 +381,10 @@ def bench_multi(P, dev, stream, steps):
     s.solve_multi(B, Xs, k)
     st = s.solve_multi(B, Xs, k)
     secs = st[0].seconds
-    F = P.fine
-    alg = F.n_elem * 34 + k * N * 2 * 4 + (0 if P.uniform else F.n_elem * 4)
-    out = {"k": k, "matvec_f32_us": round(ms * 1e3, 2), "matvec_f32_gdofs": round(N * k / (ms * 1e-3) / 1e9, 3),
-           "matvec_f32_hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-           "tile_elems": [t["tile_elems"] for t in tuned], "variants": [t["variant"] for t in tuned],
-           "solve_seconds": round(secs, 5), "seconds_per_rhs": round(secs / k, 6),
-           "iterations": [x.iterations for x in st], "max_rel_true": max(x.rel_true for x in st),
-           "all_converged": all(x.converged for x in st)}
-    log(f"batched k={k}: MatVec FP32 {ms*1e3:.1f} us ({out['matvec_f32_gdofs']} GDoF/s, "
-        f"{100*out['matvec_f32_hbm_frac']:.1f}% HBM), PCG {secs*1e3:.1f} ms for {k} RHS, its "
+    out.update({"solve_seconds": round(secs, 5), "seconds_per_rhs": round(secs / k, 6),
+                "iterations": [x.iterations for x in st], "max_rel_true": max(x.rel_true for x in st),
+                "all_converged": all(x.converged for x in st)})
+    log(f"batched k={k}: PCG {secs*1e3:.1f} ms for {k} RHS, its "
         f"{min(out['iterations'])}-{max(out['iterations'])}, max rel_true {out['max_rel_true']:.2e}")
     del s
     return out
@@ -431,13 +453,7 @@ def matvec_section(P, args, dev, stream):
     log(f"MatVec FP64 after an Infinity Cache flush: {cold*1e3:.1f} us "
         f"({100 * alg / (cold * 1e-3) / 1e9 / HBM_PEAK_GBS:.1f}% of HBM peak)")
     if args.calib_copy:
-        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev)
-        c = torch.empty_like(a)
-        a.fill_(1.0)
-        for _ in range(3):
-            c.copy_(a)
-        torch.cuda.synchronize()
-        del a, c
+        calib_copy(dev)
     x32, y32 = x.float(), torch.empty(F.n_node, dtype=torch.float32, device=dev)
     ms32, _ = time_matvec(mf, x32, y32, args.steps, args.warmup, stream, prec=da.F32)
     del x32, y32
@@ -473,6 +489,11 @@ def main():
     ap.add_argument("--no-dirichlet", action="store_true", help="skip the Dirichlet-masked MatVec (secondary)")
     ap.add_argument("--matvec-only", action="store_true",
                     help="only the headline MatVec (for profiler runs): implies --no-cpu --no-pcg --no-c2")
+    ap.add_argument("--multi-only", action="store_true",
+                    help="only the batched FP32 MatVec (32 columns) on the 1M log-normal octree (profiler runs)")
+    ap.add_argument("--multi-tile", type=int, default=0,
+                    help="with --multi-only: batched tile size (0: autotune), with --multi-variant")
+    ap.add_argument("--multi-variant", type=int, default=4, help="batched FP32 product variant with --multi-tile")
     ap.add_argument("--tile", type=int, default=0, help="operator tile size (0: autotune)")
     ap.add_argument("--block", type=int, default=0, help="workgroup size with --tile (0: library default)")
     ap.add_argument("--autotune-all", action="store_true",
@@ -500,6 +521,13 @@ def main():
     dev = torch.device("cuda:0")
     da.init(0)
     stream = torch.cuda.current_stream()
+    if args.multi_only:   # the batched operator runs on the 1M log-normal octree whatever --workload
+        Q = Problem(WORKLOADS["c2"])
+        r = bench_multi(Q, dev, stream, args.steps, solve=False, tile=args.multi_tile, variant=args.multi_variant)
+        if args.calib_copy:
+            calib_copy(dev)
+        print(json.dumps({"workload": Q.w["desc"], "multi_rhs": r}), flush=True)
+        return
     P = Problem(WORKLOADS[args.workload])
     F = P.fine
 
