@@ -42,7 +42,7 @@ EXPORTS = [
     "dgh_mesh_set_elasticity", "dgh_matvec_elasticity",
     "dgh_transfer_create", "dgh_transfer_destroy", "dgh_restrict", "dgh_prolong_add", "dgh_matvec_jac3",
     "dgh_solver_default_params", "dgh_solver_create", "dgh_solver_destroy", "dgh_solver_solve",
-    "dgh_solver_solve_multi", "dgh_solver_vcycle",
+    "dgh_solver_solve_multi", "dgh_solver_vcycle", "dgh_solver_coarse_info",
     "dgh_learn_default_params", "dgh_learned_init", "dgh_learned_get_params", "dgh_learned_set_params",
     "dgh_learned_train", "dgh_learned_apply", "dgh_learned_eval", "dgh_learned_loss",
 ]
@@ -63,7 +63,8 @@ class SolverParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("omega", C.c_double), ("max_it", C.c_int),
                 ("nu_pre", C.c_int), ("nu_post", C.c_int), ("norm_type", C.c_int),
                 ("null_space", C.c_int), ("vcycle_prec", C.c_int), ("smoother", C.c_int),
-                ("use_graph", C.c_int), ("coarse_op", C.c_int)]
+                ("use_graph", C.c_int), ("coarse_op", C.c_int), ("coarse_dense_max", C.c_int),
+                ("coarse_rtol", C.c_double), ("coarse_max_it", C.c_int)]
 
 
 class Function(C.Structure):
@@ -477,6 +478,12 @@ class Solver:
         check(lib().dgh_solver_create(len(meshes), mh, th if transfers else None, C.byref(params),
                                       C.byref(h)))
         self.h = h
+
+    def coarse_info(self):
+        """(iterative, last_iterations): 1 when the coarsest level is solved by CG."""
+        it, n = C.c_int(), C.c_int()
+        check(lib().dgh_solver_coarse_info(self.h, C.byref(it), C.byref(n)))
+        return it.value, n.value
 
     def solve(self, b, x, history_len=0, stream=None):
         st = SolveStats()

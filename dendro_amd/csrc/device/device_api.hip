@@ -336,9 +336,72 @@ template <class T> const T* ainvp(dgh_solver s);
 template <> const double* ainvp<double>(dgh_solver s) { return s->Ap64.p; }
 template <> const float* ainvp<float>(dgh_solver s) { return s->Ap32.p; }
 
-// x = Ainv b on the coarsest level (packed symmetric blocks, two passes)
+// The coarsest level's operator in the iterative coarse solve: the operator the dense path
+// assembles (Jacobian3: R A_1 P with level 1's coefficients; Dirichlet: the masked operator).
+static void coarse_op_apply(dgh_solver s, const double* x, double* y, hipStream_t st) {
+  dgh_mesh m = s->meshes[0];
+  if (s->prm.coarse_op == DGH_COARSE_JAC3 && s->nlevels > 1)
+    launch_jac3<double>(s->transfers[0], x, y, s->v64[1].r.p, s->v64[1].t.p, st);
+  else if (s->dirichlet)
+    launch_dirmv<double>(m, x, y, st);
+  else
+    launch_matvec<double>(m, x, y, st);
+}
+
+// x = A^-1 b on a coarsest level above the dense limit: Jacobi-preconditioned CG in FP64 from
+// a zero guess to ||b - A x|| <= coarse_rtol ||b||, in chunks of CCG_CHUNK iterations with the
+// device-side stopping test read back after each chunk (so this path does not run under
+// graph capture).  With a constant null space b is projected first and x afterwards: the
+// mean-free solution, as the dense path's (A + a 11^T)^-1 gives for a mean-free b.
+constexpr int CCG_CHUNK = 16;
+template <class T>
+static void coarse_cg(dgh_solver s, const T* b, T* x, hipStream_t st) {
+  const uint32_t n = uint32_t(s->nc);
+  const dim3 gb(blocks_for(n)), tb(VBLOCK);
+  double* S = s->cS.p;
+  const double* invd = s->v64[0].invd.p;
+  hipLaunchKernelGGL(ccg_load_kernel<T>, gb, tb, 0, st, n, b, s->cr.p);
+  if (s->prm.null_space) project_mean<double>(s, n, s->cr.p, st);
+  hipLaunchKernelGGL(ccg_start_kernel, gb, tb, 0, st, n, invd, (const double*)s->cr.p, s->cx.p, s->cz.p, s->cp.p);
+  DotArgs<double> rz{};
+  rz.a[0] = s->cr.p; rz.b[0] = s->cz.p;   // CC_RZN
+  rz.a[1] = s->cr.p; rz.b[1] = s->cr.p;   // CC_RR
+  DotArgs<double> pw{};
+  pw.a[0] = s->cp.p; pw.b[0] = s->cw.p;
+  reduce<double>(n, rz, 2, s->part.p, S + CC_RZN, st);
+  hipLaunchKernelGGL(ccg_init_kernel, dim3(1), dim3(1), 0, st, S, s->prm.coarse_rtol * s->prm.coarse_rtol);
+  int its = 0;
+  for (;;) {
+    for (int q = 0; q < CCG_CHUNK; ++q) {
+      coarse_op_apply(s, s->cp.p, s->cw.p, st);
+      reduce<double>(n, pw, 1, s->part.p, S + CC_PW, st);
+      hipLaunchKernelGGL(ccg_alpha_kernel, dim3(1), dim3(1), 0, st, S);
+      hipLaunchKernelGGL(ccg_update_kernel, gb, tb, 0, st, n, (const double*)S, (const double*)s->cp.p,
+                         (const double*)s->cw.p, invd, s->cx.p, s->cr.p, s->cz.p);
+      reduce<double>(n, rz, 2, s->part.p, S + CC_RZN, st);
+      hipLaunchKernelGGL(ccg_beta_kernel, dim3(1), dim3(1), 0, st, S);
+      hipLaunchKernelGGL(ccg_p_kernel, gb, tb, 0, st, n, (const double*)S, (const double*)s->cz.p, s->cp.p);
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(s->hcS, S, sizeof(double) * CC_N, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    its = int(s->hcS[CC_ITS]);
+    if (s->hcS[CC_DONE] != 0.0 || its >= s->prm.coarse_max_it) break;
+  }
+  s->coarse_its = its;
+  if (s->prm.null_space) project_mean<double>(s, n, s->cx.p, st);
+  hipLaunchKernelGGL((convert_kernel<T, double>), gb, tb, 0, st, n, (const double*)s->cx.p, x);
+  HIP_CHECK(hipGetLastError());
+}
+
+// x = Ainv b on the coarsest level (packed symmetric blocks, two passes), or the iterative
+// coarse solve above the dense limit
 template <class T>
 static void coarse_apply(dgh_solver s, const T* b, T* x, hipStream_t st) {
+  if (s->coarse_iter) {
+    coarse_cg<T>(s, b, x, st);
+    return;
+  }
   const int nblk = s->nbc * (s->nbc + 1) / 2;
   hipLaunchKernelGGL(sym_block_kernel<T>, dim3(nblk), dim3(128), 0, st, s->nc, s->npc, ainvp<T>(s), b, s->Qc.p);
   hipLaunchKernelGGL(sym_sum_kernel<T>, dim3(s->nbc), dim3(256), 0, st, s->nc, s->nbc, s->npc,
@@ -589,7 +652,19 @@ static void setup_coarse(dgh_solver s) {
   dgh_mesh m = s->meshes[0];
   const int n = int(m->N);
   s->nc = n;
-  if (n > 40000) fail(DGH_ERR_ARG, "coarsest level has %d nodes; use more levels (dense solve limit 40000)", n);
+  if (n > std::min(s->prm.coarse_dense_max, COARSE_DENSE_LIMIT)) {   // iterative coarse solve
+    s->coarse_iter = true;
+    if (!s->cx.p) {
+      s->cx.alloc(n);
+      s->cr.alloc(n);
+      s->cz.alloc(n);
+      s->cp.alloc(n);
+      s->cw.alloc(n);
+      s->cS.alloc(CC_N);
+      HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s->hcS), sizeof(double) * CC_N, hipHostMallocDefault));
+    }
+    return;
+  }
   std::vector<double> L(9216), M(9216);
   generate_stencils(L.data(), M.data(), nullptr, nullptr);
   const int ld = (n + DENSE_PAD - 1) / DENSE_PAD * DENSE_PAD;
@@ -1238,6 +1313,9 @@ int dgh_solver_default_params(dgh_solver_params* p) {
   p->smoother = DGH_SMOOTHER_JACOBI;
   p->use_graph = 0;
   p->coarse_op = DGH_COARSE_JAC2;
+  p->coarse_dense_max = COARSE_DENSE_LIMIT;
+  p->coarse_rtol = 1e-12;
+  p->coarse_max_it = 5000;
   return DGH_OK;
 }
 
@@ -1254,6 +1332,8 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
       fail(DGH_ERR_ARG, "unknown smoother %d", params->smoother);
     if (params->coarse_op != DGH_COARSE_JAC2 && params->coarse_op != DGH_COARSE_JAC3)
       fail(DGH_ERR_ARG, "unknown coarse operator %d", params->coarse_op);
+    if (params->coarse_dense_max < 0 || !(params->coarse_rtol > 0) || params->coarse_max_it < 1)
+      fail(DGH_ERR_ARG, "invalid coarse solve parameters");
     if (params->coarse_op == DGH_COARSE_JAC3 && params->smoother == DGH_SMOOTHER_LEARNED)
       fail(DGH_ERR_ARG, "the learned smoother is trained on Jacobian2 levels; use DGH_COARSE_JAC2");
     auto s = std::make_unique<dgh_solver_s>();
@@ -1297,7 +1377,7 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
       V.xa.alloc(n);
       V.xb.alloc(n);
       V.r.alloc(n);
-      const bool j3fine = params->coarse_op == DGH_COARSE_JAC3 && l >= 2;   // fine level of a Jacobian3 level
+      const bool j3fine = params->coarse_op == DGH_COARSE_JAC3 && l >= 1;   // fine level of a Jacobian3 level
       if (j3fine) V.t.alloc(n);
       solver_diag(s.get(), l, V.r.p);
       hipLaunchKernelGGL(reciprocal_kernel<double>, dim3(blocks_for(n)), dim3(VBLOCK), 0, s->st, n,
@@ -1329,6 +1409,14 @@ int dgh_solver_create(int nlevels, const dgh_mesh* meshes, const dgh_transfer* t
 int dgh_solver_destroy(dgh_solver s) {
   delete s;
   return DGH_OK;
+}
+
+int dgh_solver_coarse_info(dgh_solver s, int* iterative, int* last_iterations) {
+  return guarded([&] {
+    if (!s) fail(DGH_ERR_ARG, "NULL solver");
+    if (iterative) *iterative = s->coarse_iter ? 1 : 0;
+    if (last_iterations) *last_iterations = s->coarse_its;
+  });
 }
 
 int dgh_solver_vcycle(dgh_solver s, const double* r, double* z, void* stream) {
@@ -1395,7 +1483,7 @@ int dgh_solver_solve(dgh_solver s, const double* b, double* x, dgh_solve_stats* 
     const double bnorm = std::sqrt(s->hS[S_BB]);
     double dp = dp0;
     if (history && history_len > 0) history[0] = dp0;
-    if (s->prm.use_graph && !s->gexec) {
+    if (s->prm.use_graph && !s->gexec && !s->coarse_iter) {   // the iterative coarse solve reads back
       HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       cg_iteration(s, st);
       HIP_CHECK(hipStreamEndCapture(st, &s->graph));

@@ -70,6 +70,8 @@ constexpr size_t CLASS_ROWS_MIN_TILES = 1024;
 // 1024 threads per tile (1.28 records per element on the 44M- and 21M-element finest levels
 // of the 8M and 4M point octrees, 8-9% faster than 2048/4096 x 512), smaller ones with 2048 x 512.
 constexpr uint32_t LARGE_MESH_ELEMS = 4000000;
+// Largest coarsest level solved with a dense inverse (n^2 doubles on the host and the device).
+constexpr int COARSE_DENSE_LIMIT = 40000;
 
 struct State {
   bool ready = false;
@@ -244,6 +246,11 @@ struct dgh_solver_s {
   DevBuf<double> Qc;              // partial sums of the packed coarse apply [nbc][npc]
   DevBuf<double> c1;              // Ainv 1 (constant null space)
   int nbc = 0, npc = 0;           // coarse block rows, padded coarse size
+  bool coarse_iter = false;       // coarsest level solved by Jacobi-preconditioned CG instead
+  DevBuf<double> cx, cr, cz, cp, cw;   // its FP64 vectors (coarsest size)
+  DevBuf<double> cS;              // its scalars (CoarseCgScalar)
+  double* hcS = nullptr;          // pinned host mirror of cS
+  int coarse_its = 0;             // iterations of the last iterative coarse solve
   DevBuf<double> S;               // CG scalars (CgScalar)
   DevBuf<double> part;            // reduction partials [MAXQ][RED_BLOCKS]
   DevBuf<double> part_p;          // partials of the null-space projections
@@ -278,6 +285,7 @@ struct dgh_solver_s {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     if (hS) (void)hipHostFree(hS);
+    if (hcS) (void)hipHostFree(hcS);
     if (st) (void)hipStreamDestroy(st);
   }
 };

@@ -272,6 +272,9 @@ void column_dots_f32(uint32_t n, int k, const float* a, const float* b, double* 
 }
 
 void ensure_multi_buffers(dgh_solver s, int k) {
+  if (s->coarse_iter)
+    fail(DGH_ERR_STATE, "the batched V-cycle needs the dense coarse solve; the coarsest level has %d nodes "
+         "(dense limit %d): use more levels", s->nc, std::min(s->prm.coarse_dense_max, COARSE_DENSE_LIMIT));
   if (s->mk == k) return;
   if (s->mgexec) (void)hipGraphExecDestroy(s->mgexec);
   if (s->mgraph) (void)hipGraphDestroy(s->mgraph);

@@ -132,6 +132,77 @@ __global__ void subtract_mean_kernel(uint32_t n, const double* __restrict__ sum,
   if (i < n) v[i] = v[i] - T(sum[0] / double(n));
 }
 
+// ---------------------------------------------------------------- coarsest-level CG
+// Jacobi-preconditioned CG for coarsest levels too large for the dense inverse, FP64, with
+// its scalars on the device: the iterations run in chunks without host synchronisation;
+// once the stopping test holds (DONE) the step lengths are zero, so further iterations of
+// a chunk leave x unchanged.
+enum CoarseCgScalar { CC_RZ, CC_PW, CC_RZN, CC_RR, CC_RR0, CC_ALPHA, CC_BETA, CC_DONE, CC_ITS, CC_TOL2, CC_N };
+
+// r = b (in FP64)
+template <class T>
+__global__ void ccg_load_kernel(uint32_t n, const T* __restrict__ b, double* __restrict__ r) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = double(b[i]);
+}
+
+// x = 0, z = invd r, p = z
+static __global__ void ccg_start_kernel(uint32_t n, const double* __restrict__ invd, const double* __restrict__ r,
+                                 double* __restrict__ x, double* __restrict__ z, double* __restrict__ p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double zi = invd[i] * r[i];
+  x[i] = 0.0;
+  z[i] = zi;
+  p[i] = zi;
+}
+
+// after the first r.z and r.r: RZ, RR0, stopping tolerance, DONE for a zero right-hand side
+static __global__ void ccg_init_kernel(double* S, double tol2) {
+  S[CC_RZ] = S[CC_RZN];
+  S[CC_RR0] = S[CC_RR];
+  S[CC_TOL2] = tol2;
+  S[CC_ITS] = 0.0;
+  S[CC_DONE] = S[CC_RR] > 0.0 ? 0.0 : 1.0;
+}
+
+static __global__ void ccg_alpha_kernel(double* S) {
+  if (S[CC_DONE] == 0.0 && !(S[CC_PW] > 0.0)) S[CC_DONE] = 1.0;   // p = 0, or not positive definite
+  S[CC_ALPHA] = S[CC_DONE] == 0.0 ? S[CC_RZ] / S[CC_PW] : 0.0;
+}
+
+// x += alpha p, r -= alpha w, z = invd r
+static __global__ void ccg_update_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ p,
+                                  const double* __restrict__ w, const double* __restrict__ invd,
+                                  double* __restrict__ x, double* __restrict__ r, double* __restrict__ z) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = S[CC_ALPHA];
+  const double ri = r[i] - a * w[i];
+  x[i] = x[i] + a * p[i];
+  r[i] = ri;
+  z[i] = invd[i] * ri;
+}
+
+// beta = (r.z)_new / (r.z), iteration count, stopping test ||r||^2 <= tol^2 ||b||^2
+static __global__ void ccg_beta_kernel(double* S) {
+  if (S[CC_DONE] != 0.0) {
+    S[CC_BETA] = 0.0;
+    return;
+  }
+  S[CC_BETA] = S[CC_RZ] > 0.0 ? S[CC_RZN] / S[CC_RZ] : 0.0;
+  S[CC_RZ] = S[CC_RZN];
+  S[CC_ITS] += 1.0;
+  if (!(S[CC_RR] > S[CC_TOL2] * S[CC_RR0])) S[CC_DONE] = 1.0;
+}
+
+// p = z + beta p
+static __global__ void ccg_p_kernel(uint32_t n, const double* __restrict__ S, const double* __restrict__ z,
+                             double* __restrict__ p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + S[CC_BETA] * p[i];
+}
+
 // ---------------------------------------------------------------- CG with device scalars
 // S: RZ = z.r, PW = p.w, DP = stopping-test norm, RATIO = p-update factor, ZZ, ZR, ZW, RR,
 // BB = b.b, ALPHA.

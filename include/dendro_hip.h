@@ -370,6 +370,12 @@ typedef struct {
   int coarse_op;        /* DGH_COARSE_*: operator of residuals and sweeps below the finest */
                         /* level and of the assembled coarsest matrix (the Jacobi       */
                         /* diagonals stay each level's own)                             */
+  int coarse_dense_max; /* coarsest levels of at most this many nodes (and at most     */
+                        /* 40000) are solved with a dense Cholesky inverse; larger ones */
+                        /* by Jacobi-preconditioned CG on the device (the reference's   */
+                        /* sparse LU, -pc_type lu, is not reproduced)                   */
+  double coarse_rtol;   /* iterative coarse solve: stop at ||b - A x|| <= rtol ||b||     */
+  int coarse_max_it;    /* iterative coarse solve: iteration cap                        */
 } dgh_solver_params;
 
 typedef struct {
@@ -404,6 +410,11 @@ int dgh_solver_solve(dgh_solver s, const double* b_dev, double* x_dev, dgh_solve
  * DAMGSolve (omg.C:1121) on one hierarchy. */
 int dgh_solver_solve_multi(dgh_solver s, int k, const double* B_dev, double* X_dev, dgh_solve_stats* stats,
                            void* stream);
+/* How the solver solves its coarsest level: *iterative = 0 for the dense inverse, 1 for the
+ * Jacobi-preconditioned CG above coarse_dense_max nodes (then *last_iterations = its
+ * iterations in the most recent coarse solve).  Either output may be NULL.  No reference
+ * counterpart (the reference factors the coarsest matrix with PETSc's sparse LU). */
+int dgh_solver_coarse_info(dgh_solver s, int* iterative, int* last_iterations);
 /* z = B r: one application of the V-cycle preconditioner (PCApply_MG), FP64 vectors. */
 int dgh_solver_vcycle(dgh_solver s, const double* r_dev, double* z_dev, void* stream);
 
