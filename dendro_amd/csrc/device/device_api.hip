@@ -72,7 +72,7 @@ static void launch_op_b(dgh_mesh m, const OpIO<T>& io, hipStream_t st) {
   const T* mas = rm<T>(m);
 #define DGH_OP_LAUNCH(MASS, UNI, LANE, COMB)                                                                   \
   hipLaunchKernelGGL((tile_op_kernel<T, T, MASS, MODE, BLOCK, UNI, LANE, COMB>), dim3(grid), dim3(BLOCK), lds, st, \
-                     m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,      \
+                     m->op.tiles.p, nt, m->op.lid.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,      \
                      m->hFac, op_rows<T>(m, COMB), m->op.node_tab.p, m->op.texp.p, io)
   const int rows = op_rows_mode(m);
   if (rows == DGH_ROWS_LANE) {
@@ -261,6 +261,20 @@ void upload_coefs(dgh_mesh m, DevTiles& d) {
   }
 }
 
+// The device form of a plan's records (load_rec): each record's 8 vertex ids with its class
+// (child-0 type | level << 5) in the top 2 bits of ids 0..4, so no class array is read.
+static std::vector<uint16_t> pack_records(const TilePlan& plan) {
+  if (plan.max_touch > REC_MAX_TOUCH)
+    fail(DGH_ERR_ARG, "a tile touches %u nodes (at most %u): use a smaller tile size", plan.max_touch, REC_MAX_TOUCH);
+  std::vector<uint16_t> p(plan.lid);
+  for (size_t r = 0; r < plan.meta.size(); ++r) {
+    const uint32_t c = plan.meta[r];
+    if (c >= 1024u) fail(DGH_ERR_MESH, "record class %u out of range", c);
+    for (int q = 0; q < 5; ++q) p[r * 8 + q] = uint16_t(p[r * 8 + q] | (((c >> (2 * q)) & 3u) << 14));
+  }
+  return p;
+}
+
 // (Re)builds a tiling with the largest tile size <= T whose LDS need (lds(d)) fits; with
 // budget > 0 tiles also end before their owned plus touched nodes exceed the budget, and with
 // block > 0 (single right-hand-side operator) the owned nodes get the fixed capacity
@@ -282,8 +296,8 @@ void build_devtiles(dgh_mesh m, DevTiles& d, uint32_t T, size_t (*lds)(const Dev
   if (lds(d) > size_t(g.max_lds))
     fail(DGH_ERR_ARG, "operator tiles need %zu bytes of LDS (limit %d)", lds(d), g.max_lds);
   d.tiles.upload(d.plan.tiles.data(), d.plan.tiles.size());
-  d.lid.upload(d.plan.lid.data(), d.plan.lid.size());
-  d.meta.upload(d.plan.meta.data(), d.plan.meta.size());
+  const std::vector<uint16_t> packed = pack_records(d.plan);
+  d.lid.upload(packed.data(), packed.size());
   d.node_tab.upload(d.plan.node_tab.data(), d.plan.node_tab.size());
   upload_coefs(m, d);
 }
@@ -511,7 +525,7 @@ static void launch_cg_b(dgh_mesh m, const double* p, double* w, double* part, hi
   const double* mas = m->op.rm64.p;
 #define DGH_CG_LAUNCH(MASS, UNI, LANE, COMB)                                                                    \
   hipLaunchKernelGGL((tile_op_kernel<double, double, MASS, OP_CG, BLOCK, UNI, LANE, COMB>), dim3(grid), dim3(BLOCK), \
-                     lds, st, m->op.tiles.p, nt, m->op.lid.p, m->op.meta.p, kap, mas, m->lap_fac, m->mass_fac,      \
+                     lds, st, m->op.tiles.p, nt, m->op.lid.p, kap, mas, m->lap_fac, m->mass_fac,      \
                      m->maxD, m->hFac, op_rows<double>(m, COMB), m->op.node_tab.p, m->op.texp.p, io)
   const int rows = op_rows_mode(m);
   if (rows == DGH_ROWS_LANE) {

@@ -117,7 +117,7 @@ struct DevTiles {
   uint32_t lds64 = 0, lds32 = 0;  // single right-hand-side kernel: LDS bytes of a launch
   int block = 0;                  // single right-hand-side kernel: block size of the plan
   DevBuf<TileRec> tiles;
-  DevBuf<uint16_t> lid, meta;
+  DevBuf<uint16_t> lid;         // records: 8 vertex ids with the class in spare bits (pack_records)
   DevBuf<uint32_t> node_tab;
   DevBuf<double> rk64, rm64;    // per-record element factors (kappa h / 2, mass h^3 / 8)
   DevBuf<float> rk32, rm32;

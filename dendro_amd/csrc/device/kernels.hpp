@@ -77,6 +77,16 @@ struct Rec {
   double k1, k2;
 };
 
+// A record on the device is 16 bytes: the 8 tile-local vertex ids in 14 bits each, the
+// record's class (child-0 type | level << 5, 10 bits) in the top 2 bits of ids 0..4
+// (pack_records).
+constexpr uint32_t REC_ID_MASK = 0x3fffu;
+constexpr uint32_t REC_MAX_TOUCH = REC_ID_MASK + 1u;
+__device__ __forceinline__ uint32_t rec_class(uint4 lp) {
+  return ((lp.x >> 14) & 3u) | ((lp.x >> 30) << 2) | (((lp.y >> 14) & 3u) << 4) | ((lp.y >> 30) << 6) |
+         (((lp.z >> 14) & 3u) << 8);
+}
+
 // kappa/mass: per-record factors formed by the host, or nullptr for uniform coefficients
 // (lap_fac, mass_fac scaled by the element size here).  LSC scales the vertex ids (the
 // single right-hand-side kernel takes them as LDS byte offsets).  UNI: 1 uniform, 0 per
@@ -84,15 +94,14 @@ struct Rec {
 // FAC false: the factors are not formed (class matrices carry them).
 template <class TC, bool MASS, uint32_t LSC = 1, int UNI = -1, bool FAC = true>
 __device__ __forceinline__ void load_rec(Rec& R, uint32_t g, const uint16_t* __restrict__ lid,
-                                         const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
-                                         const TC* __restrict__ mass, double lap_fac, double mass_fac,
-                                         uint32_t maxD, double hFac) {
+                                         const TC* __restrict__ kappa, const TC* __restrict__ mass,
+                                         double lap_fac, double mass_fac, uint32_t maxD, double hFac) {
   const uint4 lp = reinterpret_cast<const uint4*>(lid)[g];
-  R.l[0] = (lp.x & 0xffffu) * LSC; R.l[1] = (lp.x >> 16) * LSC;
-  R.l[2] = (lp.y & 0xffffu) * LSC; R.l[3] = (lp.y >> 16) * LSC;
-  R.l[4] = (lp.z & 0xffffu) * LSC; R.l[5] = (lp.z >> 16) * LSC;
-  R.l[6] = (lp.w & 0xffffu) * LSC; R.l[7] = (lp.w >> 16) * LSC;
-  R.mt = meta[g];
+  R.l[0] = (lp.x & REC_ID_MASK) * LSC; R.l[1] = ((lp.x >> 16) & REC_ID_MASK) * LSC;
+  R.l[2] = (lp.y & REC_ID_MASK) * LSC; R.l[3] = ((lp.y >> 16) & REC_ID_MASK) * LSC;
+  R.l[4] = (lp.z & REC_ID_MASK) * LSC; R.l[5] = ((lp.z >> 16) & REC_ID_MASK) * LSC;
+  R.l[6] = (lp.w & REC_ID_MASK) * LSC; R.l[7] = ((lp.w >> 16) & REC_ID_MASK) * LSC;
+  R.mt = rec_class(lp);
   if (UNI < 0 ? kappa != nullptr : UNI == 0) {
     R.k1 = double(kappa[g]);
     R.k2 = MASS ? double(mass[g]) : 0.0;
@@ -423,7 +432,7 @@ constexpr int XCHUNK_PREF = 8;
 template <class T, class TC, bool MASS, int MODE, int BLOCK, bool UNI = false, bool LANE = false, bool COMB = false>
 __global__ void __launch_bounds__(BLOCK)
 tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __restrict__ lid,
-               const uint16_t* __restrict__ meta, const TC* __restrict__ kappa,
+               const TC* __restrict__ kappa,
                const TC* __restrict__ mass, double lap_fac, double mass_fac, uint32_t maxD,
                double hFac, const T* __restrict__ KF, const uint32_t* __restrict__ node_tab,
                const int2* __restrict__ texp, OpIO<T> io) {
@@ -445,7 +454,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
   // first record: its loads are issued before the x gather so both chains overlap
   Rec R;
   if (threadIdx.x < rec.n_elem)
-    load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0, !COMB>(R, rec.elem_off + threadIdx.x, lid, meta, kappa, mass, lap_fac,
+    load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0, !COMB>(R, rec.elem_off + threadIdx.x, lid, kappa, mass, lap_fac,
                                                       mass_fac, maxD, hFac);
   const uint32_t kf = tile_kf_off<T>(XS, rec.n_touch);
   if (LANE) {   // the stencil table, copied while the gather runs (visible after the barrier)
@@ -535,7 +544,7 @@ tile_op_kernel(const TileRec* __restrict__ tiles, int ntiles, const uint16_t* __
     if (base >= rec.n_elem) break;
     if (base + threadIdx.x < rec.n_elem) {   // wave-uniform bases, lane offsets fixed
       const uint32_t r0 = rec.elem_off + base;
-      load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0, !COMB>(R, threadIdx.x, lid + size_t(r0) * 8, meta + r0,
+      load_rec<TC, MASS, sizeof(T), UNI ? 1 : 0, !COMB>(R, threadIdx.x, lid + size_t(r0) * 8,
                                                  UNI ? kappa : kappa + r0, UNI || !MASS ? mass : mass + r0, lap_fac,
                                                  mass_fac, maxD, hFac);
     }

@@ -60,7 +60,7 @@ __host__ __device__ constexpr int multi_row(int ksl) { return ksl + 1; }
 template <class T, bool MASS, int MODE, int KSL, int MB, int CPL>
 __global__ void __launch_bounds__(MB)
 tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
-                  const uint16_t* __restrict__ lid, const uint16_t* __restrict__ meta,
+                  const uint16_t* __restrict__ lid,
                   const T* __restrict__ kappa, const T* __restrict__ mass, double lap_fac,
                   double mass_fac, uint32_t maxD, double hFac, const T* __restrict__ K0,
                   const uint32_t* __restrict__ node_tab, const int2* __restrict__ texp, uint32_t ys_len,
@@ -159,7 +159,7 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   Rec Rn;
   uint32_t r0 = uint32_t(wv) * EPW;
   bool hn = r0 + es < rec.n_elem;
-  if (hn) load_rec<T, MASS>(Rn, rec.elem_off + r0 + es, lid, meta, kappa, mass, lap_fac, mass_fac, maxD, hFac);
+  if (hn) load_rec<T, MASS>(Rn, rec.elem_off + r0 + es, lid, kappa, mass, lap_fac, mass_fac, maxD, hFac);
   uint32_t cur = 0xffffffffu;
   T Lr[36], Mr[36];
   for (; r0 < rec.n_elem; r0 += STRIDE) {
@@ -167,7 +167,7 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
     const bool h = hn;
     const uint32_t rn = r0 + STRIDE + es;
     hn = rn < rec.n_elem;
-    if (hn) load_rec<T, MASS>(Rn, rec.elem_off + rn, lid, meta, kappa, mass, lap_fac, mass_fac, maxD, hFac);
+    if (hn) load_rec<T, MASS>(Rn, rec.elem_off + rn, lid, kappa, mass, lap_fac, mass_fac, maxD, hFac);
     const uint32_t tp = R.mt & 31u;
     bool done = !h;
     for (;;) {
@@ -247,7 +247,7 @@ __host__ __device__ constexpr size_t multi_stencil_len() {
 template <bool MASS, int MODE, int MB>
 __global__ void __launch_bounds__(MB)
 tile_multi_mfma_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
-                       const uint16_t* __restrict__ lid, const uint16_t* __restrict__ meta,
+                       const uint16_t* __restrict__ lid,
                        const float* __restrict__ kappa, const float* __restrict__ mass, double lap_fac,
                        double mass_fac, uint32_t maxD, double hFac, const float* __restrict__ K0,
                        const uint32_t* __restrict__ node_tab, uint32_t ys_len, MultiIO<float> io) {
@@ -308,7 +308,7 @@ tile_multi_mfma_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslice
     float a[8], bv[8];
     if (act) {
       Rec R;
-      load_rec<float, MASS>(R, rec.elem_off + r, lid, meta, kappa, mass, lap_fac, mass_fac, maxD, hFac);
+      load_rec<float, MASS>(R, rec.elem_off + r, lid, kappa, mass, lap_fac, mass_fac, maxD, hFac);
       const float f1 = float(R.k1), f2 = float(R.k2);
       const float* Lrow = sK + (R.mt & 31u) * (NM * 64) + (4 * ib + q) * 8;
 #pragma unroll

@@ -62,7 +62,7 @@ static void launch_valu(dgh_mesh m, const DevTiles& d, const T* kap, const T* ma
   const unsigned grid = unsigned(NXCD * ((np + NXCD - 1) / NXCD));
   const size_t lds = multi_lds_valu<T, KSL>(d);
   hipLaunchKernelGGL((tile_multi_kernel<T, MASS, MODE, KSL, MB, CPL>), dim3(grid), dim3(MB), lds, st,
-                     d.tiles.p, nt, io.k / KSL, d.lid.p, d.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
+                     d.tiles.p, nt, io.k / KSL, d.lid.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
                      m->hFac, Ktab<T>(), d.node_tab.p, (const int2*)d.texp.p, d.ys_len, io);
 }
 
@@ -73,7 +73,7 @@ static void launch_mfma(dgh_mesh m, const DevTiles& d, const float* kap, const f
   const int np = nt * (io.k / KS);
   const unsigned grid = unsigned(NXCD * ((np + NXCD - 1) / NXCD));
   hipLaunchKernelGGL((tile_multi_mfma_kernel<MASS, MODE, MBLOCK>), dim3(grid), dim3(MBLOCK), multi_lds_mfma(d, MASS), st,
-                     d.tiles.p, nt, io.k / KS, d.lid.p, d.meta.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
+                     d.tiles.p, nt, io.k / KS, d.lid.p, kap, mas, m->lap_fac, m->mass_fac, m->maxD,
                      m->hFac, Ktab<float>(), d.node_tab.p, d.ys_len, io);
 }
 
