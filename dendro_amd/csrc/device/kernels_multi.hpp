@@ -420,45 +420,80 @@ prolong_multi_kernel(uint32_t nf, int k, const uint32_t* __restrict__ celem, con
 }
 
 // X = Ainv B for the dense coarsest inverse (row pitch ld) and k right-hand sides ([n][k]),
-// split over DM_SPLIT column ranges of Ainv (enough workgroups to stream Ainv at full
-// bandwidth): part[s][i][j] = sum over m in range s of Ainv[i][m] B[m][j] (FP64), then
-// dense_multi_sum_kernel adds the ranges in order.  A workgroup computes 32 rows x k
-// columns of one range; Ainv and B chunks of 64 columns pass through LDS.
-constexpr int DM_ROWS = 32, DM_CH = 64, DM_SPLIT = 4;
+// split over DM_SPLIT column ranges of Ainv (enough workgroups to fill the chip):
+// part[s][i][j] = sum over m in range s of Ainv[i][m] B[m][j] (FP64), then
+// dense_multi_sum_kernel adds the ranges in order.  A workgroup computes DM_ROWS rows x 32
+// columns of one range, a thread a 4 x 4 block of them (rows 4 tr .. 4 tr + 3, columns
+// 4 tc .. 4 tc + 3) from one 16-byte LDS read of each operand per m; Ainv (transposed into
+// LDS, rows padded by 4 values) and B pass through LDS in chunks of DM_CH columns of Ainv,
+// the next chunk in flight in registers meanwhile.
+constexpr int DM_ROWS = 128, DM_CH = 32, DM_SPLIT = 16;
 template <class T>
 __global__ void __launch_bounds__(256)
 dense_multi_kernel(int n, int ld, int k, const T* __restrict__ Ainv, const T* __restrict__ B,
                    double* __restrict__ part) {
-  __shared__ T sa[DM_ROWS][DM_CH + 1];
-  __shared__ T sb[DM_CH][MAXK];
+  static_assert(DM_ROWS == 4 * 32 && MAXK == 4 * 8, "256 threads of 4 x 4 blocks cover DM_ROWS x MAXK");
+  using V = vec16_t<T>;
+  static_assert(sizeof(V) == 4 * sizeof(T), "4-value vectors");
+  __shared__ __align__(16) T sa[DM_CH][DM_ROWS + 4];
+  __shared__ __align__(16) T sb[DM_CH][MAXK];
   const int row0 = blockIdx.x * DM_ROWS;
   const int per = (n + DM_SPLIT - 1) / DM_SPLIT;
   const int mlo = blockIdx.y * per, mhi = min(n, mlo + per);
-  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;   // row, column group (k / 8 columns)
-  const int cpt = k / 8;
-  double acc[MAXK / 8] = {0.0, 0.0, 0.0, 0.0};
-  for (int m0 = mlo; m0 < mhi; m0 += DM_CH) {
-    for (int i = threadIdx.x; i < DM_ROWS * DM_CH; i += 256) {
-      const int r = i / DM_CH, m = i % DM_CH;
-      sa[r][m] = (row0 + r < n && m0 + m < mhi) ? Ainv[size_t(row0 + r) * ld + m0 + m] : T(0);
+  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  // the next chunk's values are loaded into registers while the current chunk is used
+  constexpr int NA = DM_ROWS * DM_CH / 256, NB = DM_CH * MAXK / 256;
+  T ra[NA], rb[NB];
+  auto fetch = [&](int m0) {
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {   // coalesced along the rows of Ainv
+      const int i = threadIdx.x + q * 256, r = i / DM_CH, m = i % DM_CH;
+      ra[q] = (row0 + r < n && m0 + m < mhi) ? Ainv[size_t(row0 + r) * ld + m0 + m] : T(0);
     }
-    for (int i = threadIdx.x; i < DM_CH * k; i += 256) {
-      const int m = i / k, j = i % k;
-      sb[m][j] = (m0 + m < mhi) ? B[size_t(m0 + m) * k + j] : T(0);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int i = threadIdx.x + q * 256, m = i / MAXK, j = i % MAXK;
+      rb[q] = (j < k && m0 + m < mhi) ? B[size_t(m0 + m) * k + j] : T(0);
+    }
+  };
+  if (mlo < mhi) fetch(mlo);
+  for (int m0 = mlo; m0 < mhi; m0 += DM_CH) {
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+      const int i = threadIdx.x + q * 256;
+      sa[i % DM_CH][i / DM_CH] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int i = threadIdx.x + q * 256;
+      sb[i / MAXK][i % MAXK] = rb[q];
     }
     __syncthreads();
-#pragma unroll 8
+    if (m0 + DM_CH < mhi) fetch(m0 + DM_CH);
+#pragma unroll 4
     for (int m = 0; m < DM_CH; ++m) {
-      const double a = double(sa[tr][m]);
+      const V a = *reinterpret_cast<const V*>(&sa[m][4 * tr]);
+      const V b = *reinterpret_cast<const V*>(&sb[m][4 * tc]);
 #pragma unroll
-      for (int u = 0; u < MAXK / 8; ++u)
-        if (u < cpt) acc[u] += a * double(sb[m][tc * cpt + u]);
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] += double(a[u]) * double(b[v]);
     }
     __syncthreads();
   }
-  if (row0 + tr < n)
-    for (int u = 0; u < cpt; ++u)
-      part[(size_t(blockIdx.y) * n + row0 + tr) * k + tc * cpt + u] = acc[u];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int r = row0 + 4 * tr + u;
+    if (r >= n) break;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (4 * tc + v < k) part[(size_t(blockIdx.y) * n + r) * k + 4 * tc + v] = acc[u][v];
+  }
 }
 
 template <class T>
