@@ -93,8 +93,8 @@ static void launch_multi(dgh_mesh m, const MultiIO<T>& io, hipStream_t st) {
   const T* mas = dm<T>(d);
   const bool mass = !(kap && !mas);
   if constexpr (std::is_same<T, double>::value) {
-    if (mass) launch_valu<double, true, MODE, KS64, MBLOCK>(m, d, kap, mas, io, st);
-    else launch_valu<double, false, MODE, KS64, MBLOCK>(m, d, kap, mas, io, st);
+    if (mass) launch_valu<double, true, MODE, KS64, MBLOCK, 2>(m, d, kap, mas, io, st);
+    else launch_valu<double, false, MODE, KS64, MBLOCK, 2>(m, d, kap, mas, io, st);
   } else if (m->mvariant == 1) {
     if (mass) launch_mfma<true, MODE>(m, d, kap, mas, io, st);
     else launch_mfma<false, MODE>(m, d, kap, mas, io, st);
