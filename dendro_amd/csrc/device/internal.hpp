@@ -308,7 +308,7 @@ void ensure_multi(dgh_mesh m);
 void ensure_multi_buffers(dgh_solver s, int k);
 void multi_op_f32(dgh_mesh m, int mode, const float* x, float* y, const float* b, const float* invd, float om,
                   int k, hipStream_t st);
-float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st);
+float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st, bool x1_ready = false);
 void restrict_multi_f32(dgh_transfer t, int k, const float* f, float* c, hipStream_t st);
 void prolong_multi_f32(dgh_transfer t, int k, const float* c, const float* f_in, float* f_out, hipStream_t st);
 void project_mean_multi_f32(dgh_solver s, uint32_t n, int k, float* v, hipStream_t st);

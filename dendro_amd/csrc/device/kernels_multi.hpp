@@ -585,17 +585,41 @@ static __global__ void cg_p_multi_kernel(uint32_t n, int k, const double* __rest
   if (i < size_t(n) * k && S[SM_ACT * MAXK + j] != 0.0) p[i] = z[i] + S[S_RATIO * MAXK + j] * p[i];
 }
 
-static __global__ void cg_xr_multi_kernel(uint32_t n, int k, double* __restrict__ S, const double* __restrict__ p,
-                                   const double* __restrict__ w, double* __restrict__ x, double* __restrict__ r) {
+// x += alpha_j p, r -= alpha_j w for the active columns (alpha_j = RZ / PW, stored as ALPHA),
+// fused with the start of the batched V-cycle on the finest level: the FP32 right-hand side
+// b32 = float(r) and the first sweep from zero x1 = omega (b32 invd), with the arithmetic of
+// the conversion and first-sweep kernels.
+static __global__ void cg_xr_multi_fused_kernel(uint32_t n, int k, double* __restrict__ S,
+                                                const double* __restrict__ p, const double* __restrict__ w,
+                                                double* __restrict__ x, double* __restrict__ r,
+                                                float* __restrict__ b32, const float* __restrict__ invd,
+                                                float omega, float* __restrict__ x1) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int j = int(i % k);
   const bool act = S[SM_ACT * MAXK + j] != 0.0;
   const double a = act ? S[S_RZ * MAXK + j] / S[S_PW * MAXK + j] : 0.0;
   if (i < size_t(k)) S[S_ALPHA * MAXK + j] = a;
-  if (act && i < size_t(n) * k) {
+  if (i >= size_t(n) * k) return;
+  double ri = r[i];
+  if (act) {
     x[i] = x[i] + a * p[i];
-    r[i] = r[i] + (-a) * w[i];
+    ri = ri + (-a) * w[i];
+    r[i] = ri;
   }
+  const float bf = float(ri);
+  b32[i] = bf;
+  x1[i] = 0.f + omega * (bf * invd[i / k]);
+}
+
+// z = double(v) - mean_j (the column means of v, summed in FP64, in sum[j] * n): the
+// conversion of the V-cycle's result and the constant-null-space projection in one pass;
+// sum == nullptr: the conversion alone.
+static __global__ void convert_sub_mean_multi_kernel(uint32_t n, int k, const float* __restrict__ v,
+                                                     const double* __restrict__ sum, double* __restrict__ z) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(n) * k) return;
+  const double zi = double(v[i]);
+  z[i] = sum ? zi - double(sum[i % k] / double(n)) : zi;
 }
 
 static __global__ void cg_scalar_multi_kernel(int k, double* __restrict__ S, int precond_norm, int flexible, double rtol) {

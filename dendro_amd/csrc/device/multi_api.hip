@@ -154,7 +154,8 @@ static void project_mean_multi(dgh_solver s, uint32_t n, int k, T* v, hipStream_
 }
 
 // Batched FP32 V-cycle on level l (the single right-hand-side vcycle, column by column).
-float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st) {
+// x1_ready: the first sweep from zero is already in the level's first iterate buffer.
+float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st, bool x1_ready) {
   auto& V = s->m32[l];
   if (l == 0) {
     const int n = s->nc;
@@ -178,8 +179,9 @@ float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st) {
   const float* invd = s->v32[l].invd.p;
   float* xc = V.xa.p;
   float* xn = V.xb.p;
-  hipLaunchKernelGGL(smooth_first_multi_kernel<float>, dim3(blocks_for(size_t(n) * k)), dim3(VBLOCK), 0, st, n, k,
-                     om, b, invd, xc);
+  if (!x1_ready)
+    hipLaunchKernelGGL(smooth_first_multi_kernel<float>, dim3(blocks_for(size_t(n) * k)), dim3(VBLOCK), 0, st, n, k,
+                       om, b, invd, xc);
   for (int it = 1; it < s->prm.nu_pre; ++it) {
     launch_multi<float, OP_SMOOTH>(m, MultiIO<float>{xc, xn, b, invd, om, k}, st);
     std::swap(xc, xn);
@@ -195,14 +197,26 @@ float* mvcycle(dgh_solver s, int l, int k, const float* b, hipStream_t st) {
   return xc;
 }
 
-static void mprecondition(dgh_solver s, int k, const double* r, double* z, hipStream_t st) {
+// z = B r for k columns.  fused: cg_xr_multi_fused_kernel has already written the FP32
+// right-hand side and the first sweep of the finest level.  The result's conversion to FP64
+// and the constant-null-space projection are one pass (column means from the FP32 result).
+static void mprecondition(dgh_solver s, int k, const double* r, double* z, hipStream_t st, bool fused = false) {
   const int top = s->nlevels - 1;
-  const size_t nk = size_t(s->meshes[top]->N) * k;
+  const uint32_t n = s->meshes[top]->N;
+  const size_t nk = size_t(n) * k;
   auto& V = s->m32[top];
-  hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(nk)), dim3(VBLOCK), 0, st, uint32_t(nk), r, V.b.p);
-  const float* res = mvcycle(s, top, k, V.b.p, st);
-  hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(nk)), dim3(VBLOCK), 0, st, uint32_t(nk), res, z);
-  if (s->prm.null_space) project_mean_multi<double>(s, s->meshes[top]->N, k, z, st);
+  if (!fused)
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(nk)), dim3(VBLOCK), 0, st, uint32_t(nk), r,
+                       V.b.p);
+  const float* res = mvcycle(s, top, k, V.b.p, st, fused && top > 0);
+  const double* sum = nullptr;
+  if (s->prm.null_space) {
+    DotArgs<float> d{};
+    d.a[0] = res;
+    reduce_multi<float>(n, k, d, 1, s->mpart_p.p, s->mS.p + S_PROJ * MAXK, st);
+    sum = s->mS.p + S_PROJ * MAXK;
+  }
+  hipLaunchKernelGGL(convert_sub_mean_multi_kernel, dim3(blocks_for(nk)), dim3(VBLOCK), 0, st, n, k, res, sum, z);
 }
 
 static void mcg_iteration(dgh_solver s, int k, hipStream_t st) {
@@ -219,9 +233,11 @@ static void mcg_iteration(dgh_solver s, int k, hipStream_t st) {
   d.a[0] = s->mp.p;
   d.b[0] = s->mw.p;
   reduce_multi<double>(n, k, d, 1, s->mpart.p, S + S_PW * MAXK, st);
-  hipLaunchKernelGGL(cg_xr_multi_kernel, dim3(blocks_for(nk)), dim3(VBLOCK), 0, st, n, k, S, (const double*)s->mp.p,
-                     (const double*)s->mw.p, s->mx.p, s->mr.p);
-  mprecondition(s, k, s->mr.p, s->mz.p, st);
+  auto& V = s->m32[top];
+  hipLaunchKernelGGL(cg_xr_multi_fused_kernel, dim3(blocks_for(nk)), dim3(VBLOCK), 0, st, n, k, S,
+                     (const double*)s->mp.p, (const double*)s->mw.p, s->mx.p, s->mr.p, V.b.p,
+                     (const float*)s->v32[top].invd.p, float(s->om_level[top]), V.xa.p);
+  mprecondition(s, k, s->mr.p, s->mz.p, st, true);
   DotArgs<double> e{};
   e.a[0] = s->mz.p; e.b[0] = s->mz.p;    // ZZ
   e.a[1] = s->mz.p; e.b[1] = s->mr.p;    // ZR
