@@ -111,6 +111,19 @@ def test_matvec_multi_rejects_bad_k(dev, hier_tiny):
         m.matvec_multi(x, y, 12)
 
 
+def test_matvec_multi_rejects_misaligned(dev, hier_tiny):
+    """The batched kernels move 16-byte vectors of a node's row: X, Y (and B) must be 16-byte
+    aligned, checked on the host before any launch."""
+    m = da.Mesh(hier_tiny.fine)
+    n, k = hier_tiny.fine.n_node, 8
+    base = torch.zeros(n * k + 4, dtype=torch.float32, device=dev)
+    y = torch.empty(n * k, dtype=torch.float32, device=dev)
+    with pytest.raises(da.DendroError):
+        m.matvec_multi(base[1:1 + n * k], y, k, prec=da.F32)   # 4 bytes past an aligned start
+    m.matvec_multi(base[4:4 + n * k], y, k, prec=da.F32)       # 16 bytes past: accepted
+    torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize("graph", [0, 1])
 def test_solve_multi_matches_single(dev, hier_small, graph):
     h = hier_small
