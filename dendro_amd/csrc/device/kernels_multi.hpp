@@ -3,11 +3,12 @@
 // k right-hand sides are stored node-major: X[n * k + j], so the k values of a node are one
 // contiguous row and every node gather is a coalesced row segment.  The operator kernel
 // keeps the owner-computes tiling of the single right-hand-side kernel; one workgroup
-// handles one tile and one slice of KS right-hand sides, staging the slice of every touched
-// node in LDS.  In FP32 the element products Y_e = K_e X_e ([8x8] x [8xKS]) run on the
-// matrix cores as v_mfma_f32_4x4x1_16b_f32 (16 independent 4x4 blocks per instruction:
-// 4 elements x 2 row blocks x 2 column blocks); FP64 uses the vector ALUs with the same
-// staging.  Row sums of owned nodes accumulate in LDS and are stored once per node.
+// handles one tile and one slice of right-hand sides, staging the slice of every touched
+// node in LDS.  The element products Y_e = K_e X_e ([8x8] x [8 x slice]) run on the vector
+// ALUs (tile_multi_kernel, FP32 and FP64) or, as an FP32 variant, on the matrix cores as
+// v_mfma_f32_4x4x1_16b_f32 (tile_multi_mfma_kernel: 16 independent 4x4 blocks per
+// instruction, 4 elements x 2 row blocks x 2 column blocks); the autotuner picks the FP32
+// variant per mesh.  Row sums of owned nodes accumulate in LDS and are stored once per node.
 #pragma once
 
 #include <hip/hip_runtime.h>
