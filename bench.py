@@ -494,6 +494,9 @@ def main():
     ap.add_argument("--multi-tile", type=int, default=0,
                     help="with --multi-only: batched tile size (0: autotune), with --multi-variant")
     ap.add_argument("--multi-variant", type=int, default=4, help="batched FP32 product variant with --multi-tile")
+    ap.add_argument("--multi-trace", action="store_true",
+                    help="only three eager (no hipGraph) 32 right-hand-side PCG solves on the 1M log-normal "
+                         "octree, for a kernel trace")
     ap.add_argument("--tile", type=int, default=0, help="operator tile size (0: autotune)")
     ap.add_argument("--block", type=int, default=0, help="workgroup size with --tile (0: library default)")
     ap.add_argument("--autotune-all", action="store_true",
@@ -521,6 +524,19 @@ def main():
     dev = torch.device("cuda:0")
     da.init(0)
     stream = torch.cuda.current_stream()
+    if args.multi_trace:   # batched solves on the 1M log-normal octree, graph capture off
+        Q = Problem(WORKLOADS["c2"])
+        tuned = [m.autotune_multi(32) for m in Q.meshes[1:]]
+        log("batched tilings: " + ", ".join(f"{t['tile_elems']}/v{t['variant']}" for t in tuned))
+        k, N = 32, Q.fine.n_node
+        B = torch.tensor(np.stack([np.random.default_rng(400 + j).uniform(-1, 1, N) for j in range(k)], 1),
+                         dtype=torch.float64, device=dev)
+        X = torch.zeros_like(B)
+        s = da.Solver(Q.meshes, Q.trans, Q.params(vcycle_prec=da.F32, use_graph=0))
+        for _ in range(3):
+            st = s.solve_multi(B, X, k)
+            log(f"eager batched PCG: {st[0].iterations} its, {st[0].seconds * 1e3:.2f} ms")
+        return
     if args.multi_only:   # the batched operator runs on the 1M log-normal octree whatever --workload
         Q = Problem(WORKLOADS["c2"])
         r = bench_multi(Q, dev, stream, args.steps, solve=False, tile=args.multi_tile, variant=args.multi_variant)
