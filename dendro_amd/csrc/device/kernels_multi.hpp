@@ -190,6 +190,19 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
       }
     }
   }
+  // the store's right-hand sides and diagonals of the kept owned nodes: loaded before the
+  // barrier, so they are in flight while the other wavefronts finish their records
+  constexpr int MSU = MGU;
+  const size_t so = size_t(sl) * KSL + cv;
+  V bpre[MSU];
+  T dpre[MSU];
+  if (MODE != OP_MATVEC && keep) {
+#pragma unroll
+    for (int u = 0; u < MSU; ++u) {
+      bpre[u] = *reinterpret_cast<const V*>(io.b + size_t(own_n[u]) * k + so);
+      dpre[u] = MODE == OP_SMOOTH ? io.invd[own_n[u]] : T(0);
+    }
+  }
   __syncthreads();
   // scales of the thread's store columns cv + e
   int ess[VEC];
@@ -204,8 +217,6 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   }
   // stores: 16-byte vectors as in the gather, MSU (= MGU, the batch of own_n) per thread
   // with their loads in flight together
-  constexpr int MSU = MGU;
-  const size_t so = size_t(sl) * KSL + cv;
   for (uint32_t i0 = threadIdx.x; i0 < nov; i0 += MSU * MB) {
     uint32_t nid[MSU];
     V bv[MSU];
@@ -214,8 +225,13 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
     for (int u = 0; u < MSU; ++u) nid[u] = keep ? own_n[u] : nodes[min(i0 + u * MB, nov - 1) / LPN];
 #pragma unroll
     for (int u = 0; u < MSU; ++u) {
-      if (MODE != OP_MATVEC) bv[u] = *reinterpret_cast<const V*>(io.b + size_t(nid[u]) * k + so);
-      dv[u] = MODE == OP_SMOOTH ? io.invd[nid[u]] : T(0);
+      if (keep) {
+        bv[u] = bpre[u];
+        dv[u] = dpre[u];
+      } else {
+        if (MODE != OP_MATVEC) bv[u] = *reinterpret_cast<const V*>(io.b + size_t(nid[u]) * k + so);
+        dv[u] = MODE == OP_SMOOTH ? io.invd[nid[u]] : T(0);
+      }
     }
 #pragma unroll
     for (int u = 0; u < MSU; ++u) {
