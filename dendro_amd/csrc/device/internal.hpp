@@ -268,6 +268,7 @@ struct dgh_solver_s {
   DevBuf<double> mx, mr, mz, mp, mw;   // outer CG vectors [N][mk]
   DevBuf<double> mS;                   // per-column CG scalars
   DevBuf<double> mpart, mpart_p;       // column reduction partials
+  DevBuf<double> mptile;               // CG mode of the FP64 batched product: [tiles][k] partials of p.w
   DevBuf<double> mdense;               // split partial sums of the batched coarse solve
   double* hmS = nullptr;
   hipGraph_t mgraph = nullptr;

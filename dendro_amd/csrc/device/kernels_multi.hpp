@@ -34,6 +34,7 @@ struct MultiIO {
   const T* invd;   // [N] inverse diagonal (SMOOTH)
   T omega;         // Richardson scale (SMOOTH)
   int k;           // right-hand sides per node row
+  double* part;    // CG: per tile and column, the owned nodes' partial of x.y ([ntiles][k])
 };
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
@@ -196,7 +197,7 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   const size_t so = size_t(sl) * KSL + cv;
   V bpre[MSU];
   T dpre[MSU];
-  if (MODE != OP_MATVEC && keep) {
+  if ((MODE == OP_RESID || MODE == OP_SMOOTH) && keep) {
 #pragma unroll
     for (int u = 0; u < MSU; ++u) {
       bpre[u] = *reinterpret_cast<const V*>(io.b + size_t(own_n[u]) * k + so);
@@ -217,6 +218,9 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
   }
   // stores: 16-byte vectors as in the gather, MSU (= MGU, the batch of own_n) per thread
   // with their loads in flight together
+  double dot[VEC];   // CG: this thread's partials of x.y for its columns
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) dot[e] = 0.0;
   for (uint32_t i0 = threadIdx.x; i0 < nov; i0 += MSU * MB) {
     uint32_t nid[MSU];
     V bv[MSU];
@@ -229,7 +233,8 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
         bv[u] = bpre[u];
         dv[u] = dpre[u];
       } else {
-        if (MODE != OP_MATVEC) bv[u] = *reinterpret_cast<const V*>(io.b + size_t(nid[u]) * k + so);
+        if (MODE == OP_RESID || MODE == OP_SMOOTH)
+          bv[u] = *reinterpret_cast<const V*>(io.b + size_t(nid[u]) * k + so);
         dv[u] = MODE == OP_SMOOTH ? io.invd[nid[u]] : T(0);
       }
     }
@@ -242,13 +247,48 @@ tile_multi_kernel(const TileRec* __restrict__ tiles, int ntiles, int nslices,
 #pragma unroll
       for (int e = 0; e < VEC; ++e) {
         const T v = fins[e] ? T(fx_value<T>(ys[nd * RS + cv + e], ess[e])) : T(__builtin_nan(""));
-        if (MODE == OP_MATVEC) out[e] = v;
+        if (MODE == OP_MATVEC || MODE == OP_CG) out[e] = v;
         else if (MODE == OP_RESID) out[e] = bv[u][e] - v;
         else out[e] = xs[nd * RS + cv + e] + io.omega * ((bv[u][e] - v) * dv[u]);
+        if (MODE == OP_CG) dot[e] += double(xs[nd * RS + cv + e]) * double(v);
       }
       *reinterpret_cast<V*>(io.y + size_t(nid[u]) * k + so) = out;
     }
   }
+  if constexpr (MODE == OP_CG) {
+    // the tile's partial of x.y per column, summed in a fixed order: over the lanes that
+    // share the thread's columns, then over the wavefronts (in LDS over the dead sums)
+#pragma unroll
+    for (int off = 32; off >= LPN; off >>= 1)
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) dot[e] += __shfl_xor(dot[e], off);
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(smem_raw);   // [NW][KSL]
+    if (lane < LPN)
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) red[wv * KSL + cv + e] = dot[e];
+    __syncthreads();
+    if (threadIdx.x < KSL) {
+      double sum = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) sum += red[w * KSL + threadIdx.x];
+      io.part[size_t(t) * k + size_t(sl) * KSL + threadIdx.x] = sum;
+    }
+  }
+}
+
+// out[j] = sum over tiles of part[t][j] (the CG mode's tile partials), one workgroup per
+// column, in a fixed order
+static __global__ void __launch_bounds__(256)
+sum_tiles_multi_kernel(int ntiles, int k, const double* __restrict__ part, double* __restrict__ out) {
+  const int j = blockIdx.x;
+  double v = 0.0;
+  for (int t = threadIdx.x; t < ntiles; t += 256) v += part[size_t(t) * k + j];
+  v = wave_sum(v);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[j] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // Matrix-core variant (FP32): lane = 4 * block + q, block = 4 * es + 2 * ib + jb: element

@@ -228,11 +228,11 @@ static void mcg_iteration(dgh_solver s, int k, hipStream_t st) {
   double* S = s->mS.p;
   hipLaunchKernelGGL(cg_p_multi_kernel, dim3(blocks_for(nk)), dim3(VBLOCK), 0, st, n, k, (const double*)S,
                      (const double*)s->mz.p, s->mp.p);
-  launch_multi<double, OP_MATVEC>(m, MultiIO<double>{s->mp.p, s->mw.p, nullptr, nullptr, 0.0, k}, st);
-  DotArgs<double> d{};
-  d.a[0] = s->mp.p;
-  d.b[0] = s->mw.p;
-  reduce_multi<double>(n, k, d, 1, s->mpart.p, S + S_PW * MAXK, st);
+  // w = A p with the tiles' partials of p.w, summed per column in a fixed order
+  launch_multi<double, OP_CG>(m, MultiIO<double>{s->mp.p, s->mw.p, nullptr, nullptr, 0.0, k, s->mptile.p}, st);
+  const int nt = int(m->mop.plan.tiles.size());
+  hipLaunchKernelGGL(sum_tiles_multi_kernel, dim3(k), dim3(256), 0, st, nt, k, (const double*)s->mptile.p,
+                     S + S_PW * MAXK);
   auto& V = s->m32[top];
   hipLaunchKernelGGL(cg_xr_multi_fused_kernel, dim3(blocks_for(nk)), dim3(VBLOCK), 0, st, n, k, S,
                      (const double*)s->mp.p, (const double*)s->mw.p, s->mx.p, s->mr.p, V.b.p,
@@ -430,10 +430,18 @@ int dgh_solver_solve_multi(dgh_solver s, int k, const double* B, double* X, dgh_
     check_k(k);
     check_aligned(B, X);
     ensure_multi_buffers(s, k);
+    ensure_multi(s->meshes[s->nlevels - 1]);
     const uint64_t gen = mesh_generation(s);
     if (gen != s->graph_gen) {   // re-tiled or new coefficients since the graphs were captured
       drop_graphs(s);
       s->graph_gen = gen;
+    }
+    // the CG mode's tile partials (sized for the finest level's batched tiling; a re-tiling
+    // bumps the mesh generation, which drops the captured graphs above)
+    const size_t nptile = s->meshes[s->nlevels - 1]->mop.plan.tiles.size() * size_t(k);
+    if (s->mptile.n < nptile) {
+      drop_graphs(s);
+      s->mptile.alloc(nptile);
     }
     hipStream_t ust = static_cast<hipStream_t>(stream);
     hipStream_t st = s->st;
