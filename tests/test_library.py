@@ -69,3 +69,19 @@ def test_noalloc_limit_matches_header():
     txt = open(os.path.join(ROOT, "include", "dendro_hip.h")).read()
     m = re.search(r"#define DGH_NOALLOC_MAX_TILES (\d+)", txt)
     assert m and int(m.group(1)) == core.NOALLOC_MAX_TILES
+
+
+def test_solver_default_params():
+    """dgh_solver_default_params (no GPU): the reference's bin/options values, the dense coarse
+    limit and the iterative coarse solve's defaults, through the Python mirror of the struct
+    (a layout mismatch between the header and core.SolverParams shows up here)."""
+    p = da.default_params()
+    assert p.rtol == 1e-8 and p.omega == 0.667 and p.max_it == 10
+    assert p.nu_pre == 4 and p.nu_post == 4
+    assert p.norm_type == da.NORM_PRECONDITIONED and p.null_space == 0
+    assert p.vcycle_prec == da.F64 and p.use_graph == 0 and p.coarse_op == da.COARSE_JAC2
+    assert p.coarse_dense_max == 40000 and p.coarse_rtol == 1e-12 and p.coarse_max_it == 5000
+    txt = open(os.path.join(ROOT, "include", "dendro_hip.h")).read()
+    body = txt[txt.index("typedef struct {\n  double rtol;"):txt.index("} dgh_solver_params;")]
+    fields = re.findall(r"^\s+(?:double|int) (\w+);", body, re.M)
+    assert fields == [f[0] for f in core.SolverParams._fields_]
