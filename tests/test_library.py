@@ -85,3 +85,38 @@ def test_solver_default_params():
     body = txt[txt.index("typedef struct {\n  double rtol;"):txt.index("} dgh_solver_params;")]
     fields = re.findall(r"^\s+(?:double|int) (\w+);", body, re.M)
     assert fields == [f[0] for f in core.SolverParams._fields_]
+
+
+STRUCTS = {"dgh_level_info": "LevelInfo", "dgh_function": "Function", "dgh_solver_params": "SolverParams",
+           "dgh_solve_stats": "SolveStats", "dgh_learn_params": "LearnParams", "dgh_learn_stats": "LearnStats"}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every struct of include/dendro_hip.h that core.py mirrors has the same size and field
+    offsets in C (gcc on the header) as in ctypes."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dendro_hip.h"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(core, pyname)
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for l in out:
+        if l:
+            a, b, c = l.split()
+            got[(a, b)] = int(c)
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(core, pyname)
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for f in cls._fields_:
+            assert got[(cname, f[0])] == getattr(cls, f[0]).offset, (cname, f[0])
