@@ -63,3 +63,18 @@ def test_elasticity_kernels_have_no_static_lds_and_do_not_spill(asm_elas):
     for n, b in ks:
         assert int(re.search(r"\.amdhsa_group_segment_fixed_size (\d+)", b).group(1)) == 0, n
         assert int(re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", b).group(1)) == 0, n
+
+
+def test_1024_thread_scalar_row_kernels_fit_two_workgroups_per_cu(asm):
+    """The large-mesh tiling runs 1024-thread workgroups, two per CU (32 wavefronts, 8 per
+    SIMD), which holds only while a kernel needs at most 64 VGPRs; above that a CU runs one
+    workgroup and the operator loses about half its latency hiding (a held-value variant of
+    the FP64 sweeps measured 17% slower PCG).  Guarded for the product, CG, residual,
+    smoothing and diagonal modes with stencil rows through the scalar cache (per type or per
+    class): tile_op_kernel<T, TC, MASS, MODE, 1024, UNI, LANE = false, COMB>."""
+    ks = [(n, b) for n, b in _kernels(asm, "tile_op_kernel")
+          if re.search(r"ELi[0-4]ELi1024ELb[01]ELb0ELb[01]E", n)]
+    assert len(ks) >= 30
+    over = [(n, int(re.search(r"\.amdhsa_next_free_vgpr (\d+)", b).group(1))) for n, b in ks]
+    over = [(n, v) for n, v in over if v > 64]
+    assert not over, over[:3]
