@@ -1,7 +1,8 @@
 """The owner-computes tile plan (host/tiles.cpp) against the contract the operator kernels'
 fixed LDS layout relies on (owned-node capacity, node budget, ownership, halo completeness,
-vertex ids), checked on the CPU by tests/tile_plan_check.cpp for the budgets and capacities
-of the 256-, 512- and 1024-thread kernels."""
+vertex ids, ids and class within the 16-byte device record), checked on the CPU by
+tests/tile_plan_check.cpp for the budgets and capacities of the 256-, 512- and 1024-thread
+kernels (8192-element tiles included) and of the batched kernels."""
 import os
 import subprocess
 

@@ -5,7 +5,9 @@
 //   - every node is owned by exactly one tile, listed first in its node table;
 //   - a tile's records are every element touching one of its owned nodes (own or halo);
 //   - record vertex ids index the tile's node table, in the vertex order k ^ child;
-//   - records are sorted by stencil class.
+//   - records are sorted by stencil class;
+//   - the device record form fits: vertex ids in 14 bits (a tile touches at most 16384 nodes)
+//     and the class (child-0 type | level << 5) in the 10 bits the ids leave free.
 // Usage: tile_plan_check npts depth   (prints one line per plan; exit status 1 on a failure)
 #include <cstdio>
 #include <cstdlib>
@@ -45,7 +47,7 @@ int main(int argc, char** argv) {
       if (v.empty() || v.back() != e) v.push_back(e);
     }
   struct Case { uint32_t T, budget, cap; };
-  const Case cases[] = {{4096, 10176, 4096}, {1024, 10176, 4096}, {2048, 5056, 2048},
+  const Case cases[] = {{8192, 10176, 4096}, {4096, 10176, 4096}, {1024, 10176, 4096}, {2048, 5056, 2048},
                         {512, 2496, 1024}, {128, 2496, 1024}, {2048, 10176, 0}};
   for (const Case& c : cases) {
     TilePlan p;
@@ -76,9 +78,11 @@ int main(int argc, char** argv) {
         CHECK(type >= prev_type, "tile %zu: records not sorted by class", t);
         prev_type = type;
         CHECK((p.meta[g] >> 5) == da.lev[e], "record %zu: level", g);
+        CHECK(p.meta[g] < 1024u, "record %zu: class %u beyond 10 bits", g, unsigned(p.meta[g]));
         for (int k = 0; k < 8; ++k) {
           const uint32_t l = p.lid[g * 8 + k];
           CHECK(l < tr.n_touch, "record %zu: vertex id %u beyond %u touched nodes", g, l, tr.n_touch);
+          CHECK(l < 16384u, "record %zu: vertex id %u beyond 14 bits", g, l);
           if (l < tr.n_touch)
             CHECK(nodes[l] == da.nlist[size_t(e) * 8 + (k ^ da.child[e])], "record %zu vertex %d: node mismatch", g, k);
         }
@@ -90,6 +94,7 @@ int main(int argc, char** argv) {
     for (uint32_t v = 0; v < N; ++v) CHECK(owner_count[v] == 1, "node %u owned %d times", v, owner_count[v]);
     for (uint32_t e = 0; e < E; ++e) CHECK(elem_seen[e], "element %u has no record", e);
     CHECK(R == p.rec_elem.size(), "record count");
+    CHECK(p.max_touch <= 16384u, "max_touch %u beyond the 14-bit vertex ids", p.max_touch);
     std::printf("T=%u budget=%u cap=%u: E=%u N=%u tiles=%zu records=%zu max_own=%u max_touch=%u\n", c.T, c.budget,
                 c.cap, E, N, p.tiles.size(), R, p.max_own, p.max_touch);
   }
